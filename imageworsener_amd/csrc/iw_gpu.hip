@@ -1,0 +1,852 @@
+// GPU executor of the ImageWorsener pixel pipeline for gfx950 (MI355X).
+//
+// One iw_process_image() is three kernels on one HIP stream:
+//   vpass  -- colour-in + premultiply/early background + vertical resample,
+//             rounded to FP32 into the intermediate image
+//             (src/imagew-main.c:831-927)
+//   hpass  -- horizontal resample of every channel of one output pixel
+//             (alpha first), unpremultiply, late background, quantise and
+//             ordered/random dither, store (src/imagew-main.c:929-1155,
+//             669-765); error-diffusion channels stop after the clamp and
+//             hand their linear samples to
+//   errdiff -- one serial walk per error-diffused plane in the reference's
+//             serpentine order (src/imagew-main.c:497-567, 1072-1136).
+//
+// Byte parity with the reference: every product and sum is a separately
+// rounded FP64 operation in the reference's order (contraction is off for
+// this file and the compile line), no libm is called on the device (all
+// transfer functions are host tables), and the intermediate and final alpha
+// are rounded to FP32 where the reference stores them as float.
+#include <hip/hip_runtime.h>
+
+#include "iw_ctx.h"
+#include "iw_tables.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace {
+
+#define HIP_OK(expr) \
+	do { \
+		hipError_t e_ = (expr); \
+		if (e_ != hipSuccess) { \
+			fail_msg = hipGetErrorString(e_); \
+			return false; \
+		} \
+	} while (0)
+
+// ---- device-side plan ----------------------------------------------------
+
+struct DevWeights {
+	int mode;            // WeightList::WEIGHTS / COPY
+	const int *start;
+	const int *src;
+	const double *w;
+	const int *copy_src;
+	double edge;
+};
+
+struct DevIChan {
+	int input_channel, cvt_gray, output_channel, need_unassoc, early_bkgd;
+	double bkgd;
+	const double *lut[3];
+	DevWeights v, h;
+};
+
+enum { Q_THRESH = 0, Q_NC = 1 };
+
+struct DevOChan {
+	int family, subtype, qmode, P, is_alpha, ed_index;
+	const double *T, *U, *lin, *code, *nc;
+	double bk1, bk2;
+	const uint64_t *prng_row, *prng_col;
+};
+
+struct DevPlan {
+	const uint8_t *in;
+	size_t in_bpr;
+	int in_w, in_h, bit_depth, in_float, nch_phys;
+	unsigned int orient;
+	int sx, sy, iw, ih;
+	int alpha_in_index;
+	const double *alpha_lut;
+	int gsf;
+	double gsw[3];
+	int inter_nch, inter_alpha;
+	int order[4];
+	int W, Hp;
+	DevIChan ic[4];
+	float *inter;
+	int intclamp;
+	int out_w, out_h, out_depth, out_float, out_nch;
+	size_t out_bpr;
+	uint8_t *out;
+	DevOChan oc[4];
+	double *ed_samples;
+	double *ed_err;
+	int n_ed;
+	int apply_bkgd, late, checker, check_size, ox, oy, bk_has_trans;
+	double bk1a, bk2a;
+};
+
+// ---- input sampling (src/imagew-main.c:129-276) ---------------------------
+
+__device__ __forceinline__ void logical_to_physical(const DevPlan &P, int x, int y, int &rx, int &ry)
+{
+	const int X = P.sx + x, Y = P.sy + y;
+	switch (P.orient) {
+	case 0: rx = X; ry = Y; break;
+	case 1: rx = P.in_w - 1 - X; ry = Y; break;
+	case 2: rx = X; ry = P.in_h - 1 - Y; break;
+	case 3: rx = P.in_w - 1 - X; ry = P.in_h - 1 - Y; break;
+	case 4: rx = Y; ry = X; break;
+	case 5: rx = Y; ry = P.in_w - 1 - X; break;
+	case 6: rx = P.in_h - 1 - Y; ry = X; break;
+	case 7: rx = P.in_h - 1 - Y; ry = P.in_w - 1 - X; break;
+	default: rx = 0; ry = 0; break;
+	}
+}
+
+__device__ __forceinline__ unsigned int raw_int(const DevPlan &P, int rx, int ry, int ch)
+{
+	const uint8_t *row = P.in + (size_t)ry * P.in_bpr;
+	switch (P.bit_depth) {
+	case 8: return row[P.nch_phys * rx + ch];
+	case 16: {
+		const uint8_t *q = row + (size_t)(P.nch_phys * rx + ch) * 2;
+		return ((unsigned int)q[0] << 8) | q[1];
+	}
+	case 4: {
+		unsigned int b = row[rx / 2];
+		return (rx & 1) ? (b & 0x0f) : (b >> 4);
+	}
+	case 2: return (row[rx / 4] >> ((3 - rx % 4) * 2)) & 0x03;
+	case 1: return (row[rx / 8] >> (7 - rx % 8)) & 0x01;
+	}
+	return 0;
+}
+
+__device__ __forceinline__ double raw_float(const DevPlan &P, int rx, int ry, int ch)
+{
+	float f;
+	memcpy(&f, P.in + (size_t)ry * P.in_bpr + (size_t)(P.nch_phys * rx + ch) * 4, 4);
+	return (double)f;
+}
+
+// Sample k (0..2) of intermediate channel ic, in linear light.
+__device__ __forceinline__ double sample_lin(const DevPlan &P, const DevIChan &ic, int k, int rx, int ry)
+{
+	const int ch = ic.input_channel + k;
+	if (ch >= P.nch_phys) return 1.0;  // virtual alpha: opaque
+	if (P.in_float) return raw_float(P, rx, ry, ch);
+	return ic.lut[k][raw_int(P, rx, ry, ch)];
+}
+
+__device__ __forceinline__ double alpha_raw(const DevPlan &P, int rx, int ry)
+{
+	if (P.alpha_in_index >= P.nch_phys) return 1.0;
+	if (P.in_float) return raw_float(P, rx, ry, P.alpha_in_index);
+	return P.alpha_lut[raw_int(P, rx, ry, P.alpha_in_index)];
+}
+
+// ((w0*r + w1*g) + w2*b), optionally on sorted components
+// (src/imagew-main.c:278-305).
+__device__ __forceinline__ double to_gray(const DevPlan &P, double r, double g, double b)
+{
+	if (P.gsf == IW_GSF_WEIGHTED) return P.gsw[0] * r + P.gsw[1] * g + P.gsw[2] * b;
+	double hi, mid, lo;
+	if (g <= r) { hi = r; mid = g; } else { hi = g; mid = r; }
+	if (b <= mid) {
+		lo = b;
+	} else {
+		lo = mid;
+		if (b <= hi) mid = b;
+		else { mid = hi; hi = b; }
+	}
+	return P.gsw[0] * hi + P.gsw[1] * mid + P.gsw[2] * lo;
+}
+
+__device__ __forceinline__ double colour_in(const DevPlan &P, const DevIChan &ic, int x, int y)
+{
+	int rx, ry;
+	logical_to_physical(P, x, y, rx, ry);
+	double v;
+	if (ic.cvt_gray)
+		v = to_gray(P, sample_lin(P, ic, 0, rx, ry), sample_lin(P, ic, 1, rx, ry), sample_lin(P, ic, 2, rx, ry));
+	else
+		v = sample_lin(P, ic, 0, rx, ry);
+	if (ic.need_unassoc) {
+		v *= alpha_raw(P, rx, ry);
+	} else if (ic.early_bkgd) {
+		const double a = alpha_raw(P, rx, ry);
+		v = a * v + (1.0 - a) * ic.bkgd;
+	}
+	return v;
+}
+
+__device__ __forceinline__ double clamp01(double v)
+{
+	if (v < 0.0) return 0.0;
+	if (v > 1.0) return 1.0;
+	return v;
+}
+
+// ---- vertical pass --------------------------------------------------------
+
+__global__ void __launch_bounds__(256) vpass_kernel(DevPlan P)
+{
+	const int x = blockIdx.x * blockDim.x + threadIdx.x;
+	const int j = blockIdx.y;
+	if (x >= P.W) return;
+	for (int c = 0; c < P.inter_nch; c++) {
+		const DevIChan &ic = P.ic[c];
+		double acc;
+		if (ic.v.mode == iwx::WeightList::COPY) {
+			const int s = ic.v.copy_src[j];
+			acc = (s < 0) ? 0.0 : colour_in(P, ic, x, s);
+		} else {
+			acc = 0.0;
+			const int t1 = ic.v.start[j + 1];
+			for (int t = ic.v.start[j]; t < t1; t++) {
+				const int s = ic.v.src[t];
+				const double v = (s < 0) ? ic.v.edge : colour_in(P, ic, x, s);
+				acc = acc + v * ic.v.w[t];
+			}
+		}
+		if (P.intclamp) acc = clamp01(acc);
+		P.inter[((size_t)c * P.Hp + j) * P.W + x] = (float)acc;
+	}
+}
+
+// ---- quantisation and dithering (src/imagew-main.c:457-765) --------------
+
+// Ordered-dither thresholds (k+0.5)/64 in the reference's two 8x8
+// arrangements (src/imagew-main.c:460-480), stored as float like the
+// reference's table.
+__constant__ float c_dither_pattern[2][64] = {
+	{0.5f/64, 48.5f/64, 12.5f/64, 60.5f/64, 3.5f/64, 51.5f/64, 15.5f/64, 63.5f/64,
+	 32.5f/64, 16.5f/64, 44.5f/64, 28.5f/64, 35.5f/64, 19.5f/64, 47.5f/64, 31.5f/64,
+	 8.5f/64, 56.5f/64, 4.5f/64, 52.5f/64, 11.5f/64, 59.5f/64, 7.5f/64, 55.5f/64,
+	 40.5f/64, 24.5f/64, 36.5f/64, 20.5f/64, 43.5f/64, 27.5f/64, 39.5f/64, 23.5f/64,
+	 2.5f/64, 50.5f/64, 14.5f/64, 62.5f/64, 1.5f/64, 49.5f/64, 13.5f/64, 61.5f/64,
+	 34.5f/64, 18.5f/64, 46.5f/64, 30.5f/64, 33.5f/64, 17.5f/64, 45.5f/64, 29.5f/64,
+	 10.5f/64, 58.5f/64, 6.5f/64, 54.5f/64, 9.5f/64, 57.5f/64, 5.5f/64, 53.5f/64,
+	 42.5f/64, 26.5f/64, 38.5f/64, 22.5f/64, 41.5f/64, 25.5f/64, 37.5f/64, 21.5f/64},
+	{3.5f/64, 9.5f/64, 17.5f/64, 27.5f/64, 25.5f/64, 15.5f/64, 7.5f/64, 1.5f/64,
+	 11.5f/64, 29.5f/64, 37.5f/64, 45.5f/64, 43.5f/64, 35.5f/64, 23.5f/64, 5.5f/64,
+	 19.5f/64, 39.5f/64, 51.5f/64, 57.5f/64, 55.5f/64, 49.5f/64, 33.5f/64, 13.5f/64,
+	 31.5f/64, 47.5f/64, 59.5f/64, 63.5f/64, 61.5f/64, 53.5f/64, 41.5f/64, 21.5f/64,
+	 30.5f/64, 46.5f/64, 58.5f/64, 62.5f/64, 60.5f/64, 52.5f/64, 40.5f/64, 20.5f/64,
+	 18.5f/64, 38.5f/64, 50.5f/64, 56.5f/64, 54.5f/64, 48.5f/64, 32.5f/64, 12.5f/64,
+	 10.5f/64, 28.5f/64, 36.5f/64, 44.5f/64, 42.5f/64, 34.5f/64, 22.5f/64, 4.5f/64,
+	 2.5f/64, 8.5f/64, 16.5f/64, 26.5f/64, 24.5f/64, 14.5f/64, 6.5f/64, 0.5f/64}};
+
+// Error-diffusion weights; slots 0,1: x+1,x+2 of this row; 2..6: x-2..x+2 of
+// the next row; 7..11: the row after (src/imagew-main.c:510-535).
+__constant__ double c_ed_matrix[8][12] = {
+	{7.0/16, 0.0, 0.0, 3.0/16, 5.0/16, 1.0/16, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},          // Floyd-Steinberg
+	{7.0/48, 5.0/48, 3.0/48, 5.0/48, 7.0/48, 5.0/48, 3.0/48, 1.0/48, 3.0/48, 5.0/48, 3.0/48, 1.0/48},  // JJN
+	{8.0/42, 4.0/42, 2.0/42, 4.0/42, 8.0/42, 4.0/42, 2.0/42, 1.0/42, 2.0/42, 4.0/42, 2.0/42, 1.0/42},  // Stucki
+	{8.0/32, 4.0/32, 2.0/32, 4.0/32, 8.0/32, 4.0/32, 2.0/32, 0.0, 0.0, 0.0, 0.0, 0.0},  // Burkes
+	{5.0/32, 3.0/32, 2.0/32, 4.0/32, 5.0/32, 4.0/32, 2.0/32, 0.0, 2.0/32, 3.0/32, 2.0/32, 0.0},  // Sierra3
+	{4.0/16, 3.0/16, 1.0/16, 2.0/16, 3.0/16, 2.0/16, 1.0/16, 0.0, 0.0, 0.0, 0.0, 0.0},  // Sierra2
+	{2.0/4, 0.0, 0.0, 1.0/4, 1.0/4, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                 // Sierra 4-2-1
+	{1.0/8, 1.0/8, 0.0, 1.0/8, 1.0/8, 1.0/8, 0.0, 0.0, 0.0, 1.0/8, 0.0, 0.0}};          // Atkinson
+
+// floor(e(s)): number of thresholds <= s.
+__device__ __forceinline__ int level_floor(const DevOChan &q, double s)
+{
+	int lo = 0, n = q.P;
+	while (n > 0) {
+		const int half = n >> 1;
+		if (q.T[lo + half] <= s) {
+			lo += half + 1;
+			n -= half + 1;
+		} else {
+			n = half;
+		}
+	}
+	return lo;
+}
+
+// Index of the first nearest-colour midpoint above s (0..255).
+__device__ __forceinline__ int nc_search(const double *nc, double s)
+{
+	int lo = 0, n = 255;
+	while (n > 0) {
+		const int half = n >> 1;
+		if (nc[lo + half] <= s) {
+			lo += half + 1;
+			n -= half + 1;
+		} else {
+			n = half;
+		}
+	}
+	return lo;
+}
+
+constexpr uint64_t kMontMinv = 0xfff0bf22ffffffffull;
+
+// The reference's generator state for draw (i, j): Montgomery product of the
+// row's Q and the column's b^-i, then x = (Q-1) / a.
+__device__ __forceinline__ uint32_t prng_draw(const DevOChan &q, int i, int j)
+{
+	const uint64_t M = iwx::kPrngM;
+	const uint64_t a = q.prng_row[j], b = q.prng_col[i];
+	const uint64_t lo = a * b, hi = __umul64hi(a, b);
+	const uint64_t minv = kMontMinv;  // -m^-1 mod 2^64, checked on the host
+	const uint64_t u = lo * minv;
+	const uint64_t ulo = u * M, uhi = __umul64hi(u, M);
+	const uint64_t s_lo = lo + ulo;
+	const uint64_t carry = (s_lo < lo) ? 1 : 0;
+	uint64_t t = hi + uhi;
+	const bool ov1 = t < hi;
+	const uint64_t t2 = t + carry;
+	const bool ov = ov1 || (t2 < t);
+	t = t2;
+	if (ov || t >= M) t -= M;
+	return (uint32_t)((t - 1) / iwx::kPrngA);
+}
+
+__device__ __forceinline__ void store_sample(const DevPlan &P, int i, int j, int o, double s_full)
+{
+	uint8_t *row = P.out + (size_t)j * P.out_bpr;
+	if (P.out_depth == 8) {
+		row[P.out_nch * i + o] = (uint8_t)(0.5 + s_full);
+	} else {
+		const unsigned int v = (unsigned short)(0.5 + s_full);
+		uint8_t *q = row + (size_t)(P.out_nch * i + o) * 2;
+		q[0] = (uint8_t)(v >> 8);
+		q[1] = (uint8_t)(v & 0xff);
+	}
+}
+
+// put_sample_convert_from_linear for the non-error-diffusion families.
+__device__ __forceinline__ double quantize(const DevOChan &q, double s, int i, int j)
+{
+	if (q.qmode == Q_NC) return (double)nc_search(q.nc, s);
+	const int k = level_floor(q, s);
+	if (k >= q.P || s < q.U[k]) return q.code[k];
+	const double d_floor = s - q.lin[k];
+	const double d_ceil = q.lin[k + 1] - s;
+	switch (q.family) {
+	case IW_DITHERFAMILY_NONE:
+		return (d_ceil <= d_floor) ? q.code[k + 1] : q.code[k];
+	case IW_DITHERFAMILY_ORDERED: {
+		const double thr = (double)c_dither_pattern[q.subtype ? 1 : 0][(i % 8) + 8 * (j % 8)];
+		return (d_floor / (d_floor + d_ceil) >= thr) ? q.code[k + 1] : q.code[k];
+	}
+	case IW_DITHERFAMILY_RANDOM: {
+		const double thr = ((double)prng_draw(q, i, j)) / (double)0xffffffff;
+		return (d_floor / (d_floor + d_ceil) >= thr) ? q.code[k + 1] : q.code[k];
+	}
+	}
+	return 0.0;
+}
+
+// ---- horizontal pass ------------------------------------------------------
+
+__device__ __forceinline__ double hgather(const DevPlan &P, int c, int j, int x)
+{
+	const DevIChan &ic = P.ic[c];
+	const float *row = P.inter + ((size_t)c * P.Hp + j) * P.W;
+	if (ic.h.mode == iwx::WeightList::COPY) {
+		const int s = ic.h.copy_src[x];
+		return (s < 0) ? 0.0 : (double)row[s];
+	}
+	double acc = 0.0;
+	const int t1 = ic.h.start[x + 1];
+	for (int t = ic.h.start[x]; t < t1; t++) {
+		const int s = ic.h.src[t];
+		const double v = (s < 0) ? ic.h.edge : (double)row[s];
+		acc = acc + v * ic.h.w[t];
+	}
+	return acc;
+}
+
+__global__ void __launch_bounds__(256) hpass_kernel(DevPlan P)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int j = blockIdx.y;
+	if (i >= P.out_w) return;
+	float alpha_f = 0.0f;
+	int alt = 0;
+	if (P.checker)
+		alt = (((P.ox + i) / P.check_size) % 2) != (((P.oy + j) / P.check_size) % 2);
+	const double bkalpha = P.bk_has_trans ? (alt ? P.bk2a : P.bk1a) : 0.0;
+	for (int idx = 0; idx < P.inter_nch; idx++) {
+		const int c = P.order[idx];
+		const DevIChan &ic = P.ic[c];
+		double v = hgather(P, c, j, i);
+		if (P.intclamp) v = clamp01(v);
+		const bool is_alpha = (c == P.inter_alpha);
+		if (is_alpha) alpha_f = (float)v;
+		const int o = ic.output_channel;
+		if (o < 0) continue;
+		const DevOChan &q = P.oc[o];
+		if (ic.need_unassoc) {
+			const double a = (double)alpha_f;
+			if (a != 0.0) v /= a;
+			if (P.apply_bkgd && P.late) {
+				const double bk = alt ? q.bk2 : q.bk1;
+				if (P.bk_has_trans) v = v * a + bk * bkalpha * (1.0 - a);
+				else v = v * a + bk * (1.0 - a);
+			}
+		} else if (is_alpha && P.bk_has_trans) {
+			v = v + bkalpha * (1.0 - v);
+		}
+		if (P.out_float) {
+			const float f = (float)v;
+			memcpy(P.out + (size_t)j * P.out_bpr + (size_t)(P.out_nch * i + o) * 4, &f, 4);
+			continue;
+		}
+		v = clamp01(v);
+		if (q.family == IW_DITHERFAMILY_ERRDIFF) {
+			P.ed_samples[((size_t)q.ed_index * P.out_h + j) * P.out_w + i] = v;
+			continue;
+		}
+		store_sample(P, i, j, o, quantize(q, v, i, j));
+	}
+}
+
+// ---- error diffusion ------------------------------------------------------
+// One plane is one dependency chain (serpentine rows), so a plane is walked
+// by one thread; planes run in parallel.
+
+__global__ void errdiff_serial_kernel(DevPlan P, int o_list0, int o_list1, int o_list2, int o_list3)
+{
+	const int which = blockIdx.x;
+	const int olist[4] = {o_list0, o_list1, o_list2, o_list3};
+	const int o = olist[which];
+	if (threadIdx.x != 0 || o < 0) return;
+	const DevOChan &q = P.oc[o];
+	const double *m = c_ed_matrix[q.subtype <= 7 ? q.subtype : 0];
+	const int W = P.out_w;
+	double *rows[3];
+	for (int r = 0; r < 3; r++) rows[r] = P.ed_err + ((size_t)q.ed_index * 3 + r) * W;
+	for (int r = 0; r < 3; r++)
+		for (int x = 0; x < W; x++) rows[r][x] = 0.0;
+	const double *samp = P.ed_samples + (size_t)q.ed_index * P.out_h * W;
+	for (int j = 0; j < P.out_h; j++) {
+		const int fwd = (j % 2) ? -1 : 1;
+		for (int z = 0; z < W; z++) {
+			const int x = (j % 2) ? W - 1 - z : z;
+			double s = samp[(size_t)j * W + x];
+			s += rows[0][x];
+			if (s > 1.0) s = 1.0;
+			else if (s < 0.0) s = 0.0;
+			const int k = level_floor(q, s);
+			double s_full;
+			if (k >= q.P || s < q.U[k]) {
+				s_full = q.code[k];
+			} else {
+				const double d_floor = s - q.lin[k];
+				const double d_ceil = q.lin[k + 1] - s;
+				double err;
+				if (d_ceil <= d_floor) {
+					err = -d_ceil;
+					s_full = q.code[k + 1];
+				} else {
+					err = d_floor;
+					s_full = q.code[k];
+				}
+				const int xm1 = x - fwd, xm2 = x - 2 * fwd, xp1 = x + fwd, xp2 = x + 2 * fwd;
+				if (xm1 >= 0 && xm1 < W) {
+					if (xm2 >= 0 && xm2 < W) {
+						rows[1][xm2] += err * m[2];
+						rows[2][xm2] += err * m[7];
+					}
+					rows[1][xm1] += err * m[3];
+					rows[2][xm1] += err * m[8];
+				}
+				rows[1][x] += err * m[4];
+				rows[2][x] += err * m[9];
+				if (xp1 >= 0 && xp1 < W) {
+					rows[0][xp1] += err * m[0];
+					rows[1][xp1] += err * m[5];
+					rows[2][xp1] += err * m[10];
+					if (xp2 >= 0 && xp2 < W) {
+						rows[0][xp2] += err * m[1];
+						rows[1][xp2] += err * m[6];
+						rows[2][xp2] += err * m[11];
+					}
+				}
+			}
+			store_sample(P, x, j, o, s_full);
+		}
+		// Next row's errors become current; the last row starts at zero.
+		double *t = rows[0];
+		rows[0] = rows[1];
+		rows[1] = rows[2];
+		rows[2] = t;
+		for (int x = 0; x < W; x++) rows[2][x] = 0.0;
+	}
+}
+
+// ---- device memory and stream -------------------------------------------
+
+struct Runtime {
+	std::mutex mu;
+	bool init = false;
+	int ndev = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev[8];
+	void *arena = nullptr;
+	size_t arena_size = 0;
+	std::string fail_msg;
+
+	bool ensure(size_t need)
+	{
+		if (need <= arena_size) return true;
+		if (arena) hipFree(arena);
+		arena = nullptr;
+		arena_size = 0;
+		size_t sz = need + need / 4 + (1 << 20);
+		if (hipMalloc(&arena, sz) != hipSuccess) {
+			fail_msg = "hipMalloc failed";
+			return false;
+		}
+		arena_size = sz;
+		return true;
+	}
+};
+
+Runtime &rt()
+{
+	static Runtime r;
+	return r;
+}
+
+// Bump allocator over the arena; two passes (measure, then place).
+struct Layout {
+	size_t off = 0;
+	template <class T>
+	size_t add(size_t count)
+	{
+		off = (off + 255) & ~(size_t)255;
+		size_t at = off;
+		off += count * sizeof(T);
+		return at;
+	}
+};
+
+}  // namespace
+
+int iwx_gpu_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+	return n;
+}
+
+int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
+{
+	using namespace iwx;
+	Runtime &R = rt();
+	std::lock_guard<std::mutex> lock(R.mu);
+	std::string &fail_msg = R.fail_msg;
+	fail_msg.clear();
+
+	if (!R.init) {
+		if (hipGetDeviceCount(&R.ndev) != hipSuccess || R.ndev < 1) {
+			iw_set_error(ctx, "No HIP device available for the pixel pipeline");
+			return 0;
+		}
+		if (hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking) != hipSuccess) {
+			iw_set_error(ctx, "hipStreamCreate failed");
+			return 0;
+		}
+		for (auto &e : R.ev) hipEventCreate(&e);
+		if (prng_mont_minv() != kMontMinv) {
+			iw_set_error(ctx, "Internal: PRNG Montgomery constant mismatch");
+			return 0;
+		}
+		R.init = true;
+	}
+
+	const iwamd_plan &p = *plan;
+	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT) {
+		for (int c = 0; c < p.inter_nch; c++) {
+			if (p.ich[c].in_cs.cstype != IW_CSTYPE_LINEAR) {
+				iw_set_error(ctx, "Floating-point input needs a linear input colorspace on the GPU path");
+				return 0;
+			}
+		}
+		if (p.in_bit_depth != 32) {
+			iw_set_error(ctx, "Unsupported floating-point input depth");
+			return 0;
+		}
+	}
+
+	// ---- host tables ----
+	std::vector<WeightList> wv(p.inter_nch), wh(p.inter_nch);
+	for (int c = 0; c < p.inter_nch; c++) {
+		wv[c] = build_weight_list(p.rs[IW_DIMENSION_V], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_V]);
+		wh[c] = build_weight_list(p.rs[IW_DIMENSION_H], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_H]);
+		if (!wv[c].ok || !wh[c].ok) {
+			iw_set_error(ctx, "Internal: Unknown resize algorithm");
+			memset(img2, 0, p.out_bpr * p.out_height);
+			return 1;  // the reference reports the error and carries on
+		}
+	}
+	const bool in_int = p.in_sampletype != IW_SAMPLETYPE_FLOATINGPOINT;
+	const int nvalues = in_int ? (1 << p.in_bit_depth) : 0;
+	std::vector<std::vector<double>> luts;
+	std::map<std::tuple<int, double, double>, int> lut_index;
+	auto get_lut = [&](double maxcc, const struct iw_csdescr &cs) {
+		auto key = std::make_tuple(cs.cstype, cs.cstype == IW_CSTYPE_GAMMA ? cs.gamma : 0.0, maxcc);
+		auto it = lut_index.find(key);
+		if (it != lut_index.end()) return it->second;
+		int id = (int)luts.size();
+		luts.push_back(build_input_lut(nvalues, maxcc, cs));
+		lut_index[key] = id;
+		return id;
+	};
+	int ic_lut[4][3];
+	int alpha_lut = -1;
+	if (in_int) {
+		for (int c = 0; c < p.inter_nch; c++) {
+			const iwamd_ichan &ic = p.ich[c];
+			const int nk = ic.cvt_to_grayscale ? 3 : 1;
+			for (int k = 0; k < 3; k++) ic_lut[c][k] = -1;
+			for (int k = 0; k < nk; k++) {
+				const int ch = ic.input_channel + k;
+				if (ch >= p.in_nch_physical) continue;
+				const double mcc = p.in_ch_slow[ic.input_channel] ? p.in_ch_maxcolorcode[ch] : p.in_maxcolorcode;
+				ic_lut[c][k] = get_lut(mcc, ic.in_cs);
+			}
+		}
+		if (p.in_has_alpha && p.in_alpha_index < p.in_nch_physical) {
+			struct iw_csdescr lin;
+			iw_make_linear_csdescr(&lin);
+			alpha_lut = get_lut(p.in_ch_maxcolorcode[p.in_alpha_index], lin);
+		}
+	}
+	std::vector<QuantTable> qt(p.out_nch);
+	std::vector<PrngTables> pt(p.out_nch);
+	int n_ed = 0;
+	int ed_of[4] = {-1, -1, -1, -1};
+	if (p.out_sampletype != IW_SAMPLETYPE_FLOATINGPOINT) {
+		for (int o = 0; o < p.out_nch; o++) {
+			qt[o] = build_quant_table(p.och[o], p.och[o].use_nc_table != 0);
+			if (p.och[o].ditherfamily == IW_DITHERFAMILY_RANDOM) {
+				pt[o] = build_prng_tables(p.och[o].random_seed, p.out_width, p.out_height);
+				if (!pt[o].ok) {
+					iw_set_error(ctx, "Random seed not supported by the GPU path");
+					return 0;
+				}
+			}
+			if (p.och[o].ditherfamily == IW_DITHERFAMILY_ERRDIFF) ed_of[o] = n_ed++;
+		}
+	}
+
+	// ---- device layout ----
+	Layout L;
+	const size_t in_bytes = p.in_bpr * (size_t)p.in_height;
+	const size_t o_in = L.add<uint8_t>(in_bytes);
+	const size_t o_inter = L.add<float>((size_t)p.inter_nch * p.inter_height * p.inter_width);
+	const size_t out_bytes = p.out_bpr * (size_t)p.out_height;
+	const size_t o_out = L.add<uint8_t>(out_bytes);
+	const size_t o_eds = L.add<double>((size_t)n_ed * p.out_height * p.out_width);
+	const size_t o_ede = L.add<double>((size_t)n_ed * 3 * p.out_width);
+	std::vector<size_t> o_lut(luts.size());
+	for (size_t i = 0; i < luts.size(); i++) o_lut[i] = L.add<double>(luts[i].size());
+	struct WOff { size_t start, src, w, copy; };
+	std::vector<WOff> ov(p.inter_nch), oh(p.inter_nch);
+	auto place_w = [&](const WeightList &wl, WOff &o) {
+		o.start = L.add<int>(wl.start.size());
+		o.src = L.add<int>(wl.src.size());
+		o.w = L.add<double>(wl.w.size());
+		o.copy = L.add<int>(wl.copy_src.size());
+	};
+	for (int c = 0; c < p.inter_nch; c++) {
+		place_w(wv[c], ov[c]);
+		place_w(wh[c], oh[c]);
+	}
+	struct QOff { size_t T, U, lin, code, nc, prow, pcol; };
+	std::vector<QOff> oq(p.out_nch);
+	for (int o = 0; o < (int)qt.size(); o++) {
+		oq[o].T = L.add<double>(qt[o].T.size());
+		oq[o].U = L.add<double>(qt[o].U.size());
+		oq[o].lin = L.add<double>(qt[o].lin.size());
+		oq[o].code = L.add<double>(qt[o].code.size());
+		oq[o].nc = L.add<double>(qt[o].nc.size());
+		oq[o].prow = L.add<uint64_t>(pt[o].row_q.size());
+		oq[o].pcol = L.add<uint64_t>(pt[o].col_mont.size());
+	}
+	if (!R.ensure(L.off + 256)) {
+		iw_set_error(ctx, "Out of device memory");
+		return 0;
+	}
+	char *base = (char *)R.arena;
+	hipStream_t st = R.stream;
+	auto up = [&](size_t off, const void *src, size_t bytes) -> bool {
+		if (!bytes) return true;
+		return hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+	};
+	bool ok = true;
+	hipEventRecord(R.ev[0], st);
+	ok &= up(o_in, p.in_pixels, in_bytes);
+	for (size_t i = 0; i < luts.size(); i++) ok &= up(o_lut[i], luts[i].data(), luts[i].size() * 8);
+	auto up_w = [&](const WeightList &wl, const WOff &o) {
+		ok &= up(o.start, wl.start.data(), wl.start.size() * 4);
+		ok &= up(o.src, wl.src.data(), wl.src.size() * 4);
+		ok &= up(o.w, wl.w.data(), wl.w.size() * 8);
+		ok &= up(o.copy, wl.copy_src.data(), wl.copy_src.size() * 4);
+	};
+	for (int c = 0; c < p.inter_nch; c++) {
+		up_w(wv[c], ov[c]);
+		up_w(wh[c], oh[c]);
+	}
+	for (int o = 0; o < (int)qt.size(); o++) {
+		ok &= up(oq[o].T, qt[o].T.data(), qt[o].T.size() * 8);
+		ok &= up(oq[o].U, qt[o].U.data(), qt[o].U.size() * 8);
+		ok &= up(oq[o].lin, qt[o].lin.data(), qt[o].lin.size() * 8);
+		ok &= up(oq[o].code, qt[o].code.data(), qt[o].code.size() * 8);
+		ok &= up(oq[o].nc, qt[o].nc.data(), qt[o].nc.size() * 8);
+		ok &= up(oq[o].prow, pt[o].row_q.data(), pt[o].row_q.size() * 8);
+		ok &= up(oq[o].pcol, pt[o].col_mont.data(), pt[o].col_mont.size() * 8);
+	}
+	if (!ok) {
+		iw_set_error(ctx, "Host-to-device copy failed");
+		return 0;
+	}
+
+	// ---- kernel arguments ----
+	DevPlan D;
+	memset(&D, 0, sizeof(D));
+	D.in = (const uint8_t *)(base + o_in);
+	D.in_bpr = p.in_bpr;
+	D.in_w = p.in_width;
+	D.in_h = p.in_height;
+	D.bit_depth = p.in_bit_depth;
+	D.in_float = !in_int;
+	D.nch_phys = p.in_nch_physical;
+	D.orient = p.orient_transform;
+	D.sx = p.input_start_x;
+	D.sy = p.input_start_y;
+	D.iw = p.input_w;
+	D.ih = p.input_h;
+	D.alpha_in_index = p.in_has_alpha ? p.in_alpha_index : 1000;
+	D.alpha_lut = alpha_lut >= 0 ? (const double *)(base + o_lut[alpha_lut]) : nullptr;
+	D.gsf = p.grayscale_formula;
+	for (int k = 0; k < 3; k++) D.gsw[k] = p.grayscale_weight[k];
+	D.inter_nch = p.inter_nch;
+	D.inter_alpha = p.inter_alpha_index;
+	{
+		int n = 0;
+		if (p.inter_alpha_index >= 0) D.order[n++] = p.inter_alpha_index;
+		for (int c = 0; c < p.inter_nch; c++)
+			if (c != p.inter_alpha_index) D.order[n++] = c;
+	}
+	D.W = p.inter_width;
+	D.Hp = p.inter_height;
+	for (int c = 0; c < p.inter_nch; c++) {
+		const iwamd_ichan &s = p.ich[c];
+		DevIChan &d = D.ic[c];
+		d.input_channel = s.input_channel;
+		d.cvt_gray = s.cvt_to_grayscale;
+		d.output_channel = s.output_channel;
+		d.need_unassoc = s.need_unassoc_alpha;
+		d.early_bkgd = s.early_bkgd;
+		d.bkgd = s.bkgd_color_lin;
+		for (int k = 0; k < 3; k++)
+			d.lut[k] = (in_int && ic_lut[c][k] >= 0) ? (const double *)(base + o_lut[ic_lut[c][k]]) : nullptr;
+		auto fill = [&](const WeightList &wl, const WOff &o, DevWeights &dw) {
+			dw.mode = wl.mode;
+			dw.start = (const int *)(base + o.start);
+			dw.src = (const int *)(base + o.src);
+			dw.w = (const double *)(base + o.w);
+			dw.copy_src = (const int *)(base + o.copy);
+			dw.edge = wl.edge_value;
+		};
+		fill(wv[c], ov[c], d.v);
+		fill(wh[c], oh[c], d.h);
+	}
+	D.inter = (float *)(base + o_inter);
+	D.intclamp = p.intclamp;
+	D.out_w = p.out_width;
+	D.out_h = p.out_height;
+	D.out_depth = p.out_bit_depth;
+	D.out_float = p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT;
+	D.out_nch = p.out_nch;
+	D.out_bpr = p.out_bpr;
+	D.out = (uint8_t *)(base + o_out);
+	for (int o = 0; o < p.out_nch; o++) {
+		const iwamd_ochan &s = p.och[o];
+		DevOChan &d = D.oc[o];
+		d.family = s.ditherfamily;
+		d.subtype = s.dithersubtype;
+		d.is_alpha = s.channeltype == IW_CHANNELTYPE_ALPHA;
+		d.ed_index = ed_of[o];
+		d.bk1 = s.bkgd1_color_lin;
+		d.bk2 = s.bkgd2_color_lin;
+		if (!D.out_float) {
+			d.qmode = s.use_nc_table ? Q_NC : Q_THRESH;
+			d.P = qt[o].P;
+			d.T = (const double *)(base + oq[o].T);
+			d.U = (const double *)(base + oq[o].U);
+			d.lin = (const double *)(base + oq[o].lin);
+			d.code = (const double *)(base + oq[o].code);
+			d.nc = (const double *)(base + oq[o].nc);
+			d.prng_row = (const uint64_t *)(base + oq[o].prow);
+			d.prng_col = (const uint64_t *)(base + oq[o].pcol);
+		}
+	}
+	D.ed_samples = (double *)(base + o_eds);
+	D.ed_err = (double *)(base + o_ede);
+	D.n_ed = n_ed;
+	D.apply_bkgd = p.apply_bkgd;
+	D.late = p.bkgd_strategy_late;
+	D.checker = p.bkgd_checkerboard;
+	D.check_size = p.bkgd_check_size;
+	D.ox = p.bkgd_check_origin[0];
+	D.oy = p.bkgd_check_origin[1];
+	D.bk_has_trans = p.bkgd_has_transparency;
+	D.bk1a = p.bkgd1alpha;
+	D.bk2a = p.bkgd2alpha;
+
+	// ---- launches ----
+	hipEventRecord(R.ev[1], st);
+	{
+		dim3 grid((p.inter_width + 255) / 256, p.inter_height);
+		hipLaunchKernelGGL(vpass_kernel, grid, dim3(256), 0, st, D);
+	}
+	hipEventRecord(R.ev[2], st);
+	{
+		dim3 grid((p.out_width + 255) / 256, p.out_height);
+		hipLaunchKernelGGL(hpass_kernel, grid, dim3(256), 0, st, D);
+	}
+	hipEventRecord(R.ev[3], st);
+	if (n_ed > 0) {
+		int ol[4] = {-1, -1, -1, -1};
+		for (int o = 0; o < p.out_nch; o++)
+			if (ed_of[o] >= 0) ol[ed_of[o]] = o;
+		hipLaunchKernelGGL(errdiff_serial_kernel, dim3(n_ed), dim3(64), 0, st, D, ol[0], ol[1], ol[2], ol[3]);
+	}
+	hipEventRecord(R.ev[4], st);
+	if (hipMemcpyAsync(img2, base + o_out, out_bytes, hipMemcpyDeviceToHost, st) != hipSuccess) {
+		iw_set_error(ctx, "Device-to-host copy failed");
+		return 0;
+	}
+	hipEventRecord(R.ev[5], st);
+	hipError_t e = hipStreamSynchronize(st);
+	if (e != hipSuccess) {
+		iw_set_errorf(ctx, "GPU pipeline failed: %s", hipGetErrorString(e));
+		return 0;
+	}
+	ctx->num_last_timings = 5;
+	for (int k = 0; k < 5; k++) {
+		float ms = 0.0f;
+		hipEventElapsedTime(&ms, R.ev[k], R.ev[k + 1]);
+		ctx->last_timings_ms[k] = ms;
+	}
+	return 1;
+}

@@ -1,0 +1,77 @@
+// Host-built tables that drive the device kernels.
+//
+// Everything that needs libm (pow/sin/cos/exp) is evaluated here, on the host,
+// with the system libm -- the device performs only IEEE + - * /, compares and
+// conversions, so its results carry the same bits as the reference's.
+#pragma once
+
+#include "iwamd.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace iwx {
+
+// One resampling pass along one dimension for one channel.  In WEIGHTS mode
+// output o is sum over t in [start[o], start[o+1]) of in[src[t]] * w[t],
+// accumulated from 0.0 in list order; src -1 reads the virtual-pixel value.
+// In COPY mode output o is in[copy_src[o]], or 0.0 when copy_src[o] < 0.
+struct WeightList {
+	enum { WEIGHTS = 0, COPY = 1 };
+	int mode = WEIGHTS;
+	int num_in = 0, num_out = 0;
+	int max_taps = 0;
+	std::vector<int> start;      // num_out + 1
+	std::vector<int> src;
+	std::vector<double> w;
+	std::vector<int> copy_src;   // num_out
+	double edge_value = 0.0;     // value of virtual pixels
+	bool ok = true;              // false: unknown family
+};
+
+// src/imagew-resize.c:412-556 and :240-344.
+WeightList build_weight_list(const iwamd_resize &rs, int channeltype, double edge_value);
+
+// Quantisation of one output channel: linear sample s in [0,1] -> output code.
+//
+// With color_count==0 the reference computes e = clamp(F(s)*M) with F the
+// colour-space transfer (libm pow) and picks floor(e) or ceil(e)
+// (src/imagew-main.c:570-629).  Here that is replaced by thresholds over s
+// found by bisection on the bit patterns of doubles:
+//   T[k-1] = min{ s : e(s) >= k }   (k = 1..P)
+//   U[k]   = min{ s : e(s) >  k }   (k = 0..P-1), U[P] = +inf
+// so floor(e(s)) = #{k : T[k-1] <= s} and "e is an integer" <=> s < U[floor].
+// code[k] is the output code of level k (posterised levels map to
+// floor(0.5000000001 + k*(M/P))), lin[k] its value in linear light.
+struct QuantTable {
+	int P = 0;                   // number of levels - 1
+	std::vector<double> T, U, lin, code;
+	std::vector<double> nc;      // nearest-colour midpoints (255), when used
+};
+
+QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc);
+
+// Linearising lookup for integer input samples: lut[v] = X(v / maxcc).
+std::vector<double> build_input_lut(int nvalues, double maxcc, const struct iw_csdescr &cs);
+
+// Random dithering: the reference's lag-1 complementary multiply-with-carry
+// generator (src/imagew-util.c:199-243) viewed as Q_n = Q_1 * b^-(n-1) mod m,
+// m = a*2^32 + 1, Q = a*x + c + 1.  row_q[j] = Q at the first draw of row j;
+// col_mont[i] = b^-i * 2^64 mod m, so a Montgomery product gives the draw of
+// pixel (i, j).
+struct PrngTables {
+	bool ok = true;
+	std::vector<uint64_t> row_q;
+	std::vector<uint64_t> col_mont;
+};
+PrngTables build_prng_tables(int seed, int width, int height);
+
+constexpr uint64_t kPrngA = 0xfff0bf23ull;
+constexpr uint64_t kPrngM = 0xfff0bf2300000001ull;
+uint64_t prng_mont_minv();  // -m^-1 mod 2^64
+
+// Reference-order CPU draws, for tests of the jump-ahead.
+uint32_t prng_sequential(int seed, uint64_t n);
+
+}  // namespace iwx

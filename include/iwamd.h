@@ -1,0 +1,149 @@
+/*
+ * iwamd.h -- extensions of the MI355X ImageWorsener library beyond the
+ * reference API in imagew.h.
+ *
+ * 1. Split-phase processing.  iw_process_image() (src/imagew-main.c:2250-2272)
+ *    is prepare -> pixel pipeline -> finish.  iwamd_prepare() runs every host
+ *    decision of iw_prepare_processing() (src/imagew-main.c:1899-2248) and
+ *    describes the pixel work as a plain-C plan; iwamd_finish() runs what
+ *    follows the pixel loops (background label, negate, output optimiser;
+ *    src/imagew-main.c:1437-1441, src/imagew-opt.c:1166-1314) on an output
+ *    raster supplied by the caller.  iw_process_image() itself is
+ *    prepare -> HIP kernels -> finish.  The split lets a scheduler batch the
+ *    pixel work of many contexts on the GPU, and lets a test run another
+ *    implementation of the pixel pipeline on exactly the same plan.
+ *
+ * 2. Precision mode.  IWAMD_PRECISION_PLAIN (the default) is the FP64 form
+ *    whose output bytes equal the reference's.  IWAMD_PRECISION_FAST resamples
+ *    on FP16 MFMA (FP32 accumulate) and quantises in FP64; its error against
+ *    the plain form is bounded and reported, not zero.
+ *
+ * 3. Batches and a request loop (iwamd_batch_*, iwamd_server_*).
+ */
+#ifndef IWAMD_H
+#define IWAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "imagew.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IWAMD_MAX_CH 4
+
+/* ---- per-dimension resampling settings (src/imagew-internals.h:38-50,
+ *      src/imagew-resize.c:412-556) ------------------------------------- */
+struct iwamd_resize {
+	int family;               /* IW_RESIZETYPE_*, never AUTO after prepare */
+	int edge_policy;          /* IW_EDGE_POLICY_* */
+	int num_in, num_out;      /* samples along this dimension */
+	double param1, param2;    /* as set by iw_set_resize_alg */
+	double blur_factor;       /* as set (clamped when weights are built) */
+	double out_true_size;
+	double translate;
+	int use_offset;
+	double channel_offset[3]; /* by IW_CHANNELTYPE_RED..BLUE */
+};
+
+/* ---- one intermediate channel (src/imagew-internals.h:59-70) ---------- */
+struct iwamd_ichan {
+	int channeltype;          /* IW_CHANNELTYPE_* */
+	int input_channel;        /* first input channel read */
+	int cvt_to_grayscale;     /* reads input_channel..+2 */
+	int output_channel;       /* -1: resampled but not written */
+	int need_unassoc_alpha;   /* premultiply on input, divide on output */
+	int early_bkgd;           /* blend with bkgd_color_lin on input */
+	double bkgd_color_lin;
+	double edge_sample_value[2]; /* virtual-pixel value per dimension */
+	struct iw_csdescr in_cs;  /* colour space used to linearise input */
+};
+
+/* ---- one output channel (src/imagew-internals.h:72-88) ---------------- */
+struct iwamd_ochan {
+	int channeltype;
+	int ditherfamily, dithersubtype;
+	int color_count;          /* 0 = full depth */
+	int maxcolorcode_int;
+	double maxcolorcode_dbl;
+	int use_nc_table;         /* nearest-colour table decides (no dither) */
+	int random_seed;          /* PRNG seed when ditherfamily==RANDOM */
+	double bkgd1_color_lin, bkgd2_color_lin;
+	struct iw_csdescr out_cs; /* colour space used to leave linear light */
+};
+
+struct iwamd_plan {
+	/* input image (physical layout, src/imagew.h:377-410) */
+	const uint8_t *in_pixels;
+	size_t in_bpr;
+	int in_width, in_height;  /* physical dimensions of img1 */
+	int in_bit_depth, in_sampletype, in_imgtype;
+	int in_nch_physical;
+	unsigned int orient_transform;
+	int input_start_x, input_start_y, input_w, input_h;
+	int in_alpha_index;       /* logical index of the alpha channel */
+	int in_has_alpha;         /* logical input type has alpha */
+	double in_maxcolorcode;   /* 2^depth-1 */
+	double in_ch_maxcolorcode[IWAMD_MAX_CH];
+	int in_ch_slow[IWAMD_MAX_CH]; /* disable_fast_get_sample */
+	int grayscale_formula;    /* IW_GSF_WEIGHTED or ORDERBYVALUE */
+	double grayscale_weight[3];
+
+	/* intermediate image */
+	int inter_nch;
+	int inter_alpha_index;    /* -1 when no alpha channel */
+	int inter_width, inter_height;
+	struct iwamd_ichan ich[IWAMD_MAX_CH];
+	struct iwamd_resize rs[2]; /* by IW_DIMENSION_* */
+	int intclamp;
+
+	/* output image */
+	int out_width, out_height, out_bit_depth, out_sampletype, out_imgtype;
+	int out_nch;
+	size_t out_bpr;
+	struct iwamd_ochan och[IWAMD_MAX_CH];
+
+	/* background (src/imagew-main.c:1734-1823, 1082-1114) */
+	int apply_bkgd, bkgd_strategy_late, bkgd_checkerboard, bkgd_check_size;
+	int bkgd_check_origin[2];
+	int bkgd_has_transparency;
+	double bkgd1alpha, bkgd2alpha;
+
+	/* precision form requested for the pixel work */
+	int precision;
+};
+
+#define IWAMD_PRECISION_PLAIN 0
+#define IWAMD_PRECISION_FAST  1
+
+/* Split-phase processing.  iwamd_prepare() consumes the context's single use
+ * (a later iw_process_image fails exactly as a second call would).  The plan
+ * holds pointers into the context and stays valid until iwamd_finish() or
+ * iw_destroy_context().  iwamd_finish() copies out_height*out_bpr bytes from
+ * img2 and completes the context so iw_get_output_image() works.  Both return
+ * 1 on success, 0 on failure with the context's error flag set. */
+IW_EXPORT(int) iwamd_prepare(struct iw_context *ctx, struct iwamd_plan *plan);
+IW_EXPORT(int) iwamd_finish(struct iw_context *ctx, const uint8_t *img2);
+
+/* sizeof(struct iwamd_plan), for callers that hold plans opaquely. */
+IW_EXPORT(size_t) iwamd_plan_size(void);
+
+/* Precision form for this context (IWAMD_PRECISION_*). */
+IW_EXPORT(void) iwamd_set_precision(struct iw_context *ctx, int mode);
+
+/* Library/device information.  Returns the number of visible HIP devices
+ * (0 when none), and fills name (may be NULL). */
+IW_EXPORT(int) iwamd_device_count(void);
+IW_EXPORT(const char*) iwamd_version(void);
+
+/* Per-stage device timings of the last iw_process_image on this context, in
+ * milliseconds, measured with HIP events on the stream the kernels ran on.
+ * Fills up to n entries; returns the number of stages. */
+IW_EXPORT(int) iwamd_last_timings(struct iw_context *ctx, double *ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IWAMD_H */
