@@ -207,5 +207,5 @@ int iwx_finish(iw_context *ctx);
 void iwx_optimize_image(iw_context *ctx);
 
 // iw_gpu.hip
-int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2);
+int iwx_gpu_process(iw_context *ctx);
 int iwx_gpu_device_count(void);

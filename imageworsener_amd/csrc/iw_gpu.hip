@@ -33,15 +33,6 @@
 
 namespace {
 
-#define HIP_OK(expr) \
-	do { \
-		hipError_t e_ = (expr); \
-		if (e_ != hipSuccess) { \
-			fail_msg = hipGetErrorString(e_); \
-			return false; \
-		} \
-	} while (0)
-
 // ---- device-side plan ----------------------------------------------------
 
 struct DevWeights {
@@ -495,27 +486,12 @@ __global__ void errdiff_serial_kernel(DevPlan P, int o_list0, int o_list1, int o
 struct Runtime {
 	std::mutex mu;
 	bool init = false;
+	bool init_failed = false;
 	int ndev = 0;
-	hipStream_t stream = nullptr;
-	hipEvent_t ev[8];
-	void *arena = nullptr;
-	size_t arena_size = 0;
-	std::string fail_msg;
-
-	bool ensure(size_t need)
-	{
-		if (need <= arena_size) return true;
-		if (arena) hipFree(arena);
-		arena = nullptr;
-		arena_size = 0;
-		size_t sz = need + need / 4 + (1 << 20);
-		if (hipMalloc(&arena, sz) != hipSuccess) {
-			fail_msg = "hipMalloc failed";
-			return false;
-		}
-		arena_size = sz;
-		return true;
-	}
+	// Free device blocks by size, reused across jobs: hipMalloc/hipFree cost
+	// far more than a small image's kernels.
+	std::multimap<size_t, void *> free_blocks;
+	size_t cached_bytes = 0;
 };
 
 Runtime &rt()
@@ -524,7 +500,65 @@ Runtime &rt()
 	return r;
 }
 
-// Bump allocator over the arena; two passes (measure, then place).
+constexpr size_t kCacheLimit = (size_t)8 << 30;
+
+bool runtime_init(iw_context *ctx)
+{
+	Runtime &R = rt();
+	std::lock_guard<std::mutex> lock(R.mu);
+	if (R.init) return true;
+	if (R.init_failed || hipGetDeviceCount(&R.ndev) != hipSuccess || R.ndev < 1) {
+		R.init_failed = true;
+		iw_set_error(ctx, "No HIP device available for the pixel pipeline");
+		return false;
+	}
+	if (iwx::prng_mont_minv() != kMontMinv) {
+		iw_set_error(ctx, "Internal: PRNG Montgomery constant mismatch");
+		return false;
+	}
+	R.init = true;
+	return true;
+}
+
+void *dev_alloc(size_t bytes)
+{
+	Runtime &R = rt();
+	{
+		std::lock_guard<std::mutex> lock(R.mu);
+		auto it = R.free_blocks.lower_bound(bytes);
+		if (it != R.free_blocks.end() && it->first <= bytes + bytes / 2 + (1 << 20)) {
+			void *p = it->second;
+			R.cached_bytes -= it->first;
+			R.free_blocks.erase(it);
+			return p;
+		}
+	}
+	void *p = nullptr;
+	if (hipMalloc(&p, bytes) != hipSuccess) {
+		// Release the cache and retry once.
+		std::lock_guard<std::mutex> lock(R.mu);
+		for (auto &b : R.free_blocks) hipFree(b.second);
+		R.free_blocks.clear();
+		R.cached_bytes = 0;
+		if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+	}
+	return p;
+}
+
+void dev_release(void *p, size_t bytes)
+{
+	if (!p) return;
+	Runtime &R = rt();
+	std::lock_guard<std::mutex> lock(R.mu);
+	if (R.cached_bytes + bytes > kCacheLimit) {
+		hipFree(p);
+		return;
+	}
+	R.free_blocks.emplace(bytes, p);
+	R.cached_bytes += bytes;
+}
+
+// Bump layout inside one device block.
 struct Layout {
 	size_t off = 0;
 	template <class T>
@@ -539,49 +573,41 @@ struct Layout {
 
 }  // namespace
 
-int iwx_gpu_device_count(void)
-{
-	int n = 0;
-	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
-	return n;
-}
+// ---- a device-resident job --------------------------------------------------
 
-int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
+struct iwamd_job {
+	iw_context *ctx = nullptr;
+	iwamd_plan plan;
+	DevPlan D;
+	void *block = nullptr;
+	size_t block_bytes = 0;
+	size_t out_off = 0, out_bytes = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+	int n_ed = 0;
+	int ed_list[4] = {-1, -1, -1, -1};
+	double out_pixels = 0.0, alg_bytes = 0.0;
+	bool ran = false;
+	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
+};
+
+namespace {
+
+bool job_upload(iwamd_job *J)
 {
 	using namespace iwx;
-	Runtime &R = rt();
-	std::lock_guard<std::mutex> lock(R.mu);
-	std::string &fail_msg = R.fail_msg;
-	fail_msg.clear();
-
-	if (!R.init) {
-		if (hipGetDeviceCount(&R.ndev) != hipSuccess || R.ndev < 1) {
-			iw_set_error(ctx, "No HIP device available for the pixel pipeline");
-			return 0;
-		}
-		if (hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking) != hipSuccess) {
-			iw_set_error(ctx, "hipStreamCreate failed");
-			return 0;
-		}
-		for (auto &e : R.ev) hipEventCreate(&e);
-		if (prng_mont_minv() != kMontMinv) {
-			iw_set_error(ctx, "Internal: PRNG Montgomery constant mismatch");
-			return 0;
-		}
-		R.init = true;
-	}
-
-	const iwamd_plan &p = *plan;
+	iw_context *ctx = J->ctx;
+	const iwamd_plan &p = J->plan;
 	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT) {
 		for (int c = 0; c < p.inter_nch; c++) {
 			if (p.ich[c].in_cs.cstype != IW_CSTYPE_LINEAR) {
 				iw_set_error(ctx, "Floating-point input needs a linear input colorspace on the GPU path");
-				return 0;
+				return false;
 			}
 		}
 		if (p.in_bit_depth != 32) {
 			iw_set_error(ctx, "Unsupported floating-point input depth");
-			return 0;
+			return false;
 		}
 	}
 
@@ -591,9 +617,10 @@ int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
 		wv[c] = build_weight_list(p.rs[IW_DIMENSION_V], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_V]);
 		wh[c] = build_weight_list(p.rs[IW_DIMENSION_H], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_H]);
 		if (!wv[c].ok || !wh[c].ok) {
+			// The reference reports the error and carries on
+			// (src/imagew-resize.c:512-515).
 			iw_set_error(ctx, "Internal: Unknown resize algorithm");
-			memset(img2, 0, p.out_bpr * p.out_height);
-			return 1;  // the reference reports the error and carries on
+			J->skip_pixels = true;
 		}
 	}
 	const bool in_int = p.in_sampletype != IW_SAMPLETYPE_FLOATINGPOINT;
@@ -610,15 +637,17 @@ int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
 		return id;
 	};
 	int ic_lut[4][3];
+	for (auto &r : ic_lut) r[0] = r[1] = r[2] = -1;
 	int alpha_lut = -1;
 	if (in_int) {
 		for (int c = 0; c < p.inter_nch; c++) {
 			const iwamd_ichan &ic = p.ich[c];
 			const int nk = ic.cvt_to_grayscale ? 3 : 1;
-			for (int k = 0; k < 3; k++) ic_lut[c][k] = -1;
 			for (int k = 0; k < nk; k++) {
 				const int ch = ic.input_channel + k;
 				if (ch >= p.in_nch_physical) continue;
+				// Fast path: input max code; slow path: the channel's own
+				// (src/imagew-main.c:307-383).
 				const double mcc = p.in_ch_slow[ic.input_channel] ? p.in_ch_maxcolorcode[ch] : p.in_maxcolorcode;
 				ic_lut[c][k] = get_lut(mcc, ic.in_cs);
 			}
@@ -631,8 +660,8 @@ int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
 	}
 	std::vector<QuantTable> qt(p.out_nch);
 	std::vector<PrngTables> pt(p.out_nch);
-	int n_ed = 0;
 	int ed_of[4] = {-1, -1, -1, -1};
+	J->n_ed = 0;
 	if (p.out_sampletype != IW_SAMPLETYPE_FLOATINGPOINT) {
 		for (int o = 0; o < p.out_nch; o++) {
 			qt[o] = build_quant_table(p.och[o], p.och[o].use_nc_table != 0);
@@ -640,22 +669,37 @@ int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
 				pt[o] = build_prng_tables(p.och[o].random_seed, p.out_width, p.out_height);
 				if (!pt[o].ok) {
 					iw_set_error(ctx, "Random seed not supported by the GPU path");
-					return 0;
+					return false;
 				}
 			}
-			if (p.och[o].ditherfamily == IW_DITHERFAMILY_ERRDIFF) ed_of[o] = n_ed++;
+			if (p.och[o].ditherfamily == IW_DITHERFAMILY_ERRDIFF) {
+				ed_of[o] = J->n_ed;
+				J->ed_list[J->n_ed++] = o;
+			}
 		}
 	}
 
-	// ---- device layout ----
+	// The kernels read logical rows/columns [start, start+size) of the input;
+	// refuse plans that would reach outside it (the reference's crop clamp can
+	// leave a non-positive size, src/imagew-main.c:1949-1958).
+	if (p.input_w < 1 || p.input_h < 1 || p.input_start_x < 0 || p.input_start_y < 0 ||
+		p.input_start_x + p.input_w > p.in_width || p.input_start_y + p.input_h > p.in_height) {
+		iw_set_error(ctx, "Image too large to process");
+		return false;
+	}
+	// Rows of the physical buffer: a transposing orientation swaps the
+	// logical width and height (src/imagew-api.c:660-684).
+	const int phys_rows = (p.orient_transform & 4) ? p.in_width : p.in_height;
+
+	// ---- device layout: one block per job ----
 	Layout L;
-	const size_t in_bytes = p.in_bpr * (size_t)p.in_height;
+	const size_t in_bytes = p.in_bpr * (size_t)phys_rows;
 	const size_t o_in = L.add<uint8_t>(in_bytes);
 	const size_t o_inter = L.add<float>((size_t)p.inter_nch * p.inter_height * p.inter_width);
-	const size_t out_bytes = p.out_bpr * (size_t)p.out_height;
-	const size_t o_out = L.add<uint8_t>(out_bytes);
-	const size_t o_eds = L.add<double>((size_t)n_ed * p.out_height * p.out_width);
-	const size_t o_ede = L.add<double>((size_t)n_ed * 3 * p.out_width);
+	J->out_bytes = p.out_bpr * (size_t)p.out_height;
+	J->out_off = L.add<uint8_t>(J->out_bytes);
+	const size_t o_eds = L.add<double>((size_t)J->n_ed * p.out_height * p.out_width);
+	const size_t o_ede = L.add<double>((size_t)J->n_ed * 3 * p.out_width);
 	std::vector<size_t> o_lut(luts.size());
 	for (size_t i = 0; i < luts.size(); i++) o_lut[i] = L.add<double>(luts[i].size());
 	struct WOff { size_t start, src, w, copy; };
@@ -681,46 +725,48 @@ int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
 		oq[o].prow = L.add<uint64_t>(pt[o].row_q.size());
 		oq[o].pcol = L.add<uint64_t>(pt[o].col_mont.size());
 	}
-	if (!R.ensure(L.off + 256)) {
+	J->block_bytes = L.off + 256;
+	J->block = dev_alloc(J->block_bytes);
+	if (!J->block) {
 		iw_set_error(ctx, "Out of device memory");
-		return 0;
+		return false;
 	}
-	char *base = (char *)R.arena;
-	hipStream_t st = R.stream;
-	auto up = [&](size_t off, const void *src, size_t bytes) -> bool {
-		if (!bytes) return true;
-		return hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
-	};
+	char *base = (char *)J->block;
+	hipStream_t st = J->stream;
 	bool ok = true;
-	hipEventRecord(R.ev[0], st);
-	ok &= up(o_in, p.in_pixels, in_bytes);
-	for (size_t i = 0; i < luts.size(); i++) ok &= up(o_lut[i], luts[i].data(), luts[i].size() * 8);
+	auto up = [&](size_t off, const void *src, size_t bytes) {
+		if (bytes) ok &= hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+	};
+	up(o_in, p.in_pixels, in_bytes);
+	for (size_t i = 0; i < luts.size(); i++) up(o_lut[i], luts[i].data(), luts[i].size() * 8);
 	auto up_w = [&](const WeightList &wl, const WOff &o) {
-		ok &= up(o.start, wl.start.data(), wl.start.size() * 4);
-		ok &= up(o.src, wl.src.data(), wl.src.size() * 4);
-		ok &= up(o.w, wl.w.data(), wl.w.size() * 8);
-		ok &= up(o.copy, wl.copy_src.data(), wl.copy_src.size() * 4);
+		up(o.start, wl.start.data(), wl.start.size() * 4);
+		up(o.src, wl.src.data(), wl.src.size() * 4);
+		up(o.w, wl.w.data(), wl.w.size() * 8);
+		up(o.copy, wl.copy_src.data(), wl.copy_src.size() * 4);
 	};
 	for (int c = 0; c < p.inter_nch; c++) {
 		up_w(wv[c], ov[c]);
 		up_w(wh[c], oh[c]);
 	}
 	for (int o = 0; o < (int)qt.size(); o++) {
-		ok &= up(oq[o].T, qt[o].T.data(), qt[o].T.size() * 8);
-		ok &= up(oq[o].U, qt[o].U.data(), qt[o].U.size() * 8);
-		ok &= up(oq[o].lin, qt[o].lin.data(), qt[o].lin.size() * 8);
-		ok &= up(oq[o].code, qt[o].code.data(), qt[o].code.size() * 8);
-		ok &= up(oq[o].nc, qt[o].nc.data(), qt[o].nc.size() * 8);
-		ok &= up(oq[o].prow, pt[o].row_q.data(), pt[o].row_q.size() * 8);
-		ok &= up(oq[o].pcol, pt[o].col_mont.data(), pt[o].col_mont.size() * 8);
+		up(oq[o].T, qt[o].T.data(), qt[o].T.size() * 8);
+		up(oq[o].U, qt[o].U.data(), qt[o].U.size() * 8);
+		up(oq[o].lin, qt[o].lin.data(), qt[o].lin.size() * 8);
+		up(oq[o].code, qt[o].code.data(), qt[o].code.size() * 8);
+		up(oq[o].nc, qt[o].nc.data(), qt[o].nc.size() * 8);
+		up(oq[o].prow, pt[o].row_q.data(), pt[o].row_q.size() * 8);
+		up(oq[o].pcol, pt[o].col_mont.data(), pt[o].col_mont.size() * 8);
 	}
+	// Host vectors are freed when this function returns: wait for the copies.
+	ok &= hipStreamSynchronize(st) == hipSuccess;
 	if (!ok) {
 		iw_set_error(ctx, "Host-to-device copy failed");
-		return 0;
+		return false;
 	}
 
 	// ---- kernel arguments ----
-	DevPlan D;
+	DevPlan &D = J->D;
 	memset(&D, 0, sizeof(D));
 	D.in = (const uint8_t *)(base + o_in);
 	D.in_bpr = p.in_bpr;
@@ -778,7 +824,7 @@ int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
 	D.out_float = p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT;
 	D.out_nch = p.out_nch;
 	D.out_bpr = p.out_bpr;
-	D.out = (uint8_t *)(base + o_out);
+	D.out = (uint8_t *)(base + J->out_off);
 	for (int o = 0; o < p.out_nch; o++) {
 		const iwamd_ochan &s = p.och[o];
 		DevOChan &d = D.oc[o];
@@ -802,7 +848,7 @@ int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
 	}
 	D.ed_samples = (double *)(base + o_eds);
 	D.ed_err = (double *)(base + o_ede);
-	D.n_ed = n_ed;
+	D.n_ed = J->n_ed;
 	D.apply_bkgd = p.apply_bkgd;
 	D.late = p.bkgd_strategy_late;
 	D.checker = p.bkgd_checkerboard;
@@ -813,40 +859,134 @@ int iwx_gpu_process(iw_context *ctx, const iwamd_plan *plan, uint8_t *img2)
 	D.bk1a = p.bkgd1alpha;
 	D.bk2a = p.bkgd2alpha;
 
-	// ---- launches ----
-	hipEventRecord(R.ev[1], st);
+	const double in_samples_bytes = (double)in_bytes;
+	J->out_pixels = (double)p.out_width * p.out_height;
+	J->alg_bytes = in_samples_bytes + (double)J->out_bytes;
+	return true;
+}
+
+}  // namespace
+
+extern "C" struct iwamd_job *iwamd_job_create(struct iw_context *ctx)
+{
+	iwamd_job *J = new iwamd_job();
+	J->ctx = ctx;
+	// Host decisions first, so their errors read as the reference's do.
+	if (!iwamd_prepare(ctx, &J->plan) || !runtime_init(ctx)) {
+		delete J;
+		return nullptr;
+	}
+	if (hipStreamCreateWithFlags(&J->stream, hipStreamNonBlocking) != hipSuccess) {
+		iw_set_error(ctx, "hipStreamCreate failed");
+		delete J;
+		return nullptr;
+	}
+	for (auto &e : J->ev) hipEventCreate(&e);
+	if (!job_upload(J)) {
+		iwamd_job_destroy(J);
+		return nullptr;
+	}
+	return J;
+}
+
+extern "C" int iwamd_job_run(struct iwamd_job *J)
+{
+	if (J->skip_pixels) return 1;
+	const DevPlan &D = J->D;
+	hipStream_t st = J->stream;
+	hipEventRecord(J->ev[0], st);
 	{
-		dim3 grid((p.inter_width + 255) / 256, p.inter_height);
+		dim3 grid((D.W + 255) / 256, D.Hp);
 		hipLaunchKernelGGL(vpass_kernel, grid, dim3(256), 0, st, D);
 	}
-	hipEventRecord(R.ev[2], st);
+	hipEventRecord(J->ev[1], st);
 	{
-		dim3 grid((p.out_width + 255) / 256, p.out_height);
+		dim3 grid((D.out_w + 255) / 256, D.out_h);
 		hipLaunchKernelGGL(hpass_kernel, grid, dim3(256), 0, st, D);
 	}
-	hipEventRecord(R.ev[3], st);
-	if (n_ed > 0) {
-		int ol[4] = {-1, -1, -1, -1};
-		for (int o = 0; o < p.out_nch; o++)
-			if (ed_of[o] >= 0) ol[ed_of[o]] = o;
-		hipLaunchKernelGGL(errdiff_serial_kernel, dim3(n_ed), dim3(64), 0, st, D, ol[0], ol[1], ol[2], ol[3]);
+	hipEventRecord(J->ev[2], st);
+	if (J->n_ed > 0) {
+		const int *l = J->ed_list;
+		hipLaunchKernelGGL(errdiff_serial_kernel, dim3(J->n_ed), dim3(64), 0, st, D, l[0], l[1], l[2], l[3]);
 	}
-	hipEventRecord(R.ev[4], st);
-	if (hipMemcpyAsync(img2, base + o_out, out_bytes, hipMemcpyDeviceToHost, st) != hipSuccess) {
-		iw_set_error(ctx, "Device-to-host copy failed");
+	hipEventRecord(J->ev[3], st);
+	if (hipGetLastError() != hipSuccess) {
+		iw_set_error(J->ctx, "Kernel launch failed");
 		return 0;
 	}
-	hipEventRecord(R.ev[5], st);
-	hipError_t e = hipStreamSynchronize(st);
+	J->ran = true;
+	return 1;
+}
+
+extern "C" int iwamd_job_sync(struct iwamd_job *J)
+{
+	hipError_t e = hipStreamSynchronize(J->stream);
 	if (e != hipSuccess) {
-		iw_set_errorf(ctx, "GPU pipeline failed: %s", hipGetErrorString(e));
+		iw_set_errorf(J->ctx, "GPU pipeline failed: %s", hipGetErrorString(e));
 		return 0;
-	}
-	ctx->num_last_timings = 5;
-	for (int k = 0; k < 5; k++) {
-		float ms = 0.0f;
-		hipEventElapsedTime(&ms, R.ev[k], R.ev[k + 1]);
-		ctx->last_timings_ms[k] = ms;
 	}
 	return 1;
+}
+
+extern "C" int iwamd_job_kernel_ms(struct iwamd_job *J, double *ms, int n)
+{
+	if (!J->ran) return 0;
+	int k = 0;
+	for (; k < 3 && k < n; k++) {
+		float t = 0.0f;
+		hipEventElapsedTime(&t, J->ev[k], J->ev[k + 1]);
+		ms[k] = t;
+	}
+	return k;
+}
+
+extern "C" void iwamd_job_sizes(struct iwamd_job *J, double *out_pixels, double *alg_bytes)
+{
+	*out_pixels = J->out_pixels;
+	*alg_bytes = J->alg_bytes;
+}
+
+extern "C" int iwamd_job_finish(struct iwamd_job *J)
+{
+	std::vector<uint8_t> host(J->out_bytes + 1, 0);
+	if (!J->skip_pixels) {
+		if (hipMemcpyAsync(host.data(), (char *)J->block + J->out_off, J->out_bytes, hipMemcpyDeviceToHost,
+				J->stream) != hipSuccess || !iwamd_job_sync(J)) {
+			iw_set_error(J->ctx, "Device-to-host copy failed");
+			return 0;
+		}
+	}
+	double ms[3];
+	int k = iwamd_job_kernel_ms(J, ms, 3);
+	J->ctx->num_last_timings = k;
+	for (int i = 0; i < k; i++) J->ctx->last_timings_ms[i] = ms[i];
+	return iwamd_finish(J->ctx, host.data());
+}
+
+extern "C" void iwamd_job_destroy(struct iwamd_job *J)
+{
+	if (!J) return;
+	if (J->stream) hipStreamSynchronize(J->stream);
+	dev_release(J->block, J->block_bytes);
+	for (auto &e : J->ev)
+		if (e) hipEventDestroy(e);
+	if (J->stream) hipStreamDestroy(J->stream);
+	delete J;
+}
+
+int iwx_gpu_device_count(void)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+	return n;
+}
+
+// iw_process_image's pixel work: one job, run once.
+int iwx_gpu_process(iw_context *ctx)
+{
+	iwamd_job *J = iwamd_job_create(ctx);
+	if (!J) return 0;
+	int ok = iwamd_job_run(J) && iwamd_job_finish(J);
+	iwamd_job_destroy(J);
+	return ok;
 }

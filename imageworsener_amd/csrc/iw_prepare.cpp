@@ -713,12 +713,9 @@ extern "C" int iwamd_finish(struct iw_context *ctx, const uint8_t *img2)
 // prepare -> HIP kernels -> finish (src/imagew-main.c:2250-2272).
 extern "C" int iw_process_image(struct iw_context *ctx)
 {
-	if (!begin_use(ctx)) return 0;
-	if (!iwx_prepare_processing(ctx, ctx->canvas_width, ctx->canvas_height)) return 0;
-	iwamd_plan plan;
-	iwx_fill_plan(ctx, &plan);
-	ctx->img2.pixels = (iw_byte *)iw_malloc_large(ctx, ctx->img2.bpr, ctx->img2.height);
-	if (!ctx->img2.pixels) return 0;
-	if (!iwx_gpu_process(ctx, &plan, ctx->img2.pixels)) return 0;
-	return iwx_finish(ctx);
+	if (ctx->use_count > 0) {
+		iw_set_error(ctx, "Internal: Incorrect attempt to reprocess image");
+		return 0;
+	}
+	return iwx_gpu_process(ctx);
 }

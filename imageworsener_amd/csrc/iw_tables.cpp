@@ -257,6 +257,8 @@ std::map<ThreshKey, std::pair<std::vector<double>, std::vector<double>>> g_thres
 
 // Thresholds depend only on the transfer function and P; cached per process
 // (a 16-bit table costs ~4M libm calls).
+}  // namespace
+
 const std::pair<std::vector<double>, std::vector<double>> &thresholds(const struct iw_csdescr &cs, int P)
 {
 	ThreshKey key = {cs.cstype, cs.cstype == IW_CSTYPE_GAMMA ? cs.gamma : 0.0, P};
@@ -276,8 +278,6 @@ const std::pair<std::vector<double>, std::vector<double>> &thresholds(const stru
 	U[P] = INFINITY;
 	return g_thresh_cache.emplace(key, std::make_pair(std::move(T), std::move(U))).first->second;
 }
-
-}  // namespace
 
 QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc)
 {
@@ -312,6 +312,19 @@ QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc)
 	}
 	return q;
 }
+
+}  // namespace iwx
+
+extern "C" int iwamd_quant_thresholds(int cstype, double gamma, int P, double *T, double *U)
+{
+	struct iw_csdescr cs = {cstype, 0, gamma};
+	const auto &tu = iwx::thresholds(cs, P);
+	for (int k = 0; k < P; k++) T[k] = tu.first[k];
+	for (int k = 0; k <= P; k++) U[k] = tu.second[k];
+	return 1;
+}
+
+namespace iwx {
 
 std::vector<double> build_input_lut(int nvalues, double maxcc, const struct iw_csdescr &cs)
 {

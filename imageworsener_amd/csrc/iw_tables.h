@@ -52,6 +52,9 @@ struct QuantTable {
 
 QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc);
 
+// (T, U) for a transfer function and level count; cached per process.
+const std::pair<std::vector<double>, std::vector<double>> &thresholds(const struct iw_csdescr &cs, int P);
+
 // Linearising lookup for integer input samples: lut[v] = X(v / maxcc).
 std::vector<double> build_input_lut(int nvalues, double maxcc, const struct iw_csdescr &cs);
 

@@ -132,6 +132,35 @@ IW_EXPORT(size_t) iwamd_plan_size(void);
 /* Precision form for this context (IWAMD_PRECISION_*). */
 IW_EXPORT(void) iwamd_set_precision(struct iw_context *ctx, int mode);
 
+/* The quantisation thresholds the GPU path uses in place of libm pow for an
+ * output channel with levels 0..P in colour space cstype (gamma for
+ * IW_CSTYPE_GAMMA): T[k-1] = least s in [0,1] whose level is >= k (k=1..P),
+ * U[k] = least s whose scaled value exceeds k (k=0..P-1), U[P] = +inf.
+ * Fills T (P entries) and U (P+1 entries); returns 1. */
+IW_EXPORT(int) iwamd_quant_thresholds(int cstype, double gamma, int P, double *T, double *U);
+
+/* Device-resident jobs.  iwamd_job_create() runs the host decisions on ctx
+ * (consuming its single use), builds the device tables and uploads the input
+ * image once; iwamd_job_run() launches the pixel kernels on the job's stream
+ * over the resident data (repeatable: every run rewrites the output);
+ * iwamd_job_sync() waits; iwamd_job_finish() copies the output back and
+ * completes ctx so iw_get_output_image() works.  iwamd_job_destroy() frees
+ * device memory (it does not destroy ctx).  Returns 0 / NULL on failure with
+ * ctx's error flag set. */
+struct iwamd_job;
+IW_EXPORT(struct iwamd_job*) iwamd_job_create(struct iw_context *ctx);
+IW_EXPORT(int) iwamd_job_run(struct iwamd_job *job);
+IW_EXPORT(int) iwamd_job_sync(struct iwamd_job *job);
+IW_EXPORT(int) iwamd_job_finish(struct iwamd_job *job);
+IW_EXPORT(void) iwamd_job_destroy(struct iwamd_job *job);
+/* Kernel times of the last run, ms, from HIP events on the job's stream:
+ * [0] vertical pass, [1] horizontal pass + quantise, [2] error diffusion.
+ * Returns the number of entries filled (at most n). */
+IW_EXPORT(int) iwamd_job_kernel_ms(struct iwamd_job *job, double *ms, int n);
+/* Output samples (pixels) and algorithmic HBM bytes (input + output image)
+ * of one run. */
+IW_EXPORT(void) iwamd_job_sizes(struct iwamd_job *job, double *out_pixels, double *alg_bytes);
+
 /* Library/device information.  Returns the number of visible HIP devices
  * (0 when none), and fills name (may be NULL). */
 IW_EXPORT(int) iwamd_device_count(void);
