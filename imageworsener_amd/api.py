@@ -277,9 +277,9 @@ class DeviceJob:
 
     def __init__(self, lib, ctx, handle):
         self.lib, self.ctx, self.h = lib, ctx, handle
-        sizes = (ctypes.c_double * 2)()
-        lib.dll.iwamd_job_sizes(handle, ctypes.byref(sizes, 0), ctypes.byref(sizes, 8))
-        self.out_pixels, self.alg_bytes = sizes[0], sizes[1]
+        px, nb = ctypes.c_double(), ctypes.c_double()
+        lib.dll.iwamd_job_sizes(handle, ctypes.byref(px), ctypes.byref(nb))
+        self.out_pixels, self.alg_bytes = px.value, nb.value
 
     def run(self):
         if not self.lib.dll.iwamd_job_run(self.h):

@@ -22,6 +22,7 @@
 #include "iw_ctx.h"
 #include "iw_tables.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -57,6 +58,7 @@ struct DevOChan {
 	int family, subtype, qmode, P, is_alpha, ed_index;
 	const double *T, *U, *lin, *code, *nc;
 	double bk1, bk2;
+	double ed_halfgap;  // bound on |error| of one pixel: half the widest level gap
 	const uint64_t *prng_row, *prng_col;
 };
 
@@ -481,6 +483,208 @@ __global__ void errdiff_serial_kernel(DevPlan P, int o_list0, int o_list1, int o
 	}
 }
 
+// Wave-cooperative error diffusion: one 64-lane wave per plane.
+//
+// The plane is one dependency chain in serpentine order, so each row is
+//   1. staged (all lanes): the row's linear samples b and accumulated errors
+//      E from earlier rows go to LDS in scan order, and for every pixel the
+//      level range [klo, khi] its sample can reach is found from bounds on
+//      the in-row carries (|err| <= half the widest level gap, times the
+//      x+1 / x+2 weights; rounding is monotone, so the bounds are exact);
+//   2. walked (lane 0): sample + carries, clamp, at most a compare or two to
+//      pick the level, the reference's floor/ceil decision, the error; the
+//      next pixel's operands are loaded while the current one computes;
+//   3. spread (all lanes): the errors reach the next two rows as an ordered
+//      gather -- for each target cell, the sources in the row's scan order --
+//      which adds the same terms in the same order as the reference's
+//      per-pixel scatter (src/imagew-main.c:497-567), so the bits agree.
+// Quantisation tables live in LDS when they fit.
+
+#define ED_CHUNK 1024
+#define ED_LDS_TABLE 1024
+
+struct EdLds {
+	double b[ED_CHUNK], e[ED_CHUNK], err[ED_CHUNK];
+	unsigned short klo[ED_CHUNK], khi[ED_CHUNK], code[ED_CHUNK];
+	unsigned char has[ED_CHUNK];
+	double T[ED_LDS_TABLE], U[ED_LDS_TABLE + 1], lin[ED_LDS_TABLE + 1], cd[ED_LDS_TABLE + 1];
+};
+
+__device__ __forceinline__ int level_of(const double *T, int P, double s)
+{
+	int lo = 0, n = P;
+	while (n > 0) {
+		const int half = n >> 1;
+		if (T[lo + half] <= s) {
+			lo += half + 1;
+			n -= half + 1;
+		} else {
+			n = half;
+		}
+	}
+	return lo;
+}
+
+__global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0, int o_list1, int o_list2,
+	int o_list3)
+{
+	__shared__ EdLds S;
+	const int lane = threadIdx.x;
+	const int olist[4] = {o_list0, o_list1, o_list2, o_list3};
+	const int o = olist[blockIdx.x];
+	if (o < 0) return;
+	const DevOChan &q = P.oc[o];
+	const double *m = c_ed_matrix[q.subtype <= 7 ? q.subtype : 0];
+	const int W = P.out_w, H = P.out_h, Pq = q.P;
+	double *E = P.ed_err + ((size_t)q.ed_index * 4 + 0) * W;
+	double *R1 = P.ed_err + ((size_t)q.ed_index * 4 + 1) * W;
+	double *ERR = P.ed_err + ((size_t)q.ed_index * 4 + 2) * W;
+	unsigned char *HAS = (unsigned char *)(P.ed_err + ((size_t)q.ed_index * 4 + 3) * W);
+	const double *samp = P.ed_samples + (size_t)q.ed_index * H * W;
+
+	// Tables: LDS copies when small enough, else global.
+	const bool lds_tab = Pq <= ED_LDS_TABLE;
+	if (lds_tab) {
+		for (int k = lane; k <= Pq; k += 64) {
+			if (k < Pq) S.T[k] = q.T[k];
+			S.U[k] = q.U[k];
+			S.lin[k] = q.lin[k];
+			S.cd[k] = q.code[k];
+		}
+	}
+	const double *T = lds_tab ? S.T : q.T;
+	const double *U = lds_tab ? S.U : q.U;
+	const double *lin = lds_tab ? S.lin : q.lin;
+	const double *cd = lds_tab ? S.cd : q.code;
+	for (int x = lane; x < W; x += 64) {
+		E[x] = 0.0;
+		R1[x] = 0.0;
+	}
+	__syncthreads();
+
+	// Carry bounds (same for every pixel).
+	const double B1 = q.ed_halfgap * m[0], B2 = q.ed_halfgap * m[1];
+
+	for (int j = 0; j < H; j++) {
+		const bool rev = (j % 2) != 0;
+		const int fwd = rev ? -1 : 1;
+		double e1 = 0.0, e2 = 0.0;   // errors of the previous two pixels (lane 0)
+		bool h1 = false, h2 = false;
+		for (int z0 = 0; z0 < W; z0 += ED_CHUNK) {
+			const int n = min(ED_CHUNK, W - z0);
+			// 1. stage
+			for (int i = lane; i < n; i += 64) {
+				const int x = rev ? W - 1 - (z0 + i) : z0 + i;
+				const double b = samp[(size_t)j * W + x];
+				const double e = E[x];
+				S.b[i] = b;
+				S.e[i] = e;
+				double lo = b + ((e - B2) - B1), hi = b + ((e + B2) + B1);
+				lo = clamp01(lo);
+				hi = clamp01(hi);
+				S.klo[i] = (unsigned short)level_of(T, Pq, lo);
+				S.khi[i] = (unsigned short)level_of(T, Pq, hi);
+			}
+			__syncthreads();
+			// 2. walk
+			if (lane == 0) {
+				int klo = S.klo[0], khi = S.khi[0];
+				double b = S.b[0], e = S.e[0];
+				double tn = (klo < Pq) ? T[klo] : 2.0;
+				double u0 = U[klo], u1 = U[min(klo + 1, Pq)];
+				double l0 = lin[klo], l1 = lin[min(klo + 1, Pq)], l2 = lin[min(klo + 2, Pq)];
+				double c0 = cd[klo], c1 = cd[min(klo + 1, Pq)], c2 = cd[min(klo + 2, Pq)];
+				for (int i = 0; i < n; i++) {
+					// Operands of the next pixel, loaded ahead.
+					const int ni = (i + 1 < n) ? i + 1 : i;
+					const int nklo = S.klo[ni], nkhi = S.khi[ni];
+					const double nb = S.b[ni], ne = S.e[ni];
+					const double ntn = (nklo < Pq) ? T[nklo] : 2.0;
+					const double nu0 = U[nklo], nu1 = U[min(nklo + 1, Pq)];
+					const double nl0 = lin[nklo], nl1 = lin[min(nklo + 1, Pq)], nl2 = lin[min(nklo + 2, Pq)];
+					const double nc0 = cd[nklo], nc1 = cd[min(nklo + 1, Pq)], nc2 = cd[min(nklo + 2, Pq)];
+
+					double cell = e;
+					if (h2) cell = cell + e2 * m[1];
+					if (h1) cell = cell + e1 * m[0];
+					double s = clamp01(b);
+					s = s + cell;
+					if (s > 1.0) s = 1.0;
+					else if (s < 0.0) s = 0.0;
+					int k = klo;
+					double uu = u0, lo_ = l0, hi_ = l1, cl = c0, ch = c1;
+					if (khi > klo) {
+						if (tn <= s) {
+							k = klo + 1;
+							uu = u1;
+							lo_ = l1;
+							hi_ = l2;
+							cl = c1;
+							ch = c2;
+							while (k < khi && T[k] <= s) k++;
+							if (k > klo + 1) {
+								uu = U[k];
+								lo_ = lin[k];
+								hi_ = lin[min(k + 1, Pq)];
+								cl = cd[k];
+								ch = cd[min(k + 1, Pq)];
+							}
+						}
+					}
+					double err = 0.0, code;
+					bool has = false;
+					if (k >= Pq || s < uu) {
+						code = cl;
+					} else {
+						const double d_floor = s - lo_;
+						const double d_ceil = hi_ - s;
+						if (d_ceil <= d_floor) {
+							err = -d_ceil;
+							code = ch;
+						} else {
+							err = d_floor;
+							code = cl;
+						}
+						has = true;
+					}
+					S.err[i] = err;
+					S.has[i] = has;
+					S.code[i] = (unsigned short)code;
+					e2 = e1; h2 = h1;
+					e1 = err; h1 = has;
+					klo = nklo; khi = nkhi; b = nb; e = ne; tn = ntn;
+					u0 = nu0; u1 = nu1; l0 = nl0; l1 = nl1; l2 = nl2; c0 = nc0; c1 = nc1; c2 = nc2;
+				}
+			}
+			__syncthreads();
+			// Results of the chunk back to x order.
+			for (int i = lane; i < n; i += 64) {
+				const int x = rev ? W - 1 - (z0 + i) : z0 + i;
+				ERR[x] = S.err[i];
+				HAS[x] = S.has[i];
+				store_sample(P, x, j, o, (double)S.code[i]);
+			}
+			__syncthreads();
+		}
+		// 3. spread to the next two rows.  Target t receives from sources
+		// t-2f, t-f, t, t+f, t+2f (scan order) with next-row weights
+		// m[6], m[5], m[4], m[3], m[2] and row-after weights m[11..7].
+		for (int t = lane; t < W; t += 64) {
+			double a1 = R1[t], a2 = 0.0;
+			for (int d = -2; d <= 2; d++) {
+				const int p = t + d * fwd;
+				if (p < 0 || p >= W || !HAS[p]) continue;
+				const double er = ERR[p];
+				a1 = a1 + er * m[4 - d];
+				a2 = a2 + er * m[9 - d];
+			}
+			E[t] = a1;
+			R1[t] = a2;
+		}
+		__syncthreads();
+	}
+}
+
 // ---- device memory and stream -------------------------------------------
 
 struct Runtime {
@@ -699,7 +903,7 @@ bool job_upload(iwamd_job *J)
 	J->out_bytes = p.out_bpr * (size_t)p.out_height;
 	J->out_off = L.add<uint8_t>(J->out_bytes);
 	const size_t o_eds = L.add<double>((size_t)J->n_ed * p.out_height * p.out_width);
-	const size_t o_ede = L.add<double>((size_t)J->n_ed * 3 * p.out_width);
+	const size_t o_ede = L.add<double>((size_t)J->n_ed * 4 * p.out_width);
 	std::vector<size_t> o_lut(luts.size());
 	for (size_t i = 0; i < luts.size(); i++) o_lut[i] = L.add<double>(luts[i].size());
 	struct WOff { size_t start, src, w, copy; };
@@ -844,6 +1048,19 @@ bool job_upload(iwamd_job *J)
 			d.nc = (const double *)(base + oq[o].nc);
 			d.prng_row = (const uint64_t *)(base + oq[o].prow);
 			d.prng_col = (const uint64_t *)(base + oq[o].pcol);
+			// Bound on |error| of one pixel.  A non-exact sample at level k
+			// lies in [U[k], T[k]) and its error is the smaller distance to
+			// lin[k], lin[k+1]: at most half their gap, or the overshoot of
+			// the interval past either candidate.  Widened for the rounding of
+			// the differences and of the products that carry the error.
+			double bound = 0.0;
+			const QuantTable &Q = qt[o];
+			for (int k = 0; k < Q.P; k++) {
+				bound = std::max(bound, (Q.lin[k + 1] - Q.lin[k]) * 0.5);
+				bound = std::max(bound, Q.lin[k] - Q.U[k]);
+				bound = std::max(bound, Q.T[k] - Q.lin[k + 1]);
+			}
+			d.ed_halfgap = bound * (1.0 + 1e-6) + 1e-300;
 		}
 	}
 	D.ed_samples = (double *)(base + o_eds);
@@ -907,7 +1124,7 @@ extern "C" int iwamd_job_run(struct iwamd_job *J)
 	hipEventRecord(J->ev[2], st);
 	if (J->n_ed > 0) {
 		const int *l = J->ed_list;
-		hipLaunchKernelGGL(errdiff_serial_kernel, dim3(J->n_ed), dim3(64), 0, st, D, l[0], l[1], l[2], l[3]);
+		hipLaunchKernelGGL(errdiff_wave_kernel, dim3(J->n_ed), dim3(64), 0, st, D, l[0], l[1], l[2], l[3]);
 	}
 	hipEventRecord(J->ev[3], st);
 	if (hipGetLastError() != hipSuccess) {

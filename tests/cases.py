@@ -57,6 +57,16 @@ def sweep_jobs():
         jobs.append(("g16-depth16-" + d, api.image_job(synth.be16(g16), 64, 64, GRAY, 16, canvas=(45, 77), depth=16,
                                                        resize=api.resize_both("lanczos3"),
                                                        dither=[(api.CH_ALL, fam, sub)], random_seed=-5)))
+    # Error diffusion across rows wider than one LDS chunk, and 16-bit
+    # output (tables too large for LDS).
+    wide = synth.c2(160)
+    for d in ("f", "jjn", "atkinson", "sierra"):
+        fam, sub = api.DITHER[d]
+        jobs.append(("wide-ed-" + d, api.image_job(synth.be16(wide), 160, 160, GRAY, 16, canvas=(2600, 23),
+                                                   resize=api.resize_both("catrom"),
+                                                   dither=[(api.CH_ALL, fam, sub)], color_count=[(api.CH_ALL, 5)])))
+    jobs.append(("wide-ed-rgba16", api.image_job(rgba, 83, 71, RGBA, 8, canvas=(1300, 17), depth=16,
+                                                 resize=api.resize_both("lanczos2"), dither=[(api.CH_ALL, 2, 0)])))
     jobs.append(("rgba-gray", api.image_job(rgba, 83, 71, RGBA, 8, canvas=(50, 50), resize=api.resize_both("catrom"),
                                             values=[(V_CVT_GRAY, 1)])))
     jobs.append(("rgba-bkgd", api.image_job(rgba, 83, 71, RGBA, 8, canvas=(50, 50), resize=api.resize_both("catrom"),
