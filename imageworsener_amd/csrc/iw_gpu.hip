@@ -486,27 +486,31 @@ __global__ void errdiff_serial_kernel(DevPlan P, int o_list0, int o_list1, int o
 // Wave-cooperative error diffusion: one 64-lane wave per plane.
 //
 // The plane is one dependency chain in serpentine order, so each row is
-//   1. staged (all lanes): the row's linear samples b and accumulated errors
-//      E from earlier rows go to LDS in scan order, and for every pixel the
-//      level range [klo, khi] its sample can reach is found from bounds on
-//      the in-row carries (|err| <= half the widest level gap, times the
-//      x+1 / x+2 weights; rounding is monotone, so the bounds are exact);
-//   2. walked (lane 0): sample + carries, clamp, at most a compare or two to
-//      pick the level, the reference's floor/ceil decision, the error; the
-//      next pixel's operands are loaded while the current one computes;
+//   1. staged (all lanes): for every pixel of a chunk of the row, in scan
+//      order, LDS receives its linear sample b, the error E accumulated from
+//      earlier rows, and the candidate quantisation data for the one or two
+//      levels its sample can reach -- found from bounds on the in-row carries
+//      (|err| is bounded per channel; rounding is monotone, so the bounds
+//      hold exactly);
+//   2. walked (lane 0): cell = (E + e[x-2]*m1) + e[x-1]*m0, s = clamp(b +
+//      cell), one compare to pick the level, the reference's floor/ceil
+//      decision and the error; the next pixel's operands are loaded while
+//      the current one computes;
 //   3. spread (all lanes): the errors reach the next two rows as an ordered
 //      gather -- for each target cell, the sources in the row's scan order --
 //      which adds the same terms in the same order as the reference's
-//      per-pixel scatter (src/imagew-main.c:497-567), so the bits agree.
-// Quantisation tables live in LDS when they fit.
+//      per-pixel scatter (src/imagew-main.c:497-567).
+// An exact pixel (no error in the reference) carries +0.0 here: adding a
+// signed zero to an accumulator that is never -0.0 leaves it unchanged, so
+// the bits agree without per-pixel flags.
 
-#define ED_CHUNK 1024
-#define ED_LDS_TABLE 1024
+#define ED_CHUNK 512
+#define ED_LDS_TABLE 256
 
 struct EdLds {
-	double b[ED_CHUNK], e[ED_CHUNK], err[ED_CHUNK];
-	unsigned short klo[ED_CHUNK], khi[ED_CHUNK], code[ED_CHUNK];
-	unsigned char has[ED_CHUNK];
+	double b[ED_CHUNK], e[ED_CHUNK], tn[ED_CHUNK], u0[ED_CHUNK], u1[ED_CHUNK];
+	double l0[ED_CHUNK], l1[ED_CHUNK], l2[ED_CHUNK], err[ED_CHUNK];
+	unsigned short klo[ED_CHUNK], khi[ED_CHUNK], c0[ED_CHUNK], c1[ED_CHUNK], c2[ED_CHUNK], code[ED_CHUNK];
 	double T[ED_LDS_TABLE], U[ED_LDS_TABLE + 1], lin[ED_LDS_TABLE + 1], cd[ED_LDS_TABLE + 1];
 };
 
@@ -525,6 +529,7 @@ __device__ __forceinline__ int level_of(const double *T, int P, double s)
 	return lo;
 }
 
+template <bool LDS_TAB>
 __global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0, int o_list1, int o_list2,
 	int o_list3)
 {
@@ -535,16 +540,14 @@ __global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0
 	if (o < 0) return;
 	const DevOChan &q = P.oc[o];
 	const double *m = c_ed_matrix[q.subtype <= 7 ? q.subtype : 0];
+	const double m0 = m[0], m1 = m[1];
 	const int W = P.out_w, H = P.out_h, Pq = q.P;
 	double *E = P.ed_err + ((size_t)q.ed_index * 4 + 0) * W;
 	double *R1 = P.ed_err + ((size_t)q.ed_index * 4 + 1) * W;
 	double *ERR = P.ed_err + ((size_t)q.ed_index * 4 + 2) * W;
-	unsigned char *HAS = (unsigned char *)(P.ed_err + ((size_t)q.ed_index * 4 + 3) * W);
 	const double *samp = P.ed_samples + (size_t)q.ed_index * H * W;
 
-	// Tables: LDS copies when small enough, else global.
-	const bool lds_tab = Pq <= ED_LDS_TABLE;
-	if (lds_tab) {
+	if (LDS_TAB) {
 		for (int k = lane; k <= Pq; k += 64) {
 			if (k < Pq) S.T[k] = q.T[k];
 			S.U[k] = q.U[k];
@@ -552,24 +555,22 @@ __global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0
 			S.cd[k] = q.code[k];
 		}
 	}
-	const double *T = lds_tab ? S.T : q.T;
-	const double *U = lds_tab ? S.U : q.U;
-	const double *lin = lds_tab ? S.lin : q.lin;
-	const double *cd = lds_tab ? S.cd : q.code;
+	const double *T = LDS_TAB ? S.T : q.T;
+	const double *U = LDS_TAB ? S.U : q.U;
+	const double *lin = LDS_TAB ? S.lin : q.lin;
+	const double *cd = LDS_TAB ? S.cd : q.code;
 	for (int x = lane; x < W; x += 64) {
 		E[x] = 0.0;
 		R1[x] = 0.0;
 	}
 	__syncthreads();
 
-	// Carry bounds (same for every pixel).
-	const double B1 = q.ed_halfgap * m[0], B2 = q.ed_halfgap * m[1];
+	const double B1 = q.ed_halfgap * m0, B2 = q.ed_halfgap * m1;
 
 	for (int j = 0; j < H; j++) {
 		const bool rev = (j % 2) != 0;
 		const int fwd = rev ? -1 : 1;
-		double e1 = 0.0, e2 = 0.0;   // errors of the previous two pixels (lane 0)
-		bool h1 = false, h2 = false;
+		double e1 = 0.0, e2 = 0.0;  // errors of the previous two pixels (lane 0)
 		for (int z0 = 0; z0 < W; z0 += ED_CHUNK) {
 			const int n = min(ED_CHUNK, W - z0);
 			// 1. stage
@@ -577,83 +578,65 @@ __global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0
 				const int x = rev ? W - 1 - (z0 + i) : z0 + i;
 				const double b = samp[(size_t)j * W + x];
 				const double e = E[x];
+				const double lo = clamp01(b + ((e - B2) - B1)), hi = clamp01(b + ((e + B2) + B1));
+				const int klo = level_of(T, Pq, lo), khi = level_of(T, Pq, hi);
+				const int k1 = min(klo + 1, Pq), k2 = min(klo + 2, Pq);
 				S.b[i] = b;
 				S.e[i] = e;
-				double lo = b + ((e - B2) - B1), hi = b + ((e + B2) + B1);
-				lo = clamp01(lo);
-				hi = clamp01(hi);
-				S.klo[i] = (unsigned short)level_of(T, Pq, lo);
-				S.khi[i] = (unsigned short)level_of(T, Pq, hi);
+				S.klo[i] = (unsigned short)klo;
+				S.khi[i] = (unsigned short)khi;
+				S.tn[i] = (khi > klo) ? T[klo] : 2.0;
+				S.u0[i] = U[klo];
+				S.u1[i] = U[k1];
+				S.l0[i] = lin[klo];
+				S.l1[i] = lin[k1];
+				S.l2[i] = lin[k2];
+				S.c0[i] = (unsigned short)cd[klo];
+				S.c1[i] = (unsigned short)cd[k1];
+				S.c2[i] = (unsigned short)cd[k2];
 			}
 			__syncthreads();
 			// 2. walk
 			if (lane == 0) {
-				int klo = S.klo[0], khi = S.khi[0];
-				double b = S.b[0], e = S.e[0];
-				double tn = (klo < Pq) ? T[klo] : 2.0;
-				double u0 = U[klo], u1 = U[min(klo + 1, Pq)];
-				double l0 = lin[klo], l1 = lin[min(klo + 1, Pq)], l2 = lin[min(klo + 2, Pq)];
-				double c0 = cd[klo], c1 = cd[min(klo + 1, Pq)], c2 = cd[min(klo + 2, Pq)];
+				double b = S.b[0], e = S.e[0], tn = S.tn[0], u0 = S.u0[0], u1 = S.u1[0];
+				double l0 = S.l0[0], l1 = S.l1[0], l2 = S.l2[0];
+				int klo = S.klo[0], khi = S.khi[0], c0 = S.c0[0], c1 = S.c1[0], c2 = S.c2[0];
 				for (int i = 0; i < n; i++) {
-					// Operands of the next pixel, loaded ahead.
 					const int ni = (i + 1 < n) ? i + 1 : i;
-					const int nklo = S.klo[ni], nkhi = S.khi[ni];
-					const double nb = S.b[ni], ne = S.e[ni];
-					const double ntn = (nklo < Pq) ? T[nklo] : 2.0;
-					const double nu0 = U[nklo], nu1 = U[min(nklo + 1, Pq)];
-					const double nl0 = lin[nklo], nl1 = lin[min(nklo + 1, Pq)], nl2 = lin[min(nklo + 2, Pq)];
-					const double nc0 = cd[nklo], nc1 = cd[min(nklo + 1, Pq)], nc2 = cd[min(nklo + 2, Pq)];
+					const double nb = S.b[ni], ne = S.e[ni], ntn = S.tn[ni], nu0 = S.u0[ni], nu1 = S.u1[ni];
+					const double nl0 = S.l0[ni], nl1 = S.l1[ni], nl2 = S.l2[ni];
+					const int nklo = S.klo[ni], nkhi = S.khi[ni], nc0 = S.c0[ni], nc1 = S.c1[ni], nc2 = S.c2[ni];
 
-					double cell = e;
-					if (h2) cell = cell + e2 * m[1];
-					if (h1) cell = cell + e1 * m[0];
-					double s = clamp01(b);
-					s = s + cell;
-					if (s > 1.0) s = 1.0;
-					else if (s < 0.0) s = 0.0;
-					int k = klo;
-					double uu = u0, lo_ = l0, hi_ = l1, cl = c0, ch = c1;
-					if (khi > klo) {
-						if (tn <= s) {
-							k = klo + 1;
-							uu = u1;
-							lo_ = l1;
-							hi_ = l2;
-							cl = c1;
-							ch = c2;
-							while (k < khi && T[k] <= s) k++;
-							if (k > klo + 1) {
-								uu = U[k];
-								lo_ = lin[k];
-								hi_ = lin[min(k + 1, Pq)];
-								cl = cd[k];
-								ch = cd[min(k + 1, Pq)];
-							}
-						}
+					const double cell = (e + e2 * m1) + e1 * m0;
+					double s = b + cell;
+					s = fmin(fmax(s, 0.0), 1.0);  // never NaN or -0.0: exact clamp
+					const bool up = tn <= s;
+					double uu = up ? u1 : u0, lo_ = up ? l1 : l0, hi_ = up ? l2 : l1;
+					int cl = up ? c1 : c0, ch = up ? c2 : c1;
+					if (up && khi > klo + 1) {
+						// Rare: more than two reachable levels.  Reads go to
+						// global memory so the join does not wait on the LDS
+						// loads already issued for the next pixel.
+						int k = klo + 1;
+						while (k < khi && q.T[k] <= s) k++;
+						uu = q.U[k];
+						lo_ = q.lin[k];
+						hi_ = q.lin[min(k + 1, Pq)];
+						cl = (int)q.code[k];
+						ch = (int)q.code[min(k + 1, Pq)];
 					}
-					double err = 0.0, code;
-					bool has = false;
-					if (k >= Pq || s < uu) {
-						code = cl;
-					} else {
-						const double d_floor = s - lo_;
-						const double d_ceil = hi_ - s;
-						if (d_ceil <= d_floor) {
-							err = -d_ceil;
-							code = ch;
-						} else {
-							err = d_floor;
-							code = cl;
-						}
-						has = true;
-					}
+					const double d_floor = s - lo_;
+					const double d_ceil = hi_ - s;
+					const bool exact = s < uu;
+					const bool ceil_ = d_ceil <= d_floor;
+					const double err = exact ? 0.0 : (ceil_ ? -d_ceil : d_floor);
+					const int code = (exact || !ceil_) ? cl : ch;
 					S.err[i] = err;
-					S.has[i] = has;
 					S.code[i] = (unsigned short)code;
-					e2 = e1; h2 = h1;
-					e1 = err; h1 = has;
-					klo = nklo; khi = nkhi; b = nb; e = ne; tn = ntn;
-					u0 = nu0; u1 = nu1; l0 = nl0; l1 = nl1; l2 = nl2; c0 = nc0; c1 = nc1; c2 = nc2;
+					e2 = e1;
+					e1 = err;
+					b = nb; e = ne; tn = ntn; u0 = nu0; u1 = nu1; l0 = nl0; l1 = nl1; l2 = nl2;
+					klo = nklo; khi = nkhi; c0 = nc0; c1 = nc1; c2 = nc2;
 				}
 			}
 			__syncthreads();
@@ -661,7 +644,6 @@ __global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0
 			for (int i = lane; i < n; i += 64) {
 				const int x = rev ? W - 1 - (z0 + i) : z0 + i;
 				ERR[x] = S.err[i];
-				HAS[x] = S.has[i];
 				store_sample(P, x, j, o, (double)S.code[i]);
 			}
 			__syncthreads();
@@ -673,7 +655,7 @@ __global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0
 			double a1 = R1[t], a2 = 0.0;
 			for (int d = -2; d <= 2; d++) {
 				const int p = t + d * fwd;
-				if (p < 0 || p >= W || !HAS[p]) continue;
+				if (p < 0 || p >= W) continue;
 				const double er = ERR[p];
 				a1 = a1 + er * m[4 - d];
 				a2 = a2 + er * m[9 - d];
@@ -1124,7 +1106,13 @@ extern "C" int iwamd_job_run(struct iwamd_job *J)
 	hipEventRecord(J->ev[2], st);
 	if (J->n_ed > 0) {
 		const int *l = J->ed_list;
-		hipLaunchKernelGGL(errdiff_wave_kernel, dim3(J->n_ed), dim3(64), 0, st, D, l[0], l[1], l[2], l[3]);
+		// Quantisation tables fit in LDS for up to 257 levels (8-bit output).
+		bool small = true;
+		for (int e = 0; e < J->n_ed; e++) small &= D.oc[l[e]].P <= ED_LDS_TABLE;
+		if (small)
+			hipLaunchKernelGGL(errdiff_wave_kernel<true>, dim3(J->n_ed), dim3(64), 0, st, D, l[0], l[1], l[2], l[3]);
+		else
+			hipLaunchKernelGGL(errdiff_wave_kernel<false>, dim3(J->n_ed), dim3(64), 0, st, D, l[0], l[1], l[2], l[3]);
 	}
 	hipEventRecord(J->ev[3], st);
 	if (hipGetLastError() != hipSuccess) {
