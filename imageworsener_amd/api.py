@@ -369,6 +369,7 @@ class _PlanLayout:
                         ("bkgd1alpha", c.c_double), ("bkgd2alpha", c.c_double), ("precision", c.c_int)]
 
         self.Plan = Plan
+        self.OChan = OChan
 
     def out_size(self, buf):
         p = self.Plan.from_buffer(buf)

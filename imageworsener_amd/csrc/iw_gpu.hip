@@ -57,8 +57,9 @@ enum { Q_THRESH = 0, Q_NC = 1 };
 struct DevOChan {
 	int family, subtype, qmode, P, is_alpha, ed_index;
 	const double *T, *U, *lin, *code, *nc;
+	const double *M, *ebpre, *ebsuf;  // error-diffusion decision data (EdTables)
+	double eb_max;
 	double bk1, bk2;
-	double ed_halfgap;  // bound on |error| of one pixel: half the widest level gap
 	const uint64_t *prng_row, *prng_col;
 };
 
@@ -241,18 +242,6 @@ __constant__ float c_dither_pattern[2][64] = {
 	 10.5f/64, 28.5f/64, 36.5f/64, 44.5f/64, 42.5f/64, 34.5f/64, 22.5f/64, 4.5f/64,
 	 2.5f/64, 8.5f/64, 16.5f/64, 26.5f/64, 24.5f/64, 14.5f/64, 6.5f/64, 0.5f/64}};
 
-// Error-diffusion weights; slots 0,1: x+1,x+2 of this row; 2..6: x-2..x+2 of
-// the next row; 7..11: the row after (src/imagew-main.c:510-535).
-__constant__ double c_ed_matrix[8][12] = {
-	{7.0/16, 0.0, 0.0, 3.0/16, 5.0/16, 1.0/16, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},          // Floyd-Steinberg
-	{7.0/48, 5.0/48, 3.0/48, 5.0/48, 7.0/48, 5.0/48, 3.0/48, 1.0/48, 3.0/48, 5.0/48, 3.0/48, 1.0/48},  // JJN
-	{8.0/42, 4.0/42, 2.0/42, 4.0/42, 8.0/42, 4.0/42, 2.0/42, 1.0/42, 2.0/42, 4.0/42, 2.0/42, 1.0/42},  // Stucki
-	{8.0/32, 4.0/32, 2.0/32, 4.0/32, 8.0/32, 4.0/32, 2.0/32, 0.0, 0.0, 0.0, 0.0, 0.0},  // Burkes
-	{5.0/32, 3.0/32, 2.0/32, 4.0/32, 5.0/32, 4.0/32, 2.0/32, 0.0, 2.0/32, 3.0/32, 2.0/32, 0.0},  // Sierra3
-	{4.0/16, 3.0/16, 1.0/16, 2.0/16, 3.0/16, 2.0/16, 1.0/16, 0.0, 0.0, 0.0, 0.0, 0.0},  // Sierra2
-	{2.0/4, 0.0, 0.0, 1.0/4, 1.0/4, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                 // Sierra 4-2-1
-	{1.0/8, 1.0/8, 0.0, 1.0/8, 1.0/8, 1.0/8, 0.0, 0.0, 0.0, 1.0/8, 0.0, 0.0}};          // Atkinson
-
 // floor(e(s)): number of thresholds <= s.
 __device__ __forceinline__ int level_floor(const DevOChan &q, double s)
 {
@@ -410,136 +399,123 @@ __global__ void __launch_bounds__(256) hpass_kernel(DevPlan P)
 }
 
 // ---- error diffusion ------------------------------------------------------
-// One plane is one dependency chain (serpentine rows), so a plane is walked
-// by one thread; planes run in parallel.
-
-__global__ void errdiff_serial_kernel(DevPlan P, int o_list0, int o_list1, int o_list2, int o_list3)
-{
-	const int which = blockIdx.x;
-	const int olist[4] = {o_list0, o_list1, o_list2, o_list3};
-	const int o = olist[which];
-	if (threadIdx.x != 0 || o < 0) return;
-	const DevOChan &q = P.oc[o];
-	const double *m = c_ed_matrix[q.subtype <= 7 ? q.subtype : 0];
-	const int W = P.out_w;
-	double *rows[3];
-	for (int r = 0; r < 3; r++) rows[r] = P.ed_err + ((size_t)q.ed_index * 3 + r) * W;
-	for (int r = 0; r < 3; r++)
-		for (int x = 0; x < W; x++) rows[r][x] = 0.0;
-	const double *samp = P.ed_samples + (size_t)q.ed_index * P.out_h * W;
-	for (int j = 0; j < P.out_h; j++) {
-		const int fwd = (j % 2) ? -1 : 1;
-		for (int z = 0; z < W; z++) {
-			const int x = (j % 2) ? W - 1 - z : z;
-			double s = samp[(size_t)j * W + x];
-			s += rows[0][x];
-			if (s > 1.0) s = 1.0;
-			else if (s < 0.0) s = 0.0;
-			const int k = level_floor(q, s);
-			double s_full;
-			if (k >= q.P || s < q.U[k]) {
-				s_full = q.code[k];
-			} else {
-				const double d_floor = s - q.lin[k];
-				const double d_ceil = q.lin[k + 1] - s;
-				double err;
-				if (d_ceil <= d_floor) {
-					err = -d_ceil;
-					s_full = q.code[k + 1];
-				} else {
-					err = d_floor;
-					s_full = q.code[k];
-				}
-				const int xm1 = x - fwd, xm2 = x - 2 * fwd, xp1 = x + fwd, xp2 = x + 2 * fwd;
-				if (xm1 >= 0 && xm1 < W) {
-					if (xm2 >= 0 && xm2 < W) {
-						rows[1][xm2] += err * m[2];
-						rows[2][xm2] += err * m[7];
-					}
-					rows[1][xm1] += err * m[3];
-					rows[2][xm1] += err * m[8];
-				}
-				rows[1][x] += err * m[4];
-				rows[2][x] += err * m[9];
-				if (xp1 >= 0 && xp1 < W) {
-					rows[0][xp1] += err * m[0];
-					rows[1][xp1] += err * m[5];
-					rows[2][xp1] += err * m[10];
-					if (xp2 >= 0 && xp2 < W) {
-						rows[0][xp2] += err * m[1];
-						rows[1][xp2] += err * m[6];
-						rows[2][xp2] += err * m[11];
-					}
-				}
-			}
-			store_sample(P, x, j, o, s_full);
-		}
-		// Next row's errors become current; the last row starts at zero.
-		double *t = rows[0];
-		rows[0] = rows[1];
-		rows[1] = rows[2];
-		rows[2] = t;
-		for (int x = 0; x < W; x++) rows[2][x] = 0.0;
-	}
-}
-
-// Wave-cooperative error diffusion: one 64-lane wave per plane.
 //
-// The plane is one dependency chain in serpentine order, so each row is
-//   1. staged (all lanes): for every pixel of a chunk of the row, in scan
-//      order, LDS receives its linear sample b, the error E accumulated from
-//      earlier rows, and the candidate quantisation data for the one or two
-//      levels its sample can reach -- found from bounds on the in-row carries
-//      (|err| is bounded per channel; rounding is monotone, so the bounds
-//      hold exactly);
-//   2. walked (lane 0): cell = (E + e[x-2]*m1) + e[x-1]*m0, s = clamp(b +
-//      cell), one compare to pick the level, the reference's floor/ceil
-//      decision and the error; the next pixel's operands are loaded while
-//      the current one computes;
-//   3. spread (all lanes): the errors reach the next two rows as an ordered
-//      gather -- for each target cell, the sources in the row's scan order --
-//      which adds the same terms in the same order as the reference's
-//      per-pixel scatter (src/imagew-main.c:497-567).
-// An exact pixel (no error in the reference) carries +0.0 here: adding a
-// signed zero to an accumulator that is never -0.0 leaves it unchanged, so
-// the bits agree without per-pixel flags.
+// A plane is one dependency chain: rows alternate direction (src/imagew-main.c:
+// 1072-1077), so every pixel waits for the one before it in scan order and a
+// row cannot start before the previous row has finished.  One workgroup runs
+// one plane; per chunk of a row:
+//   1. stage (all threads, one pixel each): from the sample b, the error E
+//      carried from earlier rows and a bound on the two in-row carries, the
+//      one or two levels the pixel can reach and their decision data are
+//      written to LDS.  The bound is rigorous: |error| of a pixel is at most
+//      the largest eb of the levels its own sample can reach under the global
+//      bound, and rounding is monotone, so the window always holds s.  A
+//      pixel whose window spans three or more levels is flagged "wide";
+//   2. walk (one lane): cell = (E + e[x-2]*m1) + e[x-1]*m0, s = clamp(b +
+//      cell), pick the level with one compare, the reference's floor/ceil
+//      choice and error with compares and selects; only s is stored.  Wide
+//      pixels take a table search;
+//   3. finish (all threads): the code and error of every pixel are
+//      recomputed from s with the same formulas (so they are the walk's
+//      bits), the output is stored and the errors kept for the next rows;
+// and after the row, the errors reach the next two rows as an ordered gather
+// -- for each target cell, the sources in the row's scan order -- which adds
+// the same terms in the same order as the reference's per-pixel scatter
+// (src/imagew-main.c:497-567).  An exact pixel (no error in the reference)
+// carries +0.0 here: adding a signed zero to an accumulator that is never
+// -0.0 leaves it unchanged, so the bits agree without per-pixel flags.
 
-#define ED_CHUNK 512
-#define ED_LDS_TABLE 256
+// Error-diffusion weights; slots 0,1: x+1,x+2 of this row; 2..6: x-2..x+2 of
+// the next row; 7..11: the row after (src/imagew-main.c:510-535).
+__constant__ double c_ed_matrix[8][12] = {
+	{7.0/16, 0.0, 0.0, 3.0/16, 5.0/16, 1.0/16, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},          // Floyd-Steinberg
+	{7.0/48, 5.0/48, 3.0/48, 5.0/48, 7.0/48, 5.0/48, 3.0/48, 1.0/48, 3.0/48, 5.0/48, 3.0/48, 1.0/48},  // JJN
+	{8.0/42, 4.0/42, 2.0/42, 4.0/42, 8.0/42, 4.0/42, 2.0/42, 1.0/42, 2.0/42, 4.0/42, 2.0/42, 1.0/42},  // Stucki
+	{8.0/32, 4.0/32, 2.0/32, 4.0/32, 8.0/32, 4.0/32, 2.0/32, 0.0, 0.0, 0.0, 0.0, 0.0},  // Burkes
+	{5.0/32, 3.0/32, 2.0/32, 4.0/32, 5.0/32, 4.0/32, 2.0/32, 0.0, 2.0/32, 3.0/32, 2.0/32, 0.0},  // Sierra3
+	{4.0/16, 3.0/16, 1.0/16, 2.0/16, 3.0/16, 2.0/16, 1.0/16, 0.0, 0.0, 0.0, 0.0, 0.0},  // Sierra2
+	{2.0/4, 0.0, 0.0, 1.0/4, 1.0/4, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                 // Sierra 4-2-1
+	{1.0/8, 1.0/8, 0.0, 1.0/8, 1.0/8, 1.0/8, 0.0, 0.0, 0.0, 1.0/8, 0.0, 0.0}};          // Atkinson
 
-struct EdLds {
-	double b[ED_CHUNK], e[ED_CHUNK], tn[ED_CHUNK], u0[ED_CHUNK], u1[ED_CHUNK];
-	double l0[ED_CHUNK], l1[ED_CHUNK], l2[ED_CHUNK], err[ED_CHUNK];
-	unsigned short klo[ED_CHUNK], khi[ED_CHUNK], c0[ED_CHUNK], c1[ED_CHUNK], c2[ED_CHUNK], code[ED_CHUNK];
-	double T[ED_LDS_TABLE], U[ED_LDS_TABLE + 1], lin[ED_LDS_TABLE + 1], cd[ED_LDS_TABLE + 1];
+#define ED_CHUNK 256      // pixels per stage/walk/finish round; = threads per workgroup
+#define ED_LDS_TABLE 256  // largest level count - 1 whose tables live in LDS
+
+// One pixel's operands for the walk, in the order it consumes them.
+struct __align__(16) EdPix {
+	double b, e;    // clamped sample; error carried from earlier rows
+	double t1, u0;  // s >= t1: level klo+1 (2.0: unreachable); level klo exact below u0
+	double m0, u1;  // level klo rounds up from m0; level klo+1 exact below u1
+	double m1, l0;  // level klo+1 rounds up from m1; lin[klo]
+	double l1, l2;  // lin[klo+1], lin[klo+2]
 };
 
-__device__ __forceinline__ int level_of(const double *T, int P, double s)
+struct EdLds {
+	EdPix px[ED_CHUNK + 2];
+	double s[ED_CHUNK];
+	double eb[ED_CHUNK + 2];  // eb[i+2]: bound on |error| of chunk pixel i; [0], [1] carry in
+	unsigned long long wide[ED_CHUNK / 64];
+	unsigned short klo[ED_CHUNK], khi[ED_CHUNK];  // levels the pixel can reach
+	double T[ED_LDS_TABLE], U[ED_LDS_TABLE + 1], M[ED_LDS_TABLE + 1], lin[ED_LDS_TABLE + 1];
+	double cd[ED_LDS_TABLE + 1], pre[ED_LDS_TABLE + 1], suf[ED_LDS_TABLE + 1];
+};
+
+struct EdTab {
+	const double *T, *U, *M, *lin, *cd, *pre, *suf;
+	int P;
+};
+
+// Number of thresholds T[k], k in [a, b), that are <= s, plus a.
+__device__ __forceinline__ int level_in(const double *T, int a, int b, double s)
 {
-	int lo = 0, n = P;
+	int n = b - a;
 	while (n > 0) {
 		const int half = n >> 1;
-		if (T[lo + half] <= s) {
-			lo += half + 1;
+		if (T[a + half] <= s) {
+			a += half + 1;
 			n -= half + 1;
 		} else {
 			n = half;
 		}
 	}
-	return lo;
+	return a;
 }
 
-template <bool LDS_TAB>
-__global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0, int o_list1, int o_list2,
+// The reference's decision for a sample s whose level is k: exact -> error
+// 0 and code k; else floor or ceil by distance (src/imagew-main.c:712-748).
+__device__ __forceinline__ double ed_error(const EdTab &t, int k, double s, int &level)
+{
+	const bool exact = s < t.U[k];  // U[P] = +inf
+	const bool ceil_ = s >= t.M[k];
+	level = (exact || !ceil_) ? k : k + 1;
+	return exact ? 0.0 : s - t.lin[level];
+}
+
+template <bool HAS_M1>
+__device__ __forceinline__ double ed_step(const EdPix &p, double &e1, double &e2, double m0, double m1)
+{
+	const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
+	const double s = fmin(fmax(p.b + cell, 0.0), 1.0);  // never NaN or -0.0: exact clamp
+	const bool up = s >= p.t1;
+	const double uu = up ? p.u1 : p.u0, mm = up ? p.m1 : p.m0;
+	const double lo = up ? p.l1 : p.l0, hi = up ? p.l2 : p.l1;
+	const double base = (s >= mm) ? hi : lo;
+	const double err = (s < uu) ? 0.0 : s - base;
+	e2 = e1;
+	e1 = err;
+	return s;
+}
+
+template <bool LDS_TAB, bool HAS_M1>
+__global__ void __launch_bounds__(ED_CHUNK) errdiff_kernel(DevPlan P, int o_list0, int o_list1, int o_list2,
 	int o_list3)
 {
 	__shared__ EdLds S;
-	const int lane = threadIdx.x;
+	const int tid = threadIdx.x;
 	const int olist[4] = {o_list0, o_list1, o_list2, o_list3};
 	const int o = olist[blockIdx.x];
 	if (o < 0) return;
 	const DevOChan &q = P.oc[o];
-	const double *m = c_ed_matrix[q.subtype <= 7 ? q.subtype : 0];
+	const double *m = c_ed_matrix[(q.subtype >= 0 && q.subtype <= 7) ? q.subtype : 0];
 	const double m0 = m[0], m1 = m[1];
 	const int W = P.out_w, H = P.out_h, Pq = q.P;
 	double *E = P.ed_err + ((size_t)q.ed_index * 4 + 0) * W;
@@ -547,121 +523,149 @@ __global__ void __launch_bounds__(64) errdiff_wave_kernel(DevPlan P, int o_list0
 	double *ERR = P.ed_err + ((size_t)q.ed_index * 4 + 2) * W;
 	const double *samp = P.ed_samples + (size_t)q.ed_index * H * W;
 
+	EdTab t;
+	t.P = Pq;
 	if (LDS_TAB) {
-		for (int k = lane; k <= Pq; k += 64) {
+		for (int k = tid; k <= Pq; k += ED_CHUNK) {
 			if (k < Pq) S.T[k] = q.T[k];
 			S.U[k] = q.U[k];
+			S.M[k] = q.M[k];
 			S.lin[k] = q.lin[k];
 			S.cd[k] = q.code[k];
+			S.pre[k] = q.ebpre[k];
+			S.suf[k] = q.ebsuf[k];
 		}
+		t.T = S.T; t.U = S.U; t.M = S.M; t.lin = S.lin; t.cd = S.cd; t.pre = S.pre; t.suf = S.suf;
+	} else {
+		t.T = q.T; t.U = q.U; t.M = q.M; t.lin = q.lin; t.cd = q.code; t.pre = q.ebpre; t.suf = q.ebsuf;
 	}
-	const double *T = LDS_TAB ? S.T : q.T;
-	const double *U = LDS_TAB ? S.U : q.U;
-	const double *lin = LDS_TAB ? S.lin : q.lin;
-	const double *cd = LDS_TAB ? S.cd : q.code;
-	for (int x = lane; x < W; x += 64) {
+	for (int x = tid; x < W; x += ED_CHUNK) {
 		E[x] = 0.0;
 		R1[x] = 0.0;
 	}
 	__syncthreads();
 
-	const double B1 = q.ed_halfgap * m0, B2 = q.ed_halfgap * m1;
+	// Global bounds on the in-row carries, for the first window.
+	const double G1 = q.eb_max * m0, G2 = q.eb_max * m1;
+	double e1 = 0.0, e2 = 0.0;  // walker: errors of the previous two pixels
 
 	for (int j = 0; j < H; j++) {
 		const bool rev = (j % 2) != 0;
 		const int fwd = rev ? -1 : 1;
-		double e1 = 0.0, e2 = 0.0;  // errors of the previous two pixels (lane 0)
 		for (int z0 = 0; z0 < W; z0 += ED_CHUNK) {
 			const int n = min(ED_CHUNK, W - z0);
-			// 1. stage
-			for (int i = lane; i < n; i += 64) {
-				const int x = rev ? W - 1 - (z0 + i) : z0 + i;
-				const double b = samp[(size_t)j * W + x];
-				const double e = E[x];
-				const double lo = clamp01(b + ((e - B2) - B1)), hi = clamp01(b + ((e + B2) + B1));
-				const int klo = level_of(T, Pq, lo), khi = level_of(T, Pq, hi);
-				const int k1 = min(klo + 1, Pq), k2 = min(klo + 2, Pq);
-				S.b[i] = b;
-				S.e[i] = e;
-				S.klo[i] = (unsigned short)klo;
-				S.khi[i] = (unsigned short)khi;
-				S.tn[i] = (khi > klo) ? T[klo] : 2.0;
-				S.u0[i] = U[klo];
-				S.u1[i] = U[k1];
-				S.l0[i] = lin[klo];
-				S.l1[i] = lin[k1];
-				S.l2[i] = lin[k2];
-				S.c0[i] = (unsigned short)cd[klo];
-				S.c1[i] = (unsigned short)cd[k1];
-				S.c2[i] = (unsigned short)cd[k2];
+			const int i = tid;
+			const bool live = i < n;
+			const int x = rev ? W - 1 - (z0 + i) : z0 + i;
+			// 1a. window under the global bound -> this pixel's own error bound.
+			double b = 0.0, e = 0.0;
+			int kg0 = 0, kg1 = 0;
+			if (live) {
+				b = samp[(size_t)j * W + x];
+				e = E[x];
+				kg0 = level_in(t.T, 0, Pq, clamp01(b + ((e - G2) - G1)));
+				kg1 = level_in(t.T, kg0, Pq, clamp01(b + ((e + G2) + G1)));
+				S.eb[i + 2] = fmin(t.pre[kg1], t.suf[kg0]);
+			}
+			if (z0 == 0 && tid == 0) {
+				S.eb[0] = 0.0;  // no in-row carry into a row's first pixels
+				S.eb[1] = 0.0;
 			}
 			__syncthreads();
-			// 2. walk
-			if (lane == 0) {
-				double b = S.b[0], e = S.e[0], tn = S.tn[0], u0 = S.u0[0], u1 = S.u1[0];
-				double l0 = S.l0[0], l1 = S.l1[0], l2 = S.l2[0];
-				int klo = S.klo[0], khi = S.khi[0], c0 = S.c0[0], c1 = S.c1[0], c2 = S.c2[0];
-				for (int i = 0; i < n; i++) {
-					const int ni = (i + 1 < n) ? i + 1 : i;
-					const double nb = S.b[ni], ne = S.e[ni], ntn = S.tn[ni], nu0 = S.u0[ni], nu1 = S.u1[ni];
-					const double nl0 = S.l0[ni], nl1 = S.l1[ni], nl2 = S.l2[ni];
-					const int nklo = S.klo[ni], nkhi = S.khi[ni], nc0 = S.c0[ni], nc1 = S.c1[ni], nc2 = S.c2[ni];
-
-					const double cell = (e + e2 * m1) + e1 * m0;
-					double s = b + cell;
-					s = fmin(fmax(s, 0.0), 1.0);  // never NaN or -0.0: exact clamp
-					const bool up = tn <= s;
-					double uu = up ? u1 : u0, lo_ = up ? l1 : l0, hi_ = up ? l2 : l1;
-					int cl = up ? c1 : c0, ch = up ? c2 : c1;
-					if (up && khi > klo + 1) {
-						// Rare: more than two reachable levels.  Reads go to
-						// global memory so the join does not wait on the LDS
-						// loads already issued for the next pixel.
-						int k = klo + 1;
-						while (k < khi && q.T[k] <= s) k++;
-						uu = q.U[k];
-						lo_ = q.lin[k];
-						hi_ = q.lin[min(k + 1, Pq)];
-						cl = (int)q.code[k];
-						ch = (int)q.code[min(k + 1, Pq)];
+			// 1b. tight window from the two previous pixels' bounds.
+			bool wide = false;
+			if (live) {
+				const double B1 = S.eb[i + 1] * m0, B2 = S.eb[i] * m1;
+				const int klo = level_in(t.T, kg0, kg1, clamp01(b + ((e - B2) - B1)));
+				const int khi = level_in(t.T, klo, kg1, clamp01(b + ((e + B2) + B1)));
+				const int k1 = min(klo + 1, Pq), k2 = min(klo + 2, Pq);
+				EdPix &p = S.px[i];
+				p.b = b;
+				p.e = e;
+				p.t1 = (khi > klo) ? t.T[klo] : 2.0;
+				p.u0 = t.U[klo];
+				p.m0 = t.M[klo];
+				p.u1 = t.U[k1];
+				p.m1 = t.M[k1];
+				p.l0 = t.lin[klo];
+				p.l1 = t.lin[k1];
+				p.l2 = t.lin[k2];
+				S.klo[i] = (unsigned short)klo;
+				S.khi[i] = (unsigned short)khi;
+				wide = khi > klo + 1;
+			}
+			const unsigned long long wmask = __ballot(wide);
+			if ((tid & 63) == 0) S.wide[tid >> 6] = wmask;
+			__syncthreads();
+			// 2. walk.
+			if (tid == 0) {
+				if (z0 == 0) {
+					e1 = 0.0;
+					e2 = 0.0;
+				}
+				int a = 0;
+				while (a < n) {
+					int stop = n;
+					for (int w = a >> 6; w < (n + 63) >> 6; w++) {
+						unsigned long long mk = S.wide[w];
+						if (w == (a >> 6)) mk &= ~0ull << (a & 63);
+						if (mk) {
+							stop = min(n, w * 64 + __ffsll((long long)mk) - 1);
+							break;
+						}
 					}
-					const double d_floor = s - lo_;
-					const double d_ceil = hi_ - s;
-					const bool exact = s < uu;
-					const bool ceil_ = d_ceil <= d_floor;
-					const double err = exact ? 0.0 : (ceil_ ? -d_ceil : d_floor);
-					const int code = (exact || !ceil_) ? cl : ch;
-					S.err[i] = err;
-					S.code[i] = (unsigned short)code;
-					e2 = e1;
-					e1 = err;
-					b = nb; e = ne; tn = ntn; u0 = nu0; u1 = nu1; l0 = nl0; l1 = nl1; l2 = nl2;
-					klo = nklo; khi = nkhi; c0 = nc0; c1 = nc1; c2 = nc2;
+					EdPix pa = S.px[a];
+					int k = a;
+					for (; k + 2 <= stop; k += 2) {
+						const EdPix pb = S.px[k + 1];
+						S.s[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
+						pa = S.px[k + 2];
+						S.s[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
+					}
+					if (k < stop) S.s[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
+					if (stop < n) {
+						const EdPix &p = S.px[stop];
+						const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
+						const double s = fmin(fmax(p.b + cell, 0.0), 1.0);
+						const int lv = level_in(t.T, S.klo[stop], S.khi[stop], s);
+						int dummy;
+						const double err = ed_error(t, lv, s, dummy);
+						S.s[stop] = s;
+						e2 = e1;
+						e1 = err;
+					}
+					a = stop + 1;
+				}
+				if (n >= 2) {
+					S.eb[0] = S.eb[n];  // carry the last two bounds into the next chunk
+					S.eb[1] = S.eb[n + 1];
 				}
 			}
 			__syncthreads();
-			// Results of the chunk back to x order.
-			for (int i = lane; i < n; i += 64) {
-				const int x = rev ? W - 1 - (z0 + i) : z0 + i;
-				ERR[x] = S.err[i];
-				store_sample(P, x, j, o, (double)S.code[i]);
+			// 3. finish: code and error from s, as the walk computed them.
+			if (live) {
+				const double s = S.s[i];
+				const int lv = level_in(t.T, S.klo[i], S.khi[i], s);
+				int level;
+				ERR[x] = ed_error(t, lv, s, level);
+				store_sample(P, x, j, o, t.cd[level]);
 			}
 			__syncthreads();
 		}
-		// 3. spread to the next two rows.  Target t receives from sources
-		// t-2f, t-f, t, t+f, t+2f (scan order) with next-row weights
+		// 4. spread to the next two rows.  Target x receives from sources
+		// x-2f, x-f, x, x+f, x+2f (scan order) with next-row weights
 		// m[6], m[5], m[4], m[3], m[2] and row-after weights m[11..7].
-		for (int t = lane; t < W; t += 64) {
-			double a1 = R1[t], a2 = 0.0;
+		for (int x = tid; x < W; x += ED_CHUNK) {
+			double a1 = R1[x], a2 = 0.0;
 			for (int d = -2; d <= 2; d++) {
-				const int p = t + d * fwd;
+				const int p = x + d * fwd;
 				if (p < 0 || p >= W) continue;
 				const double er = ERR[p];
 				a1 = a1 + er * m[4 - d];
 				a2 = a2 + er * m[9 - d];
 			}
-			E[t] = a1;
-			R1[t] = a2;
+			E[x] = a1;
+			R1[x] = a2;
 		}
 		__syncthreads();
 	}
@@ -845,6 +849,7 @@ bool job_upload(iwamd_job *J)
 		}
 	}
 	std::vector<QuantTable> qt(p.out_nch);
+	std::vector<EdTables> et(p.out_nch);
 	std::vector<PrngTables> pt(p.out_nch);
 	int ed_of[4] = {-1, -1, -1, -1};
 	J->n_ed = 0;
@@ -859,6 +864,11 @@ bool job_upload(iwamd_job *J)
 				}
 			}
 			if (p.och[o].ditherfamily == IW_DITHERFAMILY_ERRDIFF) {
+				et[o] = build_ed_tables(qt[o]);
+				if (qt[o].P > 65535) {
+					iw_set_error(ctx, "Too many output levels for error diffusion on the GPU path");
+					return false;
+				}
 				ed_of[o] = J->n_ed;
 				J->ed_list[J->n_ed++] = o;
 			}
@@ -900,7 +910,7 @@ bool job_upload(iwamd_job *J)
 		place_w(wv[c], ov[c]);
 		place_w(wh[c], oh[c]);
 	}
-	struct QOff { size_t T, U, lin, code, nc, prow, pcol; };
+	struct QOff { size_t T, U, lin, code, nc, prow, pcol, M, pre, suf; };
 	std::vector<QOff> oq(p.out_nch);
 	for (int o = 0; o < (int)qt.size(); o++) {
 		oq[o].T = L.add<double>(qt[o].T.size());
@@ -910,6 +920,9 @@ bool job_upload(iwamd_job *J)
 		oq[o].nc = L.add<double>(qt[o].nc.size());
 		oq[o].prow = L.add<uint64_t>(pt[o].row_q.size());
 		oq[o].pcol = L.add<uint64_t>(pt[o].col_mont.size());
+		oq[o].M = L.add<double>(et[o].M.size());
+		oq[o].pre = L.add<double>(et[o].pre.size());
+		oq[o].suf = L.add<double>(et[o].suf.size());
 	}
 	J->block_bytes = L.off + 256;
 	J->block = dev_alloc(J->block_bytes);
@@ -943,6 +956,9 @@ bool job_upload(iwamd_job *J)
 		up(oq[o].nc, qt[o].nc.data(), qt[o].nc.size() * 8);
 		up(oq[o].prow, pt[o].row_q.data(), pt[o].row_q.size() * 8);
 		up(oq[o].pcol, pt[o].col_mont.data(), pt[o].col_mont.size() * 8);
+		up(oq[o].M, et[o].M.data(), et[o].M.size() * 8);
+		up(oq[o].pre, et[o].pre.data(), et[o].pre.size() * 8);
+		up(oq[o].suf, et[o].suf.data(), et[o].suf.size() * 8);
 	}
 	// Host vectors are freed when this function returns: wait for the copies.
 	ok &= hipStreamSynchronize(st) == hipSuccess;
@@ -1030,19 +1046,10 @@ bool job_upload(iwamd_job *J)
 			d.nc = (const double *)(base + oq[o].nc);
 			d.prng_row = (const uint64_t *)(base + oq[o].prow);
 			d.prng_col = (const uint64_t *)(base + oq[o].pcol);
-			// Bound on |error| of one pixel.  A non-exact sample at level k
-			// lies in [U[k], T[k]) and its error is the smaller distance to
-			// lin[k], lin[k+1]: at most half their gap, or the overshoot of
-			// the interval past either candidate.  Widened for the rounding of
-			// the differences and of the products that carry the error.
-			double bound = 0.0;
-			const QuantTable &Q = qt[o];
-			for (int k = 0; k < Q.P; k++) {
-				bound = std::max(bound, (Q.lin[k + 1] - Q.lin[k]) * 0.5);
-				bound = std::max(bound, Q.lin[k] - Q.U[k]);
-				bound = std::max(bound, Q.T[k] - Q.lin[k + 1]);
-			}
-			d.ed_halfgap = bound * (1.0 + 1e-6) + 1e-300;
+			d.M = (const double *)(base + oq[o].M);
+			d.ebpre = (const double *)(base + oq[o].pre);
+			d.ebsuf = (const double *)(base + oq[o].suf);
+			d.eb_max = et[o].eb_max;
 		}
 	}
 	D.ed_samples = (double *)(base + o_eds);
@@ -1106,13 +1113,18 @@ extern "C" int iwamd_job_run(struct iwamd_job *J)
 	hipEventRecord(J->ev[2], st);
 	if (J->n_ed > 0) {
 		const int *l = J->ed_list;
-		// Quantisation tables fit in LDS for up to 257 levels (8-bit output).
-		bool small = true;
-		for (int e = 0; e < J->n_ed; e++) small &= D.oc[l[e]].P <= ED_LDS_TABLE;
-		if (small)
-			hipLaunchKernelGGL(errdiff_wave_kernel<true>, dim3(J->n_ed), dim3(64), 0, st, D, l[0], l[1], l[2], l[3]);
-		else
-			hipLaunchKernelGGL(errdiff_wave_kernel<false>, dim3(J->n_ed), dim3(64), 0, st, D, l[0], l[1], l[2], l[3]);
+		// Decision tables fit in LDS for up to 257 levels (8-bit output); the
+		// x+2 carry is skipped where its weight is zero for every plane.
+		bool small = true, m1 = false;
+		for (int e = 0; e < J->n_ed; e++) {
+			const DevOChan &q = D.oc[l[e]];
+			small &= q.P <= ED_LDS_TABLE;
+			const int sub = (q.subtype >= 0 && q.subtype <= 7) ? q.subtype : 0;
+			m1 |= sub != 0 && sub != 6;  // Floyd-Steinberg and Sierra 4-2-1 have none
+		}
+		auto k = small ? (m1 ? errdiff_kernel<true, true> : errdiff_kernel<true, false>)
+			: (m1 ? errdiff_kernel<false, true> : errdiff_kernel<false, false>);
+		hipLaunchKernelGGL(k, dim3(J->n_ed), dim3(ED_CHUNK), 0, st, D, l[0], l[1], l[2], l[3]);
 	}
 	hipEventRecord(J->ev[3], st);
 	if (hipGetLastError() != hipSuccess) {

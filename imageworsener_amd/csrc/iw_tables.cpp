@@ -3,6 +3,7 @@
 #include "iw_tables.h"
 #include "iw_ctx.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -313,6 +314,54 @@ QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc)
 	return q;
 }
 
+EdTables build_ed_tables(const QuantTable &q)
+{
+	EdTables t;
+	const int P = q.P;
+	t.M.assign(P + 1, 2.0);
+	t.eb.assign(P + 1, 0.0);
+	for (int k = 0; k < P; k++) {
+		const double lo = q.lin[k], hi = q.lin[k + 1];
+		auto ceil_wins = [&](double s) { return (hi - s) <= (s - lo); };
+		double m = 2.0;
+		if (ceil_wins(0.0)) {
+			m = 0.0;
+		} else if (ceil_wins(1.0)) {
+			uint64_t a = 0, b = dbits(1.0);  // !ceil_wins(a), ceil_wins(b)
+			while (b - a > 1) {
+				const uint64_t mid = a + (b - a) / 2;
+				if (ceil_wins(bitsd(mid))) b = mid;
+				else a = mid;
+			}
+			m = bitsd(b);
+		}
+		t.M[k] = m;
+		// Non-exact samples of level k: [U[k], T[k]).
+		const double u = q.U[k], tk = q.T[k];
+		double e = 0.0;
+		auto at = [&](double s, double base) { e = std::max(e, fabs(s - base)); };
+		const double fend = std::min(m, tk);
+		if (u < fend) {
+			at(u, lo);
+			at(nextafter(fend, -1.0), lo);
+		}
+		const double cbeg = std::max(u, m);
+		if (cbeg < tk) {
+			at(cbeg, hi);
+			at(nextafter(tk, -1.0), hi);
+		}
+		t.eb[k] = e;
+	}
+	t.pre.resize(P + 1);
+	t.suf.resize(P + 1);
+	double run = 0.0;
+	for (int k = 0; k <= P; k++) t.pre[k] = run = std::max(run, t.eb[k]);
+	run = 0.0;
+	for (int k = P; k >= 0; k--) t.suf[k] = run = std::max(run, t.eb[k]);
+	t.eb_max = t.pre[P];
+	return t;
+}
+
 }  // namespace iwx
 
 extern "C" int iwamd_quant_thresholds(int cstype, double gamma, int P, double *T, double *U)
@@ -415,3 +464,16 @@ PrngTables build_prng_tables(int seed, int width, int height)
 }
 
 }  // namespace iwx
+
+extern "C" int iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, double *M, double *eb)
+{
+	if (och->use_nc_table) return -1;
+	const iwx::QuantTable q = iwx::build_quant_table(*och, false);
+	const iwx::EdTables t = iwx::build_ed_tables(q);
+	for (int k = 0; k <= q.P; k++) {
+		lin[k] = q.lin[k];
+		M[k] = t.M[k];
+		eb[k] = t.eb[k];
+	}
+	return q.P;
+}

@@ -52,6 +52,20 @@ struct QuantTable {
 
 QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc);
 
+// Decision data for error diffusion over a QuantTable (src/imagew-main.c:
+// 730-748).  A non-exact sample s at level k is rounded up iff
+// (lin[k+1] - s) <= (s - lin[k]) in FP64; both differences are monotone in s,
+// so that holds exactly for s >= M[k] (M[P] = 2: never).  eb[k] is the largest
+// |error| any s at level k can leave -- found exactly, as the error is
+// monotone on each of the floor and ceil ranges -- and pre/suf are its prefix
+// and suffix maxima, so max eb over levels [a, b] <= min(pre[b], suf[a]).
+struct EdTables {
+	std::vector<double> M, eb, pre, suf;
+	double eb_max = 0.0;
+};
+
+EdTables build_ed_tables(const QuantTable &q);
+
 // (T, U) for a transfer function and level count; cached per process.
 const std::pair<std::vector<double>, std::vector<double>> &thresholds(const struct iw_csdescr &cs, int P);
 

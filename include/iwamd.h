@@ -139,6 +139,14 @@ IW_EXPORT(void) iwamd_set_precision(struct iw_context *ctx, int mode);
  * Fills T (P entries) and U (P+1 entries); returns 1. */
 IW_EXPORT(int) iwamd_quant_thresholds(int cstype, double gamma, int P, double *T, double *U);
 
+/* The error-diffusion decision data for an output channel (och as in a plan,
+ * quantised to P+1 levels): lin[k] the level's value in linear light,
+ * M[k] the least s at which a non-exact sample of level k rounds up
+ * ((lin[k+1]-s) <= (s-lin[k]) in FP64; 2.0 when never, M[P] = 2.0), eb[k] the
+ * largest |error| a sample of level k can leave.  Arrays hold P+1 entries;
+ * returns P, or -1 for a floating-point or nearest-colour channel. */
+IW_EXPORT(int) iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, double *M, double *eb);
+
 /* Device-resident jobs.  iwamd_job_create() runs the host decisions on ctx
  * (consuming its single use), builds the device tables and uploads the input
  * image once; iwamd_job_run() launches the pixel kernels on the job's stream
