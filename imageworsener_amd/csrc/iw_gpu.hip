@@ -403,27 +403,29 @@ __global__ void __launch_bounds__(256) hpass_kernel(DevPlan P)
 // A plane is one dependency chain: rows alternate direction (src/imagew-main.c:
 // 1072-1077), so every pixel waits for the one before it in scan order and a
 // row cannot start before the previous row has finished.  One workgroup runs
-// one plane; per chunk of a row:
-//   1. stage (all threads, one pixel each): from the sample b, the error E
-//      carried from earlier rows and a bound on the two in-row carries, the
-//      one or two levels the pixel can reach and their decision data are
-//      written to LDS.  The bound is rigorous: |error| of a pixel is at most
-//      the largest eb of the levels its own sample can reach under the global
-//      bound, and rounding is monotone, so the window always holds s.  A
-//      pixel whose window spans three or more levels is flagged "wide";
-//   2. walk (one lane): cell = (E + e[x-2]*m1) + e[x-1]*m0, s = clamp(b +
-//      cell), pick the level with one compare, the reference's floor/ceil
-//      choice and error with compares and selects; only s is stored.  Wide
-//      pixels take a table search;
-//   3. finish (all threads): the code and error of every pixel are
-//      recomputed from s with the same formulas (so they are the walk's
-//      bits), the output is stored and the errors kept for the next rows;
-// and after the row, the errors reach the next two rows as an ordered gather
-// -- for each target cell, the sources in the row's scan order -- which adds
-// the same terms in the same order as the reference's per-pixel scatter
-// (src/imagew-main.c:497-567).  An exact pixel (no error in the reference)
-// carries +0.0 here: adding a signed zero to an accumulator that is never
-// -0.0 leaves it unchanged, so the bits agree without per-pixel flags.
+// one plane: wave 0 walks the chain while waves 1-3 ("helpers") prepare and
+// complete the chunks around the one being walked.  Per chunk of a row:
+//   stage A (helpers, one pixel each): from the sample b and the error E
+//      carried from earlier rows, the levels the pixel can reach when the two
+//      in-row carries are bounded globally, and from those the pixel's own
+//      bound on |error| (the largest eb of those levels);
+//   stage B: the tighter window from the two previous pixels' own bounds,
+//      and the decision data of its one or two levels, to LDS.  Both windows
+//      are rigorous (|error| <= eb of its level; rounding is monotone), so s
+//      always lies in them.  Windows of three or more levels are "wide";
+//   walk (one lane): cell = (E + e[x-2]*m1) + e[x-1]*m0, s = clamp(b + cell),
+//      the level by one compare, the reference's floor/ceil choice and error
+//      by compares and selects; only s is stored.  Wide pixels search;
+//   finish (helpers): code and error recomputed from s by the same formulas
+//      (so they carry the walk's bits), output stored, errors kept.
+// While wave 0 walks chunk c, the helpers finish c-1, run stage B of c+1 and
+// stage A of c+2.  At a row's end the errors reach the next two rows as an
+// ordered gather -- for each target cell, the sources in the row's scan
+// order -- which adds the same terms in the same order as the reference's
+// per-pixel scatter (src/imagew-main.c:497-567).  The error rows live in LDS
+// for rows up to ED_ROWS_LDS wide.  An exact pixel (no error in the
+// reference) carries +0.0 here: adding a signed zero to an accumulator that
+// is never -0.0 leaves it unchanged, so the bits agree without flags.
 
 // Error-diffusion weights; slots 0,1: x+1,x+2 of this row; 2..6: x-2..x+2 of
 // the next row; 7..11: the row after (src/imagew-main.c:510-535).
@@ -437,8 +439,10 @@ __constant__ double c_ed_matrix[8][12] = {
 	{2.0/4, 0.0, 0.0, 1.0/4, 1.0/4, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                 // Sierra 4-2-1
 	{1.0/8, 1.0/8, 0.0, 1.0/8, 1.0/8, 1.0/8, 0.0, 0.0, 0.0, 1.0/8, 0.0, 0.0}};          // Atkinson
 
-#define ED_CHUNK 256      // pixels per stage/walk/finish round; = threads per workgroup
-#define ED_LDS_TABLE 256  // largest level count - 1 whose tables live in LDS
+#define ED_THREADS 256     // wave 0 walks; waves 1-3 help
+#define ED_CH 192          // pixels per chunk = helper threads
+#define ED_LDS_TABLE 256   // largest level count - 1 whose tables live in LDS
+#define ED_ROWS_LDS 4096   // widest row whose error rows live in LDS
 
 // One pixel's operands for the walk, in the order it consumes them.
 struct __align__(16) EdPix {
@@ -449,14 +453,18 @@ struct __align__(16) EdPix {
 	double l1, l2;  // lin[klo+1], lin[klo+2]
 };
 
+template <bool LDS_ROWS>
 struct EdLds {
-	EdPix px[ED_CHUNK + 2];
-	double s[ED_CHUNK];
-	double eb[ED_CHUNK + 2];  // eb[i+2]: bound on |error| of chunk pixel i; [0], [1] carry in
-	unsigned long long wide[ED_CHUNK / 64];
-	unsigned short klo[ED_CHUNK], khi[ED_CHUNK];  // levels the pixel can reach
+	EdPix px[2][ED_CH + 2];                 // chunk c in px[c % 2]
+	double s[3][ED_CH];                     // walked samples, chunk c in [c % 3]
+	double eb[3][ED_CH];                    // bound on |error| per pixel
+	double b[2][ED_CH], e[2][ED_CH];        // stage A -> stage B
+	unsigned long long wide[2][ED_CH / 64];
+	unsigned short klo[3][ED_CH], khi[3][ED_CH];  // levels the pixel can reach
+	unsigned short kg0[2][ED_CH], kg1[2][ED_CH];  // ... under the global bound
 	double T[ED_LDS_TABLE], U[ED_LDS_TABLE + 1], M[ED_LDS_TABLE + 1], lin[ED_LDS_TABLE + 1];
 	double cd[ED_LDS_TABLE + 1], pre[ED_LDS_TABLE + 1], suf[ED_LDS_TABLE + 1];
+	double rows[LDS_ROWS ? 3 * ED_ROWS_LDS : 1];  // E, R1, ERR
 };
 
 struct EdTab {
@@ -505,28 +513,166 @@ __device__ __forceinline__ double ed_step(const EdPix &p, double &e1, double &e2
 	return s;
 }
 
-template <bool LDS_TAB, bool HAS_M1>
-__global__ void __launch_bounds__(ED_CHUNK) errdiff_kernel(DevPlan P, int o_list0, int o_list1, int o_list2,
+struct EdCtx {
+	EdTab t;
+	const double *samp;
+	double *E, *R1, *ERR;
+	double m0, m1, G1, G2;
+	int W, Pq, o;
+};
+
+template <bool LDS_ROWS>
+__device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int ch, int i)
+{
+	const int z = ch * ED_CH + i;
+	if (z >= c.W) return;
+	const int x = (j % 2) ? c.W - 1 - z : z;
+	const double b = c.samp[(size_t)j * c.W + x];
+	const double e = c.E[x];
+	const int kg0 = level_in(c.t.T, 0, c.Pq, clamp01(b + ((e - c.G2) - c.G1)));
+	const int kg1 = level_in(c.t.T, kg0, c.Pq, clamp01(b + ((e + c.G2) + c.G1)));
+	S.eb[ch % 3][i] = fmin(c.t.pre[kg1], c.t.suf[kg0]);
+	S.b[ch % 2][i] = b;
+	S.e[ch % 2][i] = e;
+	S.kg0[ch % 2][i] = (unsigned short)kg0;
+	S.kg1[ch % 2][i] = (unsigned short)kg1;
+}
+
+// Run by all 64 lanes of a helper wave (the wide flags are a ballot).
+template <bool LDS_ROWS>
+__device__ __forceinline__ void ed_stage_b(EdLds<LDS_ROWS> &S, const EdCtx &c, int ch, int i)
+{
+	const int z = ch * ED_CH + i;
+	bool wide = false;
+	if (z < c.W) {
+		const double b = S.b[ch % 2][i], e = S.e[ch % 2][i];
+		const int kg0 = S.kg0[ch % 2][i], kg1 = S.kg1[ch % 2][i];
+		// Bounds of the previous two pixels in scan order (none before the
+		// row's first pixel); chunk ch-1 is full when ch > 0.
+		const double eb1 = (i >= 1) ? S.eb[ch % 3][i - 1] : (z >= 1 ? S.eb[(ch + 2) % 3][ED_CH - 1] : 0.0);
+		const double eb2 = (i >= 2) ? S.eb[ch % 3][i - 2] : (z >= 2 ? S.eb[(ch + 2) % 3][ED_CH - 2 + i] : 0.0);
+		const double B1 = eb1 * c.m0, B2 = eb2 * c.m1;
+		const int klo = level_in(c.t.T, kg0, kg1, clamp01(b + ((e - B2) - B1)));
+		const int khi = level_in(c.t.T, klo, kg1, clamp01(b + ((e + B2) + B1)));
+		const int k1 = min(klo + 1, c.Pq), k2 = min(klo + 2, c.Pq);
+		EdPix &p = S.px[ch % 2][i];
+		p.b = b;
+		p.e = e;
+		p.t1 = (khi > klo) ? c.t.T[klo] : 2.0;
+		p.u0 = c.t.U[klo];
+		p.m0 = c.t.M[klo];
+		p.u1 = c.t.U[k1];
+		p.m1 = c.t.M[k1];
+		p.l0 = c.t.lin[klo];
+		p.l1 = c.t.lin[k1];
+		p.l2 = c.t.lin[k2];
+		S.klo[ch % 3][i] = (unsigned short)klo;
+		S.khi[ch % 3][i] = (unsigned short)khi;
+		wide = khi > klo + 1;
+	}
+	const unsigned long long wm = __ballot(wide);
+	if ((i & 63) == 0) S.wide[ch % 2][i >> 6] = wm;
+}
+
+template <bool LDS_ROWS>
+__device__ __forceinline__ void ed_finish(EdLds<LDS_ROWS> &S, const EdCtx &c, const DevPlan &P, int j, int ch,
+	int i)
+{
+	const int z = ch * ED_CH + i;
+	if (z >= c.W) return;
+	const int x = (j % 2) ? c.W - 1 - z : z;
+	const double s = S.s[ch % 3][i];
+	const int lv = level_in(c.t.T, S.klo[ch % 3][i], S.khi[ch % 3][i], s);
+	int level;
+	c.ERR[x] = ed_error(c.t, lv, s, level);
+	store_sample(P, x, j, c.o, c.t.cd[level]);
+}
+
+template <bool LDS_ROWS, bool HAS_M1>
+__device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int ch, double &e1, double &e2)
+{
+	const int n = min(ED_CH, c.W - ch * ED_CH);
+	const EdPix *px = S.px[ch % 2];
+	double *so = S.s[ch % 3];
+	const double m0 = c.m0, m1 = c.m1;
+	int a = 0;
+	while (a < n) {
+		// Run of ordinary pixels up to the next wide one.
+		int stop = n;
+		for (int w = a >> 6; w < (n + 63) >> 6; w++) {
+			unsigned long long mk = S.wide[ch % 2][w];
+			if (w == (a >> 6)) mk &= ~0ull << (a & 63);
+			if (mk) {
+				stop = min(n, w * 64 + __ffsll((unsigned long long)mk) - 1);
+				break;
+			}
+		}
+		// Four pixels per turn, operands loaded two pixels ahead
+		// (px has ED_CH + 2 entries, so index stop + 1 is in bounds).
+		EdPix pa = px[a], pb = px[a + 1];
+		int k = a;
+		for (; k + 4 <= stop; k += 4) {
+			const EdPix pc = px[k + 2], pd = px[k + 3];
+			so[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
+			so[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
+			pa = px[k + 4];
+			pb = px[k + 5];
+			so[k + 2] = ed_step<HAS_M1>(pc, e1, e2, m0, m1);
+			so[k + 3] = ed_step<HAS_M1>(pd, e1, e2, m0, m1);
+		}
+		if (k < stop) so[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
+		if (k + 1 < stop) so[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
+		if (k + 2 < stop) so[k + 2] = ed_step<HAS_M1>(px[k + 2], e1, e2, m0, m1);
+		if (stop < n) {
+			const EdPix &p = px[stop];
+			const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
+			const double s = fmin(fmax(p.b + cell, 0.0), 1.0);
+			const int lv = level_in(c.t.T, S.klo[ch % 3][stop], S.khi[ch % 3][stop], s);
+			int level;
+			const double err = ed_error(c.t, lv, s, level);
+			so[stop] = s;
+			e2 = e1;
+			e1 = err;
+		}
+		a = stop + 1;
+	}
+}
+
+template <bool LDS_TAB, bool HAS_M1, bool LDS_ROWS>
+__global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(DevPlan P, int o_list0, int o_list1, int o_list2,
 	int o_list3)
 {
-	__shared__ EdLds S;
+	__shared__ EdLds<LDS_ROWS> S;
 	const int tid = threadIdx.x;
 	const int olist[4] = {o_list0, o_list1, o_list2, o_list3};
 	const int o = olist[blockIdx.x];
 	if (o < 0) return;
 	const DevOChan &q = P.oc[o];
 	const double *m = c_ed_matrix[(q.subtype >= 0 && q.subtype <= 7) ? q.subtype : 0];
-	const double m0 = m[0], m1 = m[1];
 	const int W = P.out_w, H = P.out_h, Pq = q.P;
-	double *E = P.ed_err + ((size_t)q.ed_index * 4 + 0) * W;
-	double *R1 = P.ed_err + ((size_t)q.ed_index * 4 + 1) * W;
-	double *ERR = P.ed_err + ((size_t)q.ed_index * 4 + 2) * W;
-	const double *samp = P.ed_samples + (size_t)q.ed_index * H * W;
 
-	EdTab t;
-	t.P = Pq;
+	EdCtx c;
+	c.W = W;
+	c.Pq = Pq;
+	c.o = o;
+	c.m0 = m[0];
+	c.m1 = m[1];
+	// Global bounds on the in-row carries, for the first window.
+	c.G1 = q.eb_max * c.m0;
+	c.G2 = q.eb_max * c.m1;
+	c.samp = P.ed_samples + (size_t)q.ed_index * H * W;
+	if (LDS_ROWS) {
+		c.E = S.rows;
+		c.R1 = S.rows + ED_ROWS_LDS;
+		c.ERR = S.rows + 2 * ED_ROWS_LDS;
+	} else {
+		c.E = P.ed_err + ((size_t)q.ed_index * 4 + 0) * W;
+		c.R1 = P.ed_err + ((size_t)q.ed_index * 4 + 1) * W;
+		c.ERR = P.ed_err + ((size_t)q.ed_index * 4 + 2) * W;
+	}
+	c.t.P = Pq;
 	if (LDS_TAB) {
-		for (int k = tid; k <= Pq; k += ED_CHUNK) {
+		for (int k = tid; k <= Pq; k += ED_THREADS) {
 			if (k < Pq) S.T[k] = q.T[k];
 			S.U[k] = q.U[k];
 			S.M[k] = q.M[k];
@@ -535,140 +681,66 @@ __global__ void __launch_bounds__(ED_CHUNK) errdiff_kernel(DevPlan P, int o_list
 			S.pre[k] = q.ebpre[k];
 			S.suf[k] = q.ebsuf[k];
 		}
-		t.T = S.T; t.U = S.U; t.M = S.M; t.lin = S.lin; t.cd = S.cd; t.pre = S.pre; t.suf = S.suf;
+		c.t.T = S.T; c.t.U = S.U; c.t.M = S.M; c.t.lin = S.lin; c.t.cd = S.cd; c.t.pre = S.pre; c.t.suf = S.suf;
 	} else {
-		t.T = q.T; t.U = q.U; t.M = q.M; t.lin = q.lin; t.cd = q.code; t.pre = q.ebpre; t.suf = q.ebsuf;
+		c.t.T = q.T; c.t.U = q.U; c.t.M = q.M; c.t.lin = q.lin; c.t.cd = q.code; c.t.pre = q.ebpre; c.t.suf = q.ebsuf;
 	}
-	for (int x = tid; x < W; x += ED_CHUNK) {
-		E[x] = 0.0;
-		R1[x] = 0.0;
+	for (int x = tid; x < W; x += ED_THREADS) {
+		c.E[x] = 0.0;
+		c.R1[x] = 0.0;
 	}
 	__syncthreads();
 
-	// Global bounds on the in-row carries, for the first window.
-	const double G1 = q.eb_max * m0, G2 = q.eb_max * m1;
+	const bool helper = tid >= 64;
+	const int h = tid - 64;  // helper's pixel in a chunk
+	const int nch = (W + ED_CH - 1) / ED_CH;
 	double e1 = 0.0, e2 = 0.0;  // walker: errors of the previous two pixels
 
 	for (int j = 0; j < H; j++) {
-		const bool rev = (j % 2) != 0;
-		const int fwd = rev ? -1 : 1;
-		for (int z0 = 0; z0 < W; z0 += ED_CHUNK) {
-			const int n = min(ED_CHUNK, W - z0);
-			const int i = tid;
-			const bool live = i < n;
-			const int x = rev ? W - 1 - (z0 + i) : z0 + i;
-			// 1a. window under the global bound -> this pixel's own error bound.
-			double b = 0.0, e = 0.0;
-			int kg0 = 0, kg1 = 0;
-			if (live) {
-				b = samp[(size_t)j * W + x];
-				e = E[x];
-				kg0 = level_in(t.T, 0, Pq, clamp01(b + ((e - G2) - G1)));
-				kg1 = level_in(t.T, kg0, Pq, clamp01(b + ((e + G2) + G1)));
-				S.eb[i + 2] = fmin(t.pre[kg1], t.suf[kg0]);
-			}
-			if (z0 == 0 && tid == 0) {
-				S.eb[0] = 0.0;  // no in-row carry into a row's first pixels
-				S.eb[1] = 0.0;
-			}
+		// Row boundary: complete the previous row, carry its errors down,
+		// stage this row's first chunks.
+		if (j > 0) {
+			if (helper) ed_finish(S, c, P, j - 1, nch - 1, h);
 			__syncthreads();
-			// 1b. tight window from the two previous pixels' bounds.
-			bool wide = false;
-			if (live) {
-				const double B1 = S.eb[i + 1] * m0, B2 = S.eb[i] * m1;
-				const int klo = level_in(t.T, kg0, kg1, clamp01(b + ((e - B2) - B1)));
-				const int khi = level_in(t.T, klo, kg1, clamp01(b + ((e + B2) + B1)));
-				const int k1 = min(klo + 1, Pq), k2 = min(klo + 2, Pq);
-				EdPix &p = S.px[i];
-				p.b = b;
-				p.e = e;
-				p.t1 = (khi > klo) ? t.T[klo] : 2.0;
-				p.u0 = t.U[klo];
-				p.m0 = t.M[klo];
-				p.u1 = t.U[k1];
-				p.m1 = t.M[k1];
-				p.l0 = t.lin[klo];
-				p.l1 = t.lin[k1];
-				p.l2 = t.lin[k2];
-				S.klo[i] = (unsigned short)klo;
-				S.khi[i] = (unsigned short)khi;
-				wide = khi > klo + 1;
-			}
-			const unsigned long long wmask = __ballot(wide);
-			if ((tid & 63) == 0) S.wide[tid >> 6] = wmask;
-			__syncthreads();
-			// 2. walk.
-			if (tid == 0) {
-				if (z0 == 0) {
-					e1 = 0.0;
-					e2 = 0.0;
+			const int fwd = ((j - 1) % 2) ? -1 : 1;
+			// Target x receives from sources x-2f, x-f, x, x+f, x+2f (scan
+			// order) with next-row weights m[6], m[5], m[4], m[3], m[2] and
+			// row-after weights m[11..7].
+			for (int x = tid; x < W; x += ED_THREADS) {
+				double a1 = c.R1[x], a2 = 0.0;
+				for (int d = -2; d <= 2; d++) {
+					const int p = x + d * fwd;
+					if (p < 0 || p >= W) continue;
+					const double er = c.ERR[p];
+					a1 = a1 + er * m[4 - d];
+					a2 = a2 + er * m[9 - d];
 				}
-				int a = 0;
-				while (a < n) {
-					int stop = n;
-					for (int w = a >> 6; w < (n + 63) >> 6; w++) {
-						unsigned long long mk = S.wide[w];
-						if (w == (a >> 6)) mk &= ~0ull << (a & 63);
-						if (mk) {
-							stop = min(n, w * 64 + __ffsll((long long)mk) - 1);
-							break;
-						}
-					}
-					EdPix pa = S.px[a];
-					int k = a;
-					for (; k + 2 <= stop; k += 2) {
-						const EdPix pb = S.px[k + 1];
-						S.s[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
-						pa = S.px[k + 2];
-						S.s[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
-					}
-					if (k < stop) S.s[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
-					if (stop < n) {
-						const EdPix &p = S.px[stop];
-						const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
-						const double s = fmin(fmax(p.b + cell, 0.0), 1.0);
-						const int lv = level_in(t.T, S.klo[stop], S.khi[stop], s);
-						int dummy;
-						const double err = ed_error(t, lv, s, dummy);
-						S.s[stop] = s;
-						e2 = e1;
-						e1 = err;
-					}
-					a = stop + 1;
-				}
-				if (n >= 2) {
-					S.eb[0] = S.eb[n];  // carry the last two bounds into the next chunk
-					S.eb[1] = S.eb[n + 1];
-				}
-			}
-			__syncthreads();
-			// 3. finish: code and error from s, as the walk computed them.
-			if (live) {
-				const double s = S.s[i];
-				const int lv = level_in(t.T, S.klo[i], S.khi[i], s);
-				int level;
-				ERR[x] = ed_error(t, lv, s, level);
-				store_sample(P, x, j, o, t.cd[level]);
+				c.E[x] = a1;
+				c.R1[x] = a2;
 			}
 			__syncthreads();
 		}
-		// 4. spread to the next two rows.  Target x receives from sources
-		// x-2f, x-f, x, x+f, x+2f (scan order) with next-row weights
-		// m[6], m[5], m[4], m[3], m[2] and row-after weights m[11..7].
-		for (int x = tid; x < W; x += ED_CHUNK) {
-			double a1 = R1[x], a2 = 0.0;
-			for (int d = -2; d <= 2; d++) {
-				const int p = x + d * fwd;
-				if (p < 0 || p >= W) continue;
-				const double er = ERR[p];
-				a1 = a1 + er * m[4 - d];
-				a2 = a2 + er * m[9 - d];
-			}
-			E[x] = a1;
-			R1[x] = a2;
+		if (helper) {
+			ed_stage_a(S, c, j, 0, h);
+			if (nch > 1) ed_stage_a(S, c, j, 1, h);
 		}
 		__syncthreads();
+		if (helper) ed_stage_b(S, c, 0, h);
+		__syncthreads();
+		e1 = 0.0;
+		e2 = 0.0;
+		for (int ch = 0; ch < nch; ch++) {
+			if (tid == 0) {
+				ed_walk<LDS_ROWS, HAS_M1>(S, c, ch, e1, e2);
+			} else if (helper) {
+				if (ch >= 1) ed_finish(S, c, P, j, ch - 1, h);
+				if (ch + 1 < nch) ed_stage_b(S, c, ch + 1, h);
+				if (ch + 2 < nch) ed_stage_a(S, c, j, ch + 2, h);
+			}
+			__syncthreads();
+		}
 	}
+	if (helper) ed_finish(S, c, P, H - 1, nch - 1, h);
 }
 
 // ---- device memory and stream -------------------------------------------
@@ -1122,9 +1194,13 @@ extern "C" int iwamd_job_run(struct iwamd_job *J)
 			const int sub = (q.subtype >= 0 && q.subtype <= 7) ? q.subtype : 0;
 			m1 |= sub != 0 && sub != 6;  // Floyd-Steinberg and Sierra 4-2-1 have none
 		}
-		auto k = small ? (m1 ? errdiff_kernel<true, true> : errdiff_kernel<true, false>)
-			: (m1 ? errdiff_kernel<false, true> : errdiff_kernel<false, false>);
-		hipLaunchKernelGGL(k, dim3(J->n_ed), dim3(ED_CHUNK), 0, st, D, l[0], l[1], l[2], l[3]);
+		// Error rows in LDS up to ED_ROWS_LDS pixels wide.
+		const bool rows = D.out_w <= ED_ROWS_LDS;
+		auto k = small ? (m1 ? (rows ? errdiff_kernel<true, true, true> : errdiff_kernel<true, true, false>)
+		                     : (rows ? errdiff_kernel<true, false, true> : errdiff_kernel<true, false, false>))
+		               : (m1 ? (rows ? errdiff_kernel<false, true, true> : errdiff_kernel<false, true, false>)
+		                     : (rows ? errdiff_kernel<false, false, true> : errdiff_kernel<false, false, false>));
+		hipLaunchKernelGGL(k, dim3(J->n_ed), dim3(ED_THREADS), 0, st, D, l[0], l[1], l[2], l[3]);
 	}
 	hipEventRecord(J->ev[3], st);
 	if (hipGetLastError() != hipSuccess) {
