@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--out", type=int, default=1000)
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="after timing, run one plain device-to-device copy of MIB MiB (calibrates HBM counters)")
     return p.parse_args()
 
 
@@ -169,6 +171,14 @@ def main():
                   for f, v in forms.items()},
     }
     print(json.dumps(line), flush=True)
+    if a.copy_probe:
+        import torch
+        n = a.copy_probe << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes device-to-device" % n)
 
 
 if __name__ == "__main__":
