@@ -174,11 +174,12 @@ def main():
     if a.copy_probe:
         import torch
         n = a.copy_probe << 20
-        src = torch.empty(n, dtype=torch.uint8, device="cuda")
+        src = torch.zeros(n // 4, dtype=torch.int32, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
         torch.cuda.synchronize()
-        log("copy probe: %d bytes device-to-device" % n)
+        torch.add(src, 1, out=dst)  # one elementwise kernel: reads n bytes, writes n bytes
+        torch.cuda.synchronize()
+        log("copy probe: elementwise kernel over %d bytes" % n)
 
 
 if __name__ == "__main__":

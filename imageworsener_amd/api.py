@@ -101,6 +101,7 @@ _SIGS = {
     "iw_get_errorflag": (_c.c_int, [_c.c_void_p]),
     "iw_get_errormsg": (_c.c_char_p, [_c.c_void_p, _c.c_char_p, _c.c_int]),
     "iw_get_version_int": (_c.c_int, []),
+    "iw_set_option": (None, [_c.c_void_p, _c.c_char_p, _c.c_char_p]),
 }
 _EXT_SIGS = {
     "iwamd_prepare": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
@@ -242,6 +243,8 @@ class Library:
             d.iw_set_max_malloc(ctx, job["max_malloc"])
         if self.has_ext and "precision" in job:
             d.iwamd_set_precision(ctx, job["precision"])
+        for name, val in job.get("options", []):
+            d.iw_set_option(ctx, name.encode(), val.encode())
         d.iw_set_output_canvas_size(ctx, job["canvas"][0], job["canvas"][1])
 
     def _collect(self, ctx, ok):

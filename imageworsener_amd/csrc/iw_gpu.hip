@@ -43,6 +43,11 @@ struct DevWeights {
 	const double *w;
 	const int *copy_src;
 	double edge;
+	int edge_transparent;  // outside positions read edge (else the nearest sample)
+	// The list by input position (iwx::DenseWeights).
+	const int *dfirst, *dcount, *doff;
+	const double *dw;
+	int dmax;
 };
 
 struct DevIChan {
@@ -353,11 +358,14 @@ __device__ __forceinline__ double hgather(const DevPlan &P, int c, int j, int x)
 	return acc;
 }
 
-__global__ void __launch_bounds__(256) hpass_kernel(DevPlan P)
+// One output pixel: every channel's horizontally resampled value (alpha
+// first, rounded to FP32 like the reference's final_alpha32), unpremultiply,
+// late background, then store, quantise, or hand the sample to error
+// diffusion (src/imagew-main.c:1036-1123).  gather(c) returns the resampled
+// value of intermediate channel c at this pixel.
+template <class Gather>
+__device__ __forceinline__ void emit_pixel(const DevPlan &P, int i, int j, Gather gather)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	const int j = blockIdx.y;
-	if (i >= P.out_w) return;
 	float alpha_f = 0.0f;
 	int alt = 0;
 	if (P.checker)
@@ -366,7 +374,7 @@ __global__ void __launch_bounds__(256) hpass_kernel(DevPlan P)
 	for (int idx = 0; idx < P.inter_nch; idx++) {
 		const int c = P.order[idx];
 		const DevIChan &ic = P.ic[c];
-		double v = hgather(P, c, j, i);
+		double v = gather(c);
 		if (P.intclamp) v = clamp01(v);
 		const bool is_alpha = (c == P.inter_alpha);
 		if (is_alpha) alpha_f = (float)v;
@@ -395,6 +403,124 @@ __global__ void __launch_bounds__(256) hpass_kernel(DevPlan P)
 			continue;
 		}
 		store_sample(P, i, j, o, quantize(q, v, i, j));
+	}
+}
+
+__global__ void __launch_bounds__(256) hpass_kernel(DevPlan P)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int j = blockIdx.y;
+	if (i >= P.out_w) return;
+	emit_pixel(P, i, j, [&](int c) { return hgather(P, c, j, i); });
+}
+
+// ---- fused band kernel ------------------------------------------------------
+// One workgroup computes R consecutive output rows [j0, j0+R) completely: the
+// vertical pass for those rows over every intermediate column into LDS (FP32,
+// as the reference stores it), then the horizontal pass and the rest of the
+// per-pixel work from LDS.  The intermediate never reaches HBM.
+//
+// Vertical pass by input position: each thread owns a column and keeps R
+// accumulators per channel; it visits the band's input positions in
+// ascending order, reads each input sample once, and adds v*w into every
+// row's accumulator.  A row's weight list is in ascending position order
+// (src/imagew-resize.c:282-327), so each accumulator receives its terms in the
+// reference's order; positions a row does not list carry weight 0, which
+// leaves its bits unchanged (an accumulator is never -0.0 and samples are
+// finite).  In COPY mode (null / nearest) the row takes the sample itself.
+
+#define BAND_MAX_K 256  // band input positions held in LDS
+
+__device__ __forceinline__ double band_sample(const DevPlan &P, const DevIChan &ic, int x, int p, int num_in)
+{
+	if (p < 0 || p >= num_in) {
+		// Only TRANSPARENT and REPLICATE list outside positions.
+		if (ic.v.edge_transparent) return ic.v.edge;
+		p = p < 0 ? 0 : num_in - 1;
+	}
+	return colour_in(P, ic, x, p);
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) band_kernel(DevPlan P)
+{
+	extern __shared__ __align__(16) unsigned char band_lds[];
+	double *wd = (double *)band_lds;                       // [R][BAND_MAX_K]
+	float *inter = (float *)(band_lds + sizeof(double) * R * BAND_MAX_K);  // [C][R][W]
+	const int tid = threadIdx.x;
+	const int j0 = blockIdx.x * R;
+	const int rows = min(R, P.Hp - j0);
+	const int W = P.W, C = P.inter_nch;
+	const int num_in = P.ih;
+
+	for (int c = 0; c < C; c++) {
+		const DevIChan &ic = P.ic[c];
+		const DevWeights &dv = ic.v;
+		// The band's input positions.
+		int p_lo = 0x7fffffff, p_hi = -0x7fffffff;
+		for (int r = 0; r < rows; r++) {
+			const int n = dv.dcount[j0 + r];
+			if (n == 0) continue;
+			p_lo = min(p_lo, dv.dfirst[j0 + r]);
+			p_hi = max(p_hi, dv.dfirst[j0 + r] + n - 1);
+		}
+		const int K = (p_hi >= p_lo) ? p_hi - p_lo + 1 : 0;
+		__syncthreads();  // previous channel's weights no longer read
+		for (int t = tid; t < R * BAND_MAX_K; t += blockDim.x) {
+			const int r = t / BAND_MAX_K, k = t % BAND_MAX_K;
+			double w = 0.0;
+			if (r < rows && k < K) {
+				const int kk = p_lo + k - dv.dfirst[j0 + r];
+				if (kk >= 0 && kk < dv.dcount[j0 + r]) w = dv.dw[dv.doff[j0 + r] + kk];
+			}
+			wd[t] = w;
+		}
+		__syncthreads();
+		for (int x = tid; x < W; x += blockDim.x) {
+			double acc[R];
+#pragma unroll
+			for (int r = 0; r < R; r++) acc[r] = 0.0;
+			for (int k = 0; k < K; k++) {
+				const double v = band_sample(P, ic, x, p_lo + k, num_in);
+				if (dv.mode == iwx::WeightList::COPY) {
+#pragma unroll
+					for (int r = 0; r < R; r++)
+						if (wd[r * BAND_MAX_K + k] != 0.0) acc[r] = v;
+				} else {
+#pragma unroll
+					for (int r = 0; r < R; r++) acc[r] = acc[r] + v * wd[r * BAND_MAX_K + k];
+				}
+			}
+#pragma unroll
+			for (int r = 0; r < R; r++) {
+				double a = acc[r];
+				if (P.intclamp) a = clamp01(a);
+				inter[((size_t)c * R + r) * W + x] = (float)a;
+			}
+		}
+	}
+	__syncthreads();
+	// Horizontal pass and per-pixel work for the band's rows.
+	const int n_px = P.out_w * rows;
+	for (int t = tid; t < n_px; t += blockDim.x) {
+		const int r = t / P.out_w, i = t % P.out_w;
+		const int j = j0 + r;
+		emit_pixel(P, i, j, [&](int c) {
+			const DevWeights &h = P.ic[c].h;
+			const float *row = inter + ((size_t)c * R + r) * W;
+			if (h.mode == iwx::WeightList::COPY) {
+				const int s = h.copy_src[i];
+				return (s < 0) ? 0.0 : (double)row[s];
+			}
+			double acc = 0.0;
+			const int t1 = h.start[i + 1];
+			for (int u = h.start[i]; u < t1; u++) {
+				const int s = h.src[u];
+				const double v = (s < 0) ? h.edge : (double)row[s];
+				acc = acc + v * h.w[u];
+			}
+			return acc;
+		});
 	}
 }
 
@@ -754,6 +880,7 @@ struct Runtime {
 	// far more than a small image's kernels.
 	std::multimap<size_t, void *> free_blocks;
 	size_t cached_bytes = 0;
+	size_t band_lds_max = 64 * 1024;  // dynamic LDS a band kernel may use
 };
 
 Runtime &rt()
@@ -777,6 +904,16 @@ bool runtime_init(iw_context *ctx)
 	if (iwx::prng_mont_minv() != kMontMinv) {
 		iw_set_error(ctx, "Internal: PRNG Montgomery constant mismatch");
 		return false;
+	}
+	// The band kernels size their LDS per launch, up to the workgroup limit
+	// (160 KB on gfx950); where the runtime refuses, they keep 64 KB.
+	const void *bk[3] = {(const void *)band_kernel<1>, (const void *)band_kernel<2>, (const void *)band_kernel<4>};
+	R.band_lds_max = 160 * 1024;
+	for (const void *f : bk) {
+		if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)R.band_lds_max) != hipSuccess) {
+			(void)hipGetLastError();
+			R.band_lds_max = 64 * 1024;
+		}
 	}
 	R.init = true;
 	return true;
@@ -849,11 +986,17 @@ struct iwamd_job {
 	int n_ed = 0;
 	int ed_list[4] = {-1, -1, -1, -1};
 	double out_pixels = 0.0, alg_bytes = 0.0;
+	int band_rows = 0;  // rows per band of the fused kernel; 0: two-pass kernels
 	bool ran = false;
 	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
 };
 
 namespace {
+
+size_t band_lds_bytes(int R, int C, int W)
+{
+	return sizeof(double) * (size_t)R * BAND_MAX_K + sizeof(float) * (size_t)C * R * W;
+}
 
 bool job_upload(iwamd_job *J)
 {
@@ -885,6 +1028,8 @@ bool job_upload(iwamd_job *J)
 			J->skip_pixels = true;
 		}
 	}
+	std::vector<DenseWeights> dv(p.inter_nch);
+	for (int c = 0; c < p.inter_nch; c++) dv[c] = dense_weights(wv[c]);
 	const bool in_int = p.in_sampletype != IW_SAMPLETYPE_FLOATINGPOINT;
 	const int nvalues = in_int ? (1 << p.in_bit_depth) : 0;
 	std::vector<std::vector<double>> luts;
@@ -970,7 +1115,7 @@ bool job_upload(iwamd_job *J)
 	const size_t o_ede = L.add<double>((size_t)J->n_ed * 4 * p.out_width);
 	std::vector<size_t> o_lut(luts.size());
 	for (size_t i = 0; i < luts.size(); i++) o_lut[i] = L.add<double>(luts[i].size());
-	struct WOff { size_t start, src, w, copy; };
+	struct WOff { size_t start, src, w, copy, dfirst, dcount, doff, dw; };
 	std::vector<WOff> ov(p.inter_nch), oh(p.inter_nch);
 	auto place_w = [&](const WeightList &wl, WOff &o) {
 		o.start = L.add<int>(wl.start.size());
@@ -981,6 +1126,10 @@ bool job_upload(iwamd_job *J)
 	for (int c = 0; c < p.inter_nch; c++) {
 		place_w(wv[c], ov[c]);
 		place_w(wh[c], oh[c]);
+		ov[c].dfirst = L.add<int>(dv[c].first.size());
+		ov[c].dcount = L.add<int>(dv[c].count.size());
+		ov[c].doff = L.add<int>(dv[c].off.size());
+		ov[c].dw = L.add<double>(dv[c].w.size());
 	}
 	struct QOff { size_t T, U, lin, code, nc, prow, pcol, M, pre, suf; };
 	std::vector<QOff> oq(p.out_nch);
@@ -1019,6 +1168,10 @@ bool job_upload(iwamd_job *J)
 	for (int c = 0; c < p.inter_nch; c++) {
 		up_w(wv[c], ov[c]);
 		up_w(wh[c], oh[c]);
+		up(ov[c].dfirst, dv[c].first.data(), dv[c].first.size() * 4);
+		up(ov[c].dcount, dv[c].count.data(), dv[c].count.size() * 4);
+		up(ov[c].doff, dv[c].off.data(), dv[c].off.size() * 4);
+		up(ov[c].dw, dv[c].w.data(), dv[c].w.size() * 8);
 	}
 	for (int o = 0; o < (int)qt.size(); o++) {
 		up(oq[o].T, qt[o].T.data(), qt[o].T.size() * 8);
@@ -1079,16 +1232,22 @@ bool job_upload(iwamd_job *J)
 		d.bkgd = s.bkgd_color_lin;
 		for (int k = 0; k < 3; k++)
 			d.lut[k] = (in_int && ic_lut[c][k] >= 0) ? (const double *)(base + o_lut[ic_lut[c][k]]) : nullptr;
-		auto fill = [&](const WeightList &wl, const WOff &o, DevWeights &dw) {
+		auto fill = [&](const WeightList &wl, const WOff &o, DevWeights &dw, int dim) {
 			dw.mode = wl.mode;
 			dw.start = (const int *)(base + o.start);
 			dw.src = (const int *)(base + o.src);
 			dw.w = (const double *)(base + o.w);
 			dw.copy_src = (const int *)(base + o.copy);
 			dw.edge = wl.edge_value;
+			dw.edge_transparent = p.rs[dim].edge_policy == IW_EDGE_POLICY_TRANSPARENT;
 		};
-		fill(wv[c], ov[c], d.v);
-		fill(wh[c], oh[c], d.h);
+		fill(wv[c], ov[c], d.v, IW_DIMENSION_V);
+		fill(wh[c], oh[c], d.h, IW_DIMENSION_H);
+		d.v.dfirst = (const int *)(base + ov[c].dfirst);
+		d.v.dcount = (const int *)(base + ov[c].dcount);
+		d.v.doff = (const int *)(base + ov[c].doff);
+		d.v.dw = (const double *)(base + ov[c].dw);
+		d.v.dmax = dv[c].max_count;
 	}
 	D.inter = (float *)(base + o_inter);
 	D.intclamp = p.intclamp;
@@ -1137,6 +1296,37 @@ bool job_upload(iwamd_job *J)
 	D.bk1a = p.bkgd1alpha;
 	D.bk2a = p.bkgd2alpha;
 
+	// Resampling path: the fused band kernel when a band of rows fits in LDS,
+	// else the two-pass kernels with the intermediate in HBM.
+	J->band_rows = 0;
+	{
+		const char *opt = iw_get_option(ctx, "amd:resample");
+		const bool force_two = opt && !strcmp(opt, "twopass");
+		const bool force_band = opt && !strcmp(opt, "band");
+		const int cands[3] = {4, 2, 1};
+		for (int ci = 0; ci < 3 && !force_two && !J->band_rows; ci++) {
+			const int R = cands[ci];
+			if (band_lds_bytes(R, p.inter_nch, p.inter_width) > rt().band_lds_max) continue;
+			// Widest band (input positions) over every channel's list.
+			bool fits = true;
+			for (int c = 0; c < p.inter_nch && fits; c++) {
+				const DenseWeights &d = dv[c];
+				for (int j0 = 0; j0 < p.inter_height && fits; j0 += R) {
+					int lo = 0x7fffffff, hi = -0x7fffffff;
+					for (int r = 0; r < R && j0 + r < p.inter_height; r++) {
+						if (!d.count[j0 + r]) continue;
+						lo = std::min(lo, d.first[j0 + r]);
+						hi = std::max(hi, d.first[j0 + r] + d.count[j0 + r] - 1);
+					}
+					if (hi >= lo && hi - lo + 1 > BAND_MAX_K) fits = false;
+				}
+			}
+			if (fits) J->band_rows = R;
+		}
+		// Wide rows keep more workgroups per CU in the two-pass form.
+		if (!force_band && J->band_rows && p.inter_width > 2048) J->band_rows = 0;
+	}
+
 	const double in_samples_bytes = (double)in_bytes;
 	J->out_pixels = (double)p.out_width * p.out_height;
 	J->alg_bytes = in_samples_bytes + (double)J->out_bytes;
@@ -1173,14 +1363,21 @@ extern "C" int iwamd_job_run(struct iwamd_job *J)
 	const DevPlan &D = J->D;
 	hipStream_t st = J->stream;
 	hipEventRecord(J->ev[0], st);
-	{
+	if (J->band_rows) {
+		// One fused kernel; the horizontal-pass slot of the timings stays 0.
+		const int R = J->band_rows;
+		const dim3 grid((D.Hp + R - 1) / R);
+		const size_t lds = band_lds_bytes(R, D.inter_nch, D.W);
+		if (R == 4) hipLaunchKernelGGL(band_kernel<4>, grid, dim3(256), lds, st, D);
+		else if (R == 2) hipLaunchKernelGGL(band_kernel<2>, grid, dim3(256), lds, st, D);
+		else hipLaunchKernelGGL(band_kernel<1>, grid, dim3(256), lds, st, D);
+		hipEventRecord(J->ev[1], st);
+	} else {
 		dim3 grid((D.W + 255) / 256, D.Hp);
 		hipLaunchKernelGGL(vpass_kernel, grid, dim3(256), 0, st, D);
-	}
-	hipEventRecord(J->ev[1], st);
-	{
-		dim3 grid((D.out_w + 255) / 256, D.out_h);
-		hipLaunchKernelGGL(hpass_kernel, grid, dim3(256), 0, st, D);
+		hipEventRecord(J->ev[1], st);
+		dim3 grid2((D.out_w + 255) / 256, D.out_h);
+		hipLaunchKernelGGL(hpass_kernel, grid2, dim3(256), 0, st, D);
 	}
 	hipEventRecord(J->ev[2], st);
 	if (J->n_ed > 0) {

@@ -187,6 +187,7 @@ WeightList build_weight_list(const iwamd_resize &rs, int channeltype, double edg
 			else if (rs.edge_policy == IW_EDGE_POLICY_TRANSPARENT) s = -1;
 			else s = (i < 0) ? 0 : rs.num_in - 1;
 			wl.src.push_back(s);
+			wl.pos.push_back(i);
 			wl.w.push_back(v);
 		}
 		// Normalise by division by the running sum, in list order; a zero sum
@@ -312,6 +313,34 @@ QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc)
 		}
 	}
 	return q;
+}
+
+DenseWeights dense_weights(const WeightList &wl)
+{
+	DenseWeights d;
+	const int n = wl.num_out;
+	d.first.assign(n, 0);
+	d.count.assign(n, 0);
+	d.off.assign(n, 0);
+	for (int o = 0; o < n; o++) {
+		d.off[o] = (int)d.w.size();
+		if (wl.mode == WeightList::COPY) {
+			if (wl.copy_src[o] >= 0) {
+				d.first[o] = wl.copy_src[o];
+				d.count[o] = 1;
+				d.w.push_back(1.0);
+			}
+		} else if (wl.start[o + 1] > wl.start[o]) {
+			const int a = wl.start[o], b = wl.start[o + 1];
+			const int p0 = wl.pos[a], p1 = wl.pos[b - 1];
+			d.first[o] = p0;
+			d.count[o] = p1 - p0 + 1;
+			d.w.resize(d.off[o] + d.count[o], 0.0);
+			for (int t = a; t < b; t++) d.w[d.off[o] + (wl.pos[t] - p0)] = wl.w[t];
+		}
+		d.max_count = std::max(d.max_count, d.count[o]);
+	}
+	return d;
 }
 
 EdTables build_ed_tables(const QuantTable &q)

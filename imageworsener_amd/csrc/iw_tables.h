@@ -26,9 +26,23 @@ struct WeightList {
 	std::vector<int> src;
 	std::vector<double> w;
 	std::vector<int> copy_src;   // num_out
+	std::vector<int> pos;        // input position of each entry (may lie outside 0..num_in-1)
 	double edge_value = 0.0;     // value of virtual pixels
 	bool ok = true;              // false: unknown family
 };
+
+// The same list by input position: output o reads positions
+// first[o] .. first[o]+count[o]-1 with weights w[off[o] + k], zero where the
+// list has no entry (a dropped zero weight: adding v*0 to an accumulator that
+// is never -0.0 leaves its bits unchanged).  A position outside 0..num_in-1
+// reads the edge value (TRANSPARENT) or the nearest sample (REPLICATE); the
+// STANDARD policy never lists one.  In COPY mode count is 1 (weight 1) or 0.
+struct DenseWeights {
+	std::vector<int> first, count, off;
+	std::vector<double> w;
+	int max_count = 0;
+};
+DenseWeights dense_weights(const WeightList &wl);
 
 // src/imagew-resize.c:412-556 and :240-344.
 WeightList build_weight_list(const iwamd_resize &rs, int channeltype, double edge_value);

@@ -56,9 +56,20 @@ def test_c2_reduced(prod, ref, dither):
     cases.assert_same(ref.run(job), prod.run(job), "C2/4 " + dither)
 
 
+@pytest.mark.parametrize("path", ["band", "twopass"])
 @pytest.mark.parametrize("name,job", cases.sweep_jobs(), ids=[n for n, _ in cases.sweep_jobs()])
-def test_sweep(prod, ref, name, job):
-    cases.assert_same(ref.run(job), prod.run(job), name)
+def test_sweep(prod, ref, name, job, path):
+    """Every sweep case through the fused band kernel and through the
+    two-pass kernels."""
+    j = dict(job, options=[("amd:resample", path)])
+    cases.assert_same(ref.run(job), prod.run(j), name + " " + path)
+
+
+@pytest.mark.parametrize("path", ["band", "twopass"])
+def test_c1_reduced_paths(prod, ref, path):
+    job = cases.c1_job(size=1024, out=250)
+    say("C1/4 via " + path)
+    cases.assert_same(ref.run(job), prod.run(dict(job, options=[("amd:resample", path)])), "C1/4 " + path)
 
 
 def test_c1_full(prod, ref):
