@@ -172,14 +172,12 @@ def main():
     }
     print(json.dumps(line), flush=True)
     if a.copy_probe:
-        import torch
         n = a.copy_probe << 20
-        src = torch.zeros(n // 4, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        torch.add(src, 1, out=dst)  # one elementwise kernel: reads n bytes, writes n bytes
-        torch.cuda.synchronize()
-        log("copy probe: elementwise kernel over %d bytes" % n)
+        t = ctypes.c_double()
+        lib.dll.iwamd_copy_probe.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
+        if not lib.dll.iwamd_copy_probe(n, ctypes.byref(t)):
+            raise RuntimeError("copy probe failed")
+        log("copy probe: %d bytes in %.3f ms (%.1f GB/s read+write)" % (n, t.value, 2 * n / t.value / 1e6))
 
 
 if __name__ == "__main__":

@@ -1464,6 +1464,46 @@ extern "C" void iwamd_job_destroy(struct iwamd_job *J)
 	delete J;
 }
 
+// ---- HBM counter calibration ---------------------------------------------
+
+__global__ void __launch_bounds__(256) copy_probe_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+	size_t n)
+{
+	for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+		dst[k] = src[k];
+}
+
+extern "C" int iwamd_copy_probe(size_t bytes, double *ms)
+{
+	const size_t n = bytes / sizeof(uint4);
+	if (n == 0) return 0;
+	void *a = nullptr, *b = nullptr;
+	if (hipMalloc(&a, n * sizeof(uint4)) != hipSuccess) return 0;
+	if (hipMalloc(&b, n * sizeof(uint4)) != hipSuccess) {
+		(void)hipFree(a);
+		return 0;
+	}
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+	ok = ok && hipMemset(a, 0, n * sizeof(uint4)) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+	if (ok) {
+		hipEventRecord(e0, 0);
+		hipLaunchKernelGGL(copy_probe_kernel, dim3(4096), dim3(256), 0, 0, (const uint4 *)a, (uint4 *)b, n);
+		hipEventRecord(e1, 0);
+		ok = hipEventSynchronize(e1) == hipSuccess;
+		float t = 0.0f;
+		if (ok && ms) {
+			hipEventElapsedTime(&t, e0, e1);
+			*ms = t;
+		}
+	}
+	if (e0) hipEventDestroy(e0);
+	if (e1) hipEventDestroy(e1);
+	(void)hipFree(a);
+	(void)hipFree(b);
+	return ok ? 1 : 0;
+}
+
 int iwx_gpu_device_count(void)
 {
 	int n = 0;

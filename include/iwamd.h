@@ -172,6 +172,11 @@ IW_EXPORT(void) iwamd_job_sizes(struct iwamd_job *job, double *out_pixels, doubl
 /* Library/device information.  Returns the number of visible HIP devices
  * (0 when none), and fills name (may be NULL). */
 IW_EXPORT(int) iwamd_device_count(void);
+/* HBM counter calibration: copies `bytes` (rounded down to 16) between two
+ * fresh device buffers with one plain copy kernel (copy_probe_kernel) and
+ * reports its time in *ms.  Returns 1 on success.  Profilers read its
+ * FETCH_SIZE / WRITE_SIZE against the known byte count. */
+IW_EXPORT(int) iwamd_copy_probe(size_t bytes, double *ms);
 IW_EXPORT(const char*) iwamd_version(void);
 
 /* Per-stage device timings of the last iw_process_image on this context, in
