@@ -993,16 +993,69 @@ struct iwamd_job {
 
 namespace {
 
+// Host side of one job's device data: the tables, where each lives in the
+// job's part of a device block, and the resampling path.
+struct WOff { size_t start, src, w, copy, dfirst, dcount, doff, dw; };
+struct QOff { size_t T, U, lin, code, nc, prow, pcol, M, pre, suf; };
+
+struct Staged {
+	std::vector<iwx::WeightList> wv, wh;
+	std::vector<iwx::DenseWeights> dv;
+	std::vector<std::vector<double>> luts;
+	int ic_lut[4][3];
+	int alpha_lut = -1;
+	std::vector<iwx::QuantTable> qt;
+	std::vector<iwx::EdTables> et;
+	std::vector<iwx::PrngTables> pt;
+	int ed_of[4] = {-1, -1, -1, -1};
+	int n_ed = 0;
+	int ed_list[4] = {-1, -1, -1, -1};
+	bool in_int = true;
+	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
+	int band_rows = 0;         // rows per band of the fused kernel; 0: two-pass kernels
+	size_t in_bytes = 0, out_bytes = 0;
+	size_t o_in = 0, o_inter = 0, o_out = 0, o_eds = 0, o_ede = 0;
+	std::vector<size_t> o_lut;
+	std::vector<WOff> ov, oh;
+	std::vector<QOff> oq;
+	size_t bytes = 0;          // size of the job's part of a block
+};
+
 size_t band_lds_bytes(int R, int C, int W)
 {
 	return sizeof(double) * (size_t)R * BAND_MAX_K + sizeof(float) * (size_t)C * R * W;
 }
 
-bool job_upload(iwamd_job *J)
+// Largest band height (4, 2 or 1 rows) whose LDS and input-position span fit,
+// or 0.
+int choose_band_rows(const iwamd_plan &p, const std::vector<iwx::DenseWeights> &dv)
+{
+	const int cands[3] = {4, 2, 1};
+	for (int R : cands) {
+		if (band_lds_bytes(R, p.inter_nch, p.inter_width) > rt().band_lds_max) continue;
+		bool fits = true;
+		for (int c = 0; c < p.inter_nch && fits; c++) {
+			const iwx::DenseWeights &d = dv[c];
+			for (int j0 = 0; j0 < p.inter_height && fits; j0 += R) {
+				int lo = 0x7fffffff, hi = -0x7fffffff;
+				for (int r = 0; r < R && j0 + r < p.inter_height; r++) {
+					if (!d.count[j0 + r]) continue;
+					lo = std::min(lo, d.first[j0 + r]);
+					hi = std::max(hi, d.first[j0 + r] + d.count[j0 + r] - 1);
+				}
+				if (hi >= lo && hi - lo + 1 > BAND_MAX_K) fits = false;
+			}
+		}
+		if (fits) return R;
+	}
+	return 0;
+}
+
+// Build the host tables of a plan, choose its path and lay out its device
+// data (offsets from the job's base).  path: "band", "twopass" or null.
+bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path)
 {
 	using namespace iwx;
-	iw_context *ctx = J->ctx;
-	const iwamd_plan &p = J->plan;
 	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT) {
 		for (int c = 0; c < p.inter_nch; c++) {
 			if (p.ich[c].in_cs.cstype != IW_CSTYPE_LINEAR) {
@@ -1015,38 +1068,34 @@ bool job_upload(iwamd_job *J)
 			return false;
 		}
 	}
-
-	// ---- host tables ----
-	std::vector<WeightList> wv(p.inter_nch), wh(p.inter_nch);
+	S.wv.resize(p.inter_nch);
+	S.wh.resize(p.inter_nch);
+	S.dv.resize(p.inter_nch);
 	for (int c = 0; c < p.inter_nch; c++) {
-		wv[c] = build_weight_list(p.rs[IW_DIMENSION_V], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_V]);
-		wh[c] = build_weight_list(p.rs[IW_DIMENSION_H], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_H]);
-		if (!wv[c].ok || !wh[c].ok) {
+		S.wv[c] = build_weight_list(p.rs[IW_DIMENSION_V], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_V]);
+		S.wh[c] = build_weight_list(p.rs[IW_DIMENSION_H], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_H]);
+		if (!S.wv[c].ok || !S.wh[c].ok) {
 			// The reference reports the error and carries on
 			// (src/imagew-resize.c:512-515).
 			iw_set_error(ctx, "Internal: Unknown resize algorithm");
-			J->skip_pixels = true;
+			S.skip_pixels = true;
 		}
+		S.dv[c] = dense_weights(S.wv[c]);
 	}
-	std::vector<DenseWeights> dv(p.inter_nch);
-	for (int c = 0; c < p.inter_nch; c++) dv[c] = dense_weights(wv[c]);
-	const bool in_int = p.in_sampletype != IW_SAMPLETYPE_FLOATINGPOINT;
-	const int nvalues = in_int ? (1 << p.in_bit_depth) : 0;
-	std::vector<std::vector<double>> luts;
+	S.in_int = p.in_sampletype != IW_SAMPLETYPE_FLOATINGPOINT;
+	const int nvalues = S.in_int ? (1 << p.in_bit_depth) : 0;
 	std::map<std::tuple<int, double, double>, int> lut_index;
 	auto get_lut = [&](double maxcc, const struct iw_csdescr &cs) {
 		auto key = std::make_tuple(cs.cstype, cs.cstype == IW_CSTYPE_GAMMA ? cs.gamma : 0.0, maxcc);
 		auto it = lut_index.find(key);
 		if (it != lut_index.end()) return it->second;
-		int id = (int)luts.size();
-		luts.push_back(build_input_lut(nvalues, maxcc, cs));
+		int id = (int)S.luts.size();
+		S.luts.push_back(build_input_lut(nvalues, maxcc, cs));
 		lut_index[key] = id;
 		return id;
 	};
-	int ic_lut[4][3];
-	for (auto &r : ic_lut) r[0] = r[1] = r[2] = -1;
-	int alpha_lut = -1;
-	if (in_int) {
+	for (auto &r : S.ic_lut) r[0] = r[1] = r[2] = -1;
+	if (S.in_int) {
 		for (int c = 0; c < p.inter_nch; c++) {
 			const iwamd_ichan &ic = p.ich[c];
 			const int nk = ic.cvt_to_grayscale ? 3 : 1;
@@ -1056,38 +1105,36 @@ bool job_upload(iwamd_job *J)
 				// Fast path: input max code; slow path: the channel's own
 				// (src/imagew-main.c:307-383).
 				const double mcc = p.in_ch_slow[ic.input_channel] ? p.in_ch_maxcolorcode[ch] : p.in_maxcolorcode;
-				ic_lut[c][k] = get_lut(mcc, ic.in_cs);
+				S.ic_lut[c][k] = get_lut(mcc, ic.in_cs);
 			}
 		}
 		if (p.in_has_alpha && p.in_alpha_index < p.in_nch_physical) {
 			struct iw_csdescr lin;
 			iw_make_linear_csdescr(&lin);
-			alpha_lut = get_lut(p.in_ch_maxcolorcode[p.in_alpha_index], lin);
+			S.alpha_lut = get_lut(p.in_ch_maxcolorcode[p.in_alpha_index], lin);
 		}
 	}
-	std::vector<QuantTable> qt(p.out_nch);
-	std::vector<EdTables> et(p.out_nch);
-	std::vector<PrngTables> pt(p.out_nch);
-	int ed_of[4] = {-1, -1, -1, -1};
-	J->n_ed = 0;
+	S.qt.resize(p.out_nch);
+	S.et.resize(p.out_nch);
+	S.pt.resize(p.out_nch);
 	if (p.out_sampletype != IW_SAMPLETYPE_FLOATINGPOINT) {
 		for (int o = 0; o < p.out_nch; o++) {
-			qt[o] = build_quant_table(p.och[o], p.och[o].use_nc_table != 0);
+			S.qt[o] = build_quant_table(p.och[o], p.och[o].use_nc_table != 0);
 			if (p.och[o].ditherfamily == IW_DITHERFAMILY_RANDOM) {
-				pt[o] = build_prng_tables(p.och[o].random_seed, p.out_width, p.out_height);
-				if (!pt[o].ok) {
+				S.pt[o] = build_prng_tables(p.och[o].random_seed, p.out_width, p.out_height);
+				if (!S.pt[o].ok) {
 					iw_set_error(ctx, "Random seed not supported by the GPU path");
 					return false;
 				}
 			}
 			if (p.och[o].ditherfamily == IW_DITHERFAMILY_ERRDIFF) {
-				et[o] = build_ed_tables(qt[o]);
-				if (qt[o].P > 65535) {
+				S.et[o] = build_ed_tables(S.qt[o]);
+				if (S.qt[o].P > 65535) {
 					iw_set_error(ctx, "Too many output levels for error diffusion on the GPU path");
 					return false;
 				}
-				ed_of[o] = J->n_ed;
-				J->ed_list[J->n_ed++] = o;
+				S.ed_of[o] = S.n_ed;
+				S.ed_list[S.n_ed++] = o;
 			}
 		}
 	}
@@ -1104,19 +1151,26 @@ bool job_upload(iwamd_job *J)
 	// logical width and height (src/imagew-api.c:660-684).
 	const int phys_rows = (p.orient_transform & 4) ? p.in_width : p.in_height;
 
-	// ---- device layout: one block per job ----
+	// Resampling path: the fused band kernel when a band of rows fits in LDS
+	// (wide rows keep more workgroups per CU in the two-pass form), else the
+	// two-pass kernels with the intermediate in HBM.
+	const bool force_two = path && !strcmp(path, "twopass");
+	const bool force_band = path && !strcmp(path, "band");
+	S.band_rows = force_two ? 0 : choose_band_rows(p, S.dv);
+	if (!force_band && S.band_rows && p.inter_width > 2048) S.band_rows = 0;
+
 	Layout L;
-	const size_t in_bytes = p.in_bpr * (size_t)phys_rows;
-	const size_t o_in = L.add<uint8_t>(in_bytes);
-	const size_t o_inter = L.add<float>((size_t)p.inter_nch * p.inter_height * p.inter_width);
-	J->out_bytes = p.out_bpr * (size_t)p.out_height;
-	J->out_off = L.add<uint8_t>(J->out_bytes);
-	const size_t o_eds = L.add<double>((size_t)J->n_ed * p.out_height * p.out_width);
-	const size_t o_ede = L.add<double>((size_t)J->n_ed * 4 * p.out_width);
-	std::vector<size_t> o_lut(luts.size());
-	for (size_t i = 0; i < luts.size(); i++) o_lut[i] = L.add<double>(luts[i].size());
-	struct WOff { size_t start, src, w, copy, dfirst, dcount, doff, dw; };
-	std::vector<WOff> ov(p.inter_nch), oh(p.inter_nch);
+	S.in_bytes = p.in_bpr * (size_t)phys_rows;
+	S.o_in = L.add<uint8_t>(S.in_bytes);
+	S.o_inter = L.add<float>(S.band_rows ? 0 : (size_t)p.inter_nch * p.inter_height * p.inter_width);
+	S.out_bytes = p.out_bpr * (size_t)p.out_height;
+	S.o_out = L.add<uint8_t>(S.out_bytes);
+	S.o_eds = L.add<double>((size_t)S.n_ed * p.out_height * p.out_width);
+	S.o_ede = L.add<double>((size_t)S.n_ed * 4 * p.out_width);
+	S.o_lut.resize(S.luts.size());
+	for (size_t i = 0; i < S.luts.size(); i++) S.o_lut[i] = L.add<double>(S.luts[i].size());
+	S.ov.resize(p.inter_nch);
+	S.oh.resize(p.inter_nch);
 	auto place_w = [&](const WeightList &wl, WOff &o) {
 		o.start = L.add<int>(wl.start.size());
 		o.src = L.add<int>(wl.src.size());
@@ -1124,83 +1178,81 @@ bool job_upload(iwamd_job *J)
 		o.copy = L.add<int>(wl.copy_src.size());
 	};
 	for (int c = 0; c < p.inter_nch; c++) {
-		place_w(wv[c], ov[c]);
-		place_w(wh[c], oh[c]);
-		ov[c].dfirst = L.add<int>(dv[c].first.size());
-		ov[c].dcount = L.add<int>(dv[c].count.size());
-		ov[c].doff = L.add<int>(dv[c].off.size());
-		ov[c].dw = L.add<double>(dv[c].w.size());
+		place_w(S.wv[c], S.ov[c]);
+		place_w(S.wh[c], S.oh[c]);
+		S.ov[c].dfirst = L.add<int>(S.dv[c].first.size());
+		S.ov[c].dcount = L.add<int>(S.dv[c].count.size());
+		S.ov[c].doff = L.add<int>(S.dv[c].off.size());
+		S.ov[c].dw = L.add<double>(S.dv[c].w.size());
 	}
-	struct QOff { size_t T, U, lin, code, nc, prow, pcol, M, pre, suf; };
-	std::vector<QOff> oq(p.out_nch);
-	for (int o = 0; o < (int)qt.size(); o++) {
-		oq[o].T = L.add<double>(qt[o].T.size());
-		oq[o].U = L.add<double>(qt[o].U.size());
-		oq[o].lin = L.add<double>(qt[o].lin.size());
-		oq[o].code = L.add<double>(qt[o].code.size());
-		oq[o].nc = L.add<double>(qt[o].nc.size());
-		oq[o].prow = L.add<uint64_t>(pt[o].row_q.size());
-		oq[o].pcol = L.add<uint64_t>(pt[o].col_mont.size());
-		oq[o].M = L.add<double>(et[o].M.size());
-		oq[o].pre = L.add<double>(et[o].pre.size());
-		oq[o].suf = L.add<double>(et[o].suf.size());
+	S.oq.resize(p.out_nch);
+	for (int o = 0; o < (int)S.qt.size(); o++) {
+		QOff &q = S.oq[o];
+		q.T = L.add<double>(S.qt[o].T.size());
+		q.U = L.add<double>(S.qt[o].U.size());
+		q.lin = L.add<double>(S.qt[o].lin.size());
+		q.code = L.add<double>(S.qt[o].code.size());
+		q.nc = L.add<double>(S.qt[o].nc.size());
+		q.prow = L.add<uint64_t>(S.pt[o].row_q.size());
+		q.pcol = L.add<uint64_t>(S.pt[o].col_mont.size());
+		q.M = L.add<double>(S.et[o].M.size());
+		q.pre = L.add<double>(S.et[o].pre.size());
+		q.suf = L.add<double>(S.et[o].suf.size());
 	}
-	J->block_bytes = L.off + 256;
-	J->block = dev_alloc(J->block_bytes);
-	if (!J->block) {
-		iw_set_error(ctx, "Out of device memory");
-		return false;
-	}
-	char *base = (char *)J->block;
-	hipStream_t st = J->stream;
+	S.bytes = (L.off + 255) & ~(size_t)255;
+	return true;
+}
+
+// Queue the host-to-device copies of a staged job into its part of a block.
+// The host tables must live until the stream has been synchronised.
+bool upload_job(const Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
+{
 	bool ok = true;
 	auto up = [&](size_t off, const void *src, size_t bytes) {
 		if (bytes) ok &= hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
 	};
-	up(o_in, p.in_pixels, in_bytes);
-	for (size_t i = 0; i < luts.size(); i++) up(o_lut[i], luts[i].data(), luts[i].size() * 8);
-	auto up_w = [&](const WeightList &wl, const WOff &o) {
+	up(S.o_in, p.in_pixels, S.in_bytes);
+	for (size_t i = 0; i < S.luts.size(); i++) up(S.o_lut[i], S.luts[i].data(), S.luts[i].size() * 8);
+	auto up_w = [&](const iwx::WeightList &wl, const WOff &o) {
 		up(o.start, wl.start.data(), wl.start.size() * 4);
 		up(o.src, wl.src.data(), wl.src.size() * 4);
 		up(o.w, wl.w.data(), wl.w.size() * 8);
 		up(o.copy, wl.copy_src.data(), wl.copy_src.size() * 4);
 	};
 	for (int c = 0; c < p.inter_nch; c++) {
-		up_w(wv[c], ov[c]);
-		up_w(wh[c], oh[c]);
-		up(ov[c].dfirst, dv[c].first.data(), dv[c].first.size() * 4);
-		up(ov[c].dcount, dv[c].count.data(), dv[c].count.size() * 4);
-		up(ov[c].doff, dv[c].off.data(), dv[c].off.size() * 4);
-		up(ov[c].dw, dv[c].w.data(), dv[c].w.size() * 8);
+		up_w(S.wv[c], S.ov[c]);
+		up_w(S.wh[c], S.oh[c]);
+		up(S.ov[c].dfirst, S.dv[c].first.data(), S.dv[c].first.size() * 4);
+		up(S.ov[c].dcount, S.dv[c].count.data(), S.dv[c].count.size() * 4);
+		up(S.ov[c].doff, S.dv[c].off.data(), S.dv[c].off.size() * 4);
+		up(S.ov[c].dw, S.dv[c].w.data(), S.dv[c].w.size() * 8);
 	}
-	for (int o = 0; o < (int)qt.size(); o++) {
-		up(oq[o].T, qt[o].T.data(), qt[o].T.size() * 8);
-		up(oq[o].U, qt[o].U.data(), qt[o].U.size() * 8);
-		up(oq[o].lin, qt[o].lin.data(), qt[o].lin.size() * 8);
-		up(oq[o].code, qt[o].code.data(), qt[o].code.size() * 8);
-		up(oq[o].nc, qt[o].nc.data(), qt[o].nc.size() * 8);
-		up(oq[o].prow, pt[o].row_q.data(), pt[o].row_q.size() * 8);
-		up(oq[o].pcol, pt[o].col_mont.data(), pt[o].col_mont.size() * 8);
-		up(oq[o].M, et[o].M.data(), et[o].M.size() * 8);
-		up(oq[o].pre, et[o].pre.data(), et[o].pre.size() * 8);
-		up(oq[o].suf, et[o].suf.data(), et[o].suf.size() * 8);
+	for (int o = 0; o < (int)S.qt.size(); o++) {
+		const QOff &q = S.oq[o];
+		up(q.T, S.qt[o].T.data(), S.qt[o].T.size() * 8);
+		up(q.U, S.qt[o].U.data(), S.qt[o].U.size() * 8);
+		up(q.lin, S.qt[o].lin.data(), S.qt[o].lin.size() * 8);
+		up(q.code, S.qt[o].code.data(), S.qt[o].code.size() * 8);
+		up(q.nc, S.qt[o].nc.data(), S.qt[o].nc.size() * 8);
+		up(q.prow, S.pt[o].row_q.data(), S.pt[o].row_q.size() * 8);
+		up(q.pcol, S.pt[o].col_mont.data(), S.pt[o].col_mont.size() * 8);
+		up(q.M, S.et[o].M.data(), S.et[o].M.size() * 8);
+		up(q.pre, S.et[o].pre.data(), S.et[o].pre.size() * 8);
+		up(q.suf, S.et[o].suf.data(), S.et[o].suf.size() * 8);
 	}
-	// Host vectors are freed when this function returns: wait for the copies.
-	ok &= hipStreamSynchronize(st) == hipSuccess;
-	if (!ok) {
-		iw_set_error(ctx, "Host-to-device copy failed");
-		return false;
-	}
+	return ok;
+}
 
-	// ---- kernel arguments ----
-	DevPlan &D = J->D;
+// Kernel arguments of a staged job placed at base.
+void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
+{
 	memset(&D, 0, sizeof(D));
-	D.in = (const uint8_t *)(base + o_in);
+	D.in = (const uint8_t *)(base + S.o_in);
 	D.in_bpr = p.in_bpr;
 	D.in_w = p.in_width;
 	D.in_h = p.in_height;
 	D.bit_depth = p.in_bit_depth;
-	D.in_float = !in_int;
+	D.in_float = !S.in_int;
 	D.nch_phys = p.in_nch_physical;
 	D.orient = p.orient_transform;
 	D.sx = p.input_start_x;
@@ -1208,7 +1260,7 @@ bool job_upload(iwamd_job *J)
 	D.iw = p.input_w;
 	D.ih = p.input_h;
 	D.alpha_in_index = p.in_has_alpha ? p.in_alpha_index : 1000;
-	D.alpha_lut = alpha_lut >= 0 ? (const double *)(base + o_lut[alpha_lut]) : nullptr;
+	D.alpha_lut = S.alpha_lut >= 0 ? (const double *)(base + S.o_lut[S.alpha_lut]) : nullptr;
 	D.gsf = p.grayscale_formula;
 	for (int k = 0; k < 3; k++) D.gsw[k] = p.grayscale_weight[k];
 	D.inter_nch = p.inter_nch;
@@ -1231,8 +1283,8 @@ bool job_upload(iwamd_job *J)
 		d.early_bkgd = s.early_bkgd;
 		d.bkgd = s.bkgd_color_lin;
 		for (int k = 0; k < 3; k++)
-			d.lut[k] = (in_int && ic_lut[c][k] >= 0) ? (const double *)(base + o_lut[ic_lut[c][k]]) : nullptr;
-		auto fill = [&](const WeightList &wl, const WOff &o, DevWeights &dw, int dim) {
+			d.lut[k] = (S.in_int && S.ic_lut[c][k] >= 0) ? (const double *)(base + S.o_lut[S.ic_lut[c][k]]) : nullptr;
+		auto fill = [&](const iwx::WeightList &wl, const WOff &o, DevWeights &dw, int dim) {
 			dw.mode = wl.mode;
 			dw.start = (const int *)(base + o.start);
 			dw.src = (const int *)(base + o.src);
@@ -1241,15 +1293,15 @@ bool job_upload(iwamd_job *J)
 			dw.edge = wl.edge_value;
 			dw.edge_transparent = p.rs[dim].edge_policy == IW_EDGE_POLICY_TRANSPARENT;
 		};
-		fill(wv[c], ov[c], d.v, IW_DIMENSION_V);
-		fill(wh[c], oh[c], d.h, IW_DIMENSION_H);
-		d.v.dfirst = (const int *)(base + ov[c].dfirst);
-		d.v.dcount = (const int *)(base + ov[c].dcount);
-		d.v.doff = (const int *)(base + ov[c].doff);
-		d.v.dw = (const double *)(base + ov[c].dw);
-		d.v.dmax = dv[c].max_count;
+		fill(S.wv[c], S.ov[c], d.v, IW_DIMENSION_V);
+		fill(S.wh[c], S.oh[c], d.h, IW_DIMENSION_H);
+		d.v.dfirst = (const int *)(base + S.ov[c].dfirst);
+		d.v.dcount = (const int *)(base + S.ov[c].dcount);
+		d.v.doff = (const int *)(base + S.ov[c].doff);
+		d.v.dw = (const double *)(base + S.ov[c].dw);
+		d.v.dmax = S.dv[c].max_count;
 	}
-	D.inter = (float *)(base + o_inter);
+	D.inter = (float *)(base + S.o_inter);
 	D.intclamp = p.intclamp;
 	D.out_w = p.out_width;
 	D.out_h = p.out_height;
@@ -1257,35 +1309,36 @@ bool job_upload(iwamd_job *J)
 	D.out_float = p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT;
 	D.out_nch = p.out_nch;
 	D.out_bpr = p.out_bpr;
-	D.out = (uint8_t *)(base + J->out_off);
+	D.out = (uint8_t *)(base + S.o_out);
 	for (int o = 0; o < p.out_nch; o++) {
 		const iwamd_ochan &s = p.och[o];
 		DevOChan &d = D.oc[o];
 		d.family = s.ditherfamily;
 		d.subtype = s.dithersubtype;
 		d.is_alpha = s.channeltype == IW_CHANNELTYPE_ALPHA;
-		d.ed_index = ed_of[o];
+		d.ed_index = S.ed_of[o];
 		d.bk1 = s.bkgd1_color_lin;
 		d.bk2 = s.bkgd2_color_lin;
 		if (!D.out_float) {
+			const QOff &q = S.oq[o];
 			d.qmode = s.use_nc_table ? Q_NC : Q_THRESH;
-			d.P = qt[o].P;
-			d.T = (const double *)(base + oq[o].T);
-			d.U = (const double *)(base + oq[o].U);
-			d.lin = (const double *)(base + oq[o].lin);
-			d.code = (const double *)(base + oq[o].code);
-			d.nc = (const double *)(base + oq[o].nc);
-			d.prng_row = (const uint64_t *)(base + oq[o].prow);
-			d.prng_col = (const uint64_t *)(base + oq[o].pcol);
-			d.M = (const double *)(base + oq[o].M);
-			d.ebpre = (const double *)(base + oq[o].pre);
-			d.ebsuf = (const double *)(base + oq[o].suf);
-			d.eb_max = et[o].eb_max;
+			d.P = S.qt[o].P;
+			d.T = (const double *)(base + q.T);
+			d.U = (const double *)(base + q.U);
+			d.lin = (const double *)(base + q.lin);
+			d.code = (const double *)(base + q.code);
+			d.nc = (const double *)(base + q.nc);
+			d.prng_row = (const uint64_t *)(base + q.prow);
+			d.prng_col = (const uint64_t *)(base + q.pcol);
+			d.M = (const double *)(base + q.M);
+			d.ebpre = (const double *)(base + q.pre);
+			d.ebsuf = (const double *)(base + q.suf);
+			d.eb_max = S.et[o].eb_max;
 		}
 	}
-	D.ed_samples = (double *)(base + o_eds);
-	D.ed_err = (double *)(base + o_ede);
-	D.n_ed = J->n_ed;
+	D.ed_samples = (double *)(base + S.o_eds);
+	D.ed_err = (double *)(base + S.o_ede);
+	D.n_ed = S.n_ed;
 	D.apply_bkgd = p.apply_bkgd;
 	D.late = p.bkgd_strategy_late;
 	D.checker = p.bkgd_checkerboard;
@@ -1295,41 +1348,37 @@ bool job_upload(iwamd_job *J)
 	D.bk_has_trans = p.bkgd_has_transparency;
 	D.bk1a = p.bkgd1alpha;
 	D.bk2a = p.bkgd2alpha;
+}
 
-	// Resampling path: the fused band kernel when a band of rows fits in LDS,
-	// else the two-pass kernels with the intermediate in HBM.
-	J->band_rows = 0;
-	{
-		const char *opt = iw_get_option(ctx, "amd:resample");
-		const bool force_two = opt && !strcmp(opt, "twopass");
-		const bool force_band = opt && !strcmp(opt, "band");
-		const int cands[3] = {4, 2, 1};
-		for (int ci = 0; ci < 3 && !force_two && !J->band_rows; ci++) {
-			const int R = cands[ci];
-			if (band_lds_bytes(R, p.inter_nch, p.inter_width) > rt().band_lds_max) continue;
-			// Widest band (input positions) over every channel's list.
-			bool fits = true;
-			for (int c = 0; c < p.inter_nch && fits; c++) {
-				const DenseWeights &d = dv[c];
-				for (int j0 = 0; j0 < p.inter_height && fits; j0 += R) {
-					int lo = 0x7fffffff, hi = -0x7fffffff;
-					for (int r = 0; r < R && j0 + r < p.inter_height; r++) {
-						if (!d.count[j0 + r]) continue;
-						lo = std::min(lo, d.first[j0 + r]);
-						hi = std::max(hi, d.first[j0 + r] + d.count[j0 + r] - 1);
-					}
-					if (hi >= lo && hi - lo + 1 > BAND_MAX_K) fits = false;
-				}
-			}
-			if (fits) J->band_rows = R;
-		}
-		// Wide rows keep more workgroups per CU in the two-pass form.
-		if (!force_band && J->band_rows && p.inter_width > 2048) J->band_rows = 0;
+bool job_upload(iwamd_job *J)
+{
+	iw_context *ctx = J->ctx;
+	const iwamd_plan &p = J->plan;
+	Staged S;
+	if (!stage_job(ctx, p, S, iw_get_option(ctx, "amd:resample"))) return false;
+	J->skip_pixels = S.skip_pixels;
+	J->n_ed = S.n_ed;
+	for (int e = 0; e < 4; e++) J->ed_list[e] = S.ed_list[e];
+	J->band_rows = S.band_rows;
+	J->out_off = S.o_out;
+	J->out_bytes = S.out_bytes;
+	J->block_bytes = S.bytes + 256;
+	J->block = dev_alloc(J->block_bytes);
+	if (!J->block) {
+		iw_set_error(ctx, "Out of device memory");
+		return false;
 	}
-
-	const double in_samples_bytes = (double)in_bytes;
+	char *base = (char *)J->block;
+	bool ok = upload_job(S, p, base, J->stream);
+	// Host tables are freed when S goes out of scope: wait for the copies.
+	ok &= hipStreamSynchronize(J->stream) == hipSuccess;
+	if (!ok) {
+		iw_set_error(ctx, "Host-to-device copy failed");
+		return false;
+	}
+	fill_dev_plan(S, p, base, J->D);
 	J->out_pixels = (double)p.out_width * p.out_height;
-	J->alg_bytes = in_samples_bytes + (double)J->out_bytes;
+	J->alg_bytes = (double)S.in_bytes + (double)S.out_bytes;
 	return true;
 }
 
