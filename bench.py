@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--out", type=int, default=1000)
+    p.add_argument("--workload", choices=["c1", "c3"], default="c1",
+                   help="c1: the headline single image; c3: the 4096-image batch")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="after timing, run one plain device-to-device copy of MIB MiB (calibrates HBM counters)")
     return p.parse_args()
@@ -118,12 +120,79 @@ def cpu_baseline(job, out_pixels):
             "ok": r["ok"]}, r
 
 
+def bench_c3(a):
+    """4096 RGBA8 images (sides log-uniform 64..1024, factors 0.25..2 per axis,
+    triangle / Catmull-Rom / Lanczos2 / Lanczos3, 8-bit, no dither) as one
+    batch: one device block, one hipGraph replay per step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from imageworsener_amd import api
+    import cases
+    lib = api.Library()
+    log("generating the C3 batch (4096 images)")
+    t0 = time.perf_counter()
+    jobs = cases.c3_jobs(4096)
+    log("generated in %.1f s; staging and uploading" % (time.perf_counter() - t0))
+    b = lib.batch(jobs)
+    try:
+        out_px, alg_bytes, n = b.sizes()
+        for _ in range(a.warmup):
+            b.run()
+        b.sync()
+        lat, kern = [], []
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            s0 = time.perf_counter()
+            b.run()
+            b.sync()
+            lat.append((time.perf_counter() - s0) * 1e3)
+            kern.append(b.kernel_ms())
+        wall = time.perf_counter() - t0
+        res = b.finish()
+    finally:
+        b.close()
+    ms = wall * 1e3 / a.steps
+    kms = sum(kern) / len(kern)
+    base = None
+    if not a.no_cpu_baseline:
+        ref_path = os.path.join(ROOT, "oracle", "_ref", "libiwref.so")
+        kind = "reference" if os.path.exists(ref_path) else None
+        if kind:
+            ref = api.Library(ref_path, extensions=False)
+            done_px, t_cpu, same, k = 0.0, 0.0, True, 0
+            while k < len(jobs) and t_cpu < 10.0:
+                c0 = time.perf_counter()
+                r = ref.run(jobs[k])
+                t_cpu += time.perf_counter() - c0
+                done_px += r["width"] * r["height"]
+                same &= bool(r["ok"] and res[k]["ok"] and (r["pixels"] == res[k]["pixels"]).all())
+                k += 1
+            base = {"value": round(done_px / 1e6 / t_cpu, 4), "unit": "Mpix/s", "cores": 1, "kind": kind,
+                    "sample": "first %d of the 4096 images through iw_process_image on one host thread (%.1f s)"
+                              % (k, t_cpu), "plain_bytes_identical": same}
+    line = {"metric": METRIC, "value": round(out_px / 1e6 / (ms / 1e3), 3), "unit": "Mpix/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "C3: 4096 RGBA8 images, sides 64..1024, x0.25..2 per axis, "
+                                   "triangle/catrom/lanczos2/lanczos3, 8-bit, no dither; one batch",
+                       "global_batch": n, "precision": "plain"},
+            "roofline": {"kernel": "band (persistent) + graph", "bound": "hbm",
+                         "achieved": round(alg_bytes / (kms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg_bytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": None, "alg_bytes_per_launch": alg_bytes},
+            "cpu_baseline": base,
+            "latency_ms": {"p50": round(sorted(lat)[len(lat) // 2], 4), "p99": round(max(lat), 4)},
+            "kernel_ms": round(kms, 4), "out_mpix": round(out_px / 1e6, 3)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (shard by image or "
               "strip across processes; see DESIGN.md)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.workload == "c3":
+        return bench_c3(a)
     from imageworsener_amd import api, synth
     lib = api.Library()
     log("generating %dx%d RGBA16 input" % (a.size, a.size))

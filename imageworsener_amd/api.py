@@ -118,6 +118,16 @@ _EXT_SIGS = {
     "iwamd_job_destroy": (None, [_c.c_void_p]),
     "iwamd_job_kernel_ms": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_double), _c.c_int]),
     "iwamd_job_sizes": (None, [_c.c_void_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "iwamd_batch_create": (_c.c_void_p, []),
+    "iwamd_batch_add": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
+    "iwamd_batch_upload": (_c.c_int, [_c.c_void_p]),
+    "iwamd_batch_run": (_c.c_int, [_c.c_void_p]),
+    "iwamd_batch_sync": (_c.c_int, [_c.c_void_p]),
+    "iwamd_batch_finish": (_c.c_int, [_c.c_void_p]),
+    "iwamd_batch_destroy": (None, [_c.c_void_p]),
+    "iwamd_batch_kernel_ms": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_double), _c.c_int]),
+    "iwamd_batch_sizes": (None, [_c.c_void_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double),
+                                 _c.POINTER(_c.c_int)]),
 }
 
 # Byte offsets into struct iwamd_plan used by executors driven from Python.
@@ -160,6 +170,27 @@ class Library:
             return self._run(ctx, job, executor)
         finally:
             d.iw_destroy_context(ctx)
+
+    def batch(self, jobs):
+        """A Batch holding `jobs`, uploaded."""
+        b = Batch(self)
+        try:
+            for j in jobs:
+                b.add(j)
+            b.upload()
+        except Exception:
+            b.close()
+            raise
+        return b
+
+    def run_batch(self, jobs):
+        """Run `jobs` as one batch; results in order."""
+        b = self.batch(jobs)
+        try:
+            b.run()
+            return b.finish()
+        finally:
+            b.close()
 
     def device_job(self, job):
         """A DeviceJob: settings applied, input and tables resident on the
@@ -272,6 +303,65 @@ class Library:
             k = d.iwamd_last_timings(ctx, t, 8)
             res["timings_ms"] = list(t[:k])
         return res
+
+
+class Batch:
+    """Many jobs in one device block and one hipGraph (iwamd_batch_*).
+    add() prepares a job; upload() places them all; run() replays the graph;
+    finish() returns the results in add order."""
+
+    def __init__(self, lib):
+        self.lib = lib
+        self.h = lib.dll.iwamd_batch_create()
+        if not self.h:
+            raise RuntimeError("iwamd_batch_create failed")
+        self.ctxs = []
+        self.failed = {}
+
+    def add(self, job):
+        d = self.lib.dll
+        ctx = d.iw_create_context(None)
+        if not ctx:
+            raise MemoryError("iw_create_context failed")
+        self.lib._setup(ctx, job)
+        idx = d.iwamd_batch_add(self.h, ctx)
+        if idx < 0:
+            self.failed[len(self.ctxs)] = True
+        self.ctxs.append(ctx)
+        return idx
+
+    def upload(self):
+        if not self.lib.dll.iwamd_batch_upload(self.h):
+            raise RuntimeError("iwamd_batch_upload failed")
+
+    def run(self):
+        if not self.lib.dll.iwamd_batch_run(self.h):
+            raise RuntimeError("iwamd_batch_run failed")
+
+    def sync(self):
+        if not self.lib.dll.iwamd_batch_sync(self.h):
+            raise RuntimeError("iwamd_batch_sync failed")
+
+    def kernel_ms(self):
+        t = ctypes.c_double()
+        return t.value if self.lib.dll.iwamd_batch_kernel_ms(self.h, ctypes.byref(t), 1) else None
+
+    def sizes(self):
+        px, nb, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self.lib.dll.iwamd_batch_sizes(self.h, ctypes.byref(px), ctypes.byref(nb), ctypes.byref(n))
+        return px.value, nb.value, n.value
+
+    def finish(self):
+        ok = self.lib.dll.iwamd_batch_finish(self.h)
+        return [self.lib._collect(ctx, ok and i not in self.failed) for i, ctx in enumerate(self.ctxs)]
+
+    def close(self):
+        if self.h:
+            self.lib.dll.iwamd_batch_destroy(self.h)
+            self.h = None
+        for ctx in self.ctxs:
+            self.lib.dll.iw_destroy_context(ctx)
+        self.ctxs = []
 
 
 class DeviceJob:

@@ -442,13 +442,11 @@ __device__ __forceinline__ double band_sample(const DevPlan &P, const DevIChan &
 }
 
 template <int R>
-__global__ void __launch_bounds__(256) band_kernel(DevPlan P)
+__device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned char *band_lds)
 {
-	extern __shared__ __align__(16) unsigned char band_lds[];
 	double *wd = (double *)band_lds;                       // [R][BAND_MAX_K]
 	float *inter = (float *)(band_lds + sizeof(double) * R * BAND_MAX_K);  // [C][R][W]
 	const int tid = threadIdx.x;
-	const int j0 = blockIdx.x * R;
 	const int rows = min(R, P.Hp - j0);
 	const int W = P.W, C = P.inter_nch;
 	const int num_in = P.ih;
@@ -522,6 +520,34 @@ __global__ void __launch_bounds__(256) band_kernel(DevPlan P)
 			return acc;
 		});
 	}
+}
+
+// Work item (plan, first row of the band).  Workgroup g first takes item g;
+// with a counter (reset to 0 before the launch) it then keeps taking items
+// gridDim.x + atomicAdd(counter, 1) until none is left -- a persistent
+// kernel over a whole batch.
+template <int R>
+__global__ void __launch_bounds__(256) band_kernel(const DevPlan *__restrict__ plans,
+	const int2 *__restrict__ items, int n_items, int *counter)
+{
+	extern __shared__ __align__(16) unsigned char band_lds[];
+	__shared__ int next_item;
+	int it = blockIdx.x;
+	while (it < n_items) {
+		const int2 w = items[it];
+		band_body<R>(plans[w.x], w.y, band_lds);
+		if (!counter) break;
+		__syncthreads();  // LDS of this item is no longer read
+		if (threadIdx.x == 0) next_item = (int)gridDim.x + atomicAdd(counter, 1);
+		__syncthreads();
+		it = next_item;
+	}
+}
+
+__global__ void reset_counters_kernel(int *c, int n)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k < n) c[k] = 0;
 }
 
 // ---- error diffusion ------------------------------------------------------
@@ -764,15 +790,16 @@ __device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int 
 	}
 }
 
+// One workgroup per plane: planes[b] = (plan, output channel).
 template <bool LDS_TAB, bool HAS_M1, bool LDS_ROWS>
-__global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(DevPlan P, int o_list0, int o_list1, int o_list2,
-	int o_list3)
+__global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__restrict__ plans,
+	const int2 *__restrict__ planes)
 {
 	__shared__ EdLds<LDS_ROWS> S;
 	const int tid = threadIdx.x;
-	const int olist[4] = {o_list0, o_list1, o_list2, o_list3};
-	const int o = olist[blockIdx.x];
-	if (o < 0) return;
+	const int2 pl = planes[blockIdx.x];
+	const DevPlan &P = plans[pl.x];
+	const int o = pl.y;
 	const DevOChan &q = P.oc[o];
 	const double *m = c_ed_matrix[(q.subtype >= 0 && q.subtype <= 7) ? q.subtype : 0];
 	const int W = P.out_w, H = P.out_h, Pq = q.P;
@@ -989,6 +1016,7 @@ struct iwamd_job {
 	int band_rows = 0;  // rows per band of the fused kernel; 0: two-pass kernels
 	bool ran = false;
 	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
+	void *ws = nullptr;        // the job's work set (WorkSet)
 };
 
 namespace {
@@ -1053,7 +1081,7 @@ int choose_band_rows(const iwamd_plan &p, const std::vector<iwx::DenseWeights> &
 
 // Build the host tables of a plan, choose its path and lay out its device
 // data (offsets from the job's base).  path: "band", "twopass" or null.
-bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path)
+bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path, bool out_elsewhere = false)
 {
 	using namespace iwx;
 	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT) {
@@ -1164,7 +1192,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	S.o_in = L.add<uint8_t>(S.in_bytes);
 	S.o_inter = L.add<float>(S.band_rows ? 0 : (size_t)p.inter_nch * p.inter_height * p.inter_width);
 	S.out_bytes = p.out_bpr * (size_t)p.out_height;
-	S.o_out = L.add<uint8_t>(S.out_bytes);
+	S.o_out = out_elsewhere ? 0 : L.add<uint8_t>(S.out_bytes);
 	S.o_eds = L.add<double>((size_t)S.n_ed * p.out_height * p.out_width);
 	S.o_ede = L.add<double>((size_t)S.n_ed * 4 * p.out_width);
 	S.o_lut.resize(S.luts.size());
@@ -1350,6 +1378,149 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 	D.bk2a = p.bkgd2alpha;
 }
 
+// ---- work sets ---------------------------------------------------------------
+// Device work lists for a set of staged jobs: the plan array (the descriptor
+// table the kernels index), the bands of each band height, the
+// error-diffusion planes of each kernel variant, and the jobs that take the
+// two-pass kernels.
+
+constexpr int kBandR[3] = {1, 2, 4};
+
+typedef void (*EdKernel)(const DevPlan *, const int2 *);
+const EdKernel kEdKernels[8] = {
+	errdiff_kernel<false, false, false>, errdiff_kernel<false, false, true>,
+	errdiff_kernel<false, true, false>, errdiff_kernel<false, true, true>,
+	errdiff_kernel<true, false, false>, errdiff_kernel<true, false, true>,
+	errdiff_kernel<true, true, false>, errdiff_kernel<true, true, true>};
+
+// Kernel variant of one error-diffusion plane: decision tables in LDS for up
+// to 257 levels, the x+2 carry only where its weight is nonzero (not
+// Floyd-Steinberg or Sierra 4-2-1), error rows in LDS up to ED_ROWS_LDS wide.
+int ed_variant(const Staged &S, const iwamd_plan &p, int o)
+{
+	const int subtype = p.och[o].dithersubtype;
+	const int sub = (subtype >= 0 && subtype <= 7) ? subtype : 0;
+	const bool m1 = sub != 0 && sub != 6;
+	return (S.qt[o].P <= ED_LDS_TABLE ? 4 : 0) + (m1 ? 2 : 0) + (p.out_width <= ED_ROWS_LDS ? 1 : 0);
+}
+
+struct WorkSet {
+	std::vector<DevPlan> plans;       // host copies; the two-pass kernels take them by value
+	std::vector<int2> bands[3];       // (plan, first row), by band height kBandR
+	size_t band_lds[3] = {0, 0, 0};   // largest LDS of a band of each height
+	std::vector<int2> planes[8];      // (plan, output channel), by ed_variant
+	std::vector<int> twopass;         // plans resampled by vpass + hpass
+	size_t o_plans = 0, o_bands[3] = {0, 0, 0}, o_planes[8] = {0}, o_counters = 0, bytes = 0;
+	const DevPlan *d_plans = nullptr;
+	const int2 *d_bands[3] = {nullptr, nullptr, nullptr};
+	const int2 *d_planes[8] = {nullptr};
+	int *d_counters = nullptr;
+};
+
+// Adds a job's work; its kernel arguments are set later (W.plans[index]),
+// once its device data has a place.  Returns the job's index.
+int ws_add(WorkSet &W, const Staged &S, const iwamd_plan &p)
+{
+	const int jx = (int)W.plans.size();
+	W.plans.push_back(DevPlan());
+	if (S.skip_pixels) return jx;
+	if (S.band_rows) {
+		const int ri = S.band_rows == 4 ? 2 : (S.band_rows == 2 ? 1 : 0);
+		for (int j0 = 0; j0 < p.inter_height; j0 += S.band_rows) W.bands[ri].push_back(make_int2(jx, j0));
+		W.band_lds[ri] = std::max(W.band_lds[ri], band_lds_bytes(S.band_rows, p.inter_nch, p.inter_width));
+	} else {
+		W.twopass.push_back(jx);
+	}
+	for (int e = 0; e < S.n_ed; e++) {
+		const int o = S.ed_list[e];
+		W.planes[ed_variant(S, p, o)].push_back(make_int2(jx, o));
+	}
+	return jx;
+}
+
+void ws_layout(WorkSet &W)
+{
+	Layout L;
+	W.o_plans = L.add<DevPlan>(W.plans.size());
+	for (int r = 0; r < 3; r++) W.o_bands[r] = L.add<int2>(W.bands[r].size());
+	for (int v = 0; v < 8; v++) W.o_planes[v] = L.add<int2>(W.planes[v].size());
+	W.o_counters = L.add<int>(3);
+	W.bytes = (L.off + 255) & ~(size_t)255;
+}
+
+// Queue the copies of the work lists to base (W must live until the stream
+// is synchronised).
+bool ws_upload(WorkSet &W, char *base, hipStream_t st)
+{
+	bool ok = true;
+	auto up = [&](size_t off, const void *src, size_t bytes) {
+		if (bytes) ok &= hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+	};
+	W.d_plans = (const DevPlan *)(base + W.o_plans);
+	up(W.o_plans, W.plans.data(), W.plans.size() * sizeof(DevPlan));
+	for (int r = 0; r < 3; r++) {
+		W.d_bands[r] = (const int2 *)(base + W.o_bands[r]);
+		up(W.o_bands[r], W.bands[r].data(), W.bands[r].size() * sizeof(int2));
+	}
+	for (int v = 0; v < 8; v++) {
+		W.d_planes[v] = (const int2 *)(base + W.o_planes[v]);
+		up(W.o_planes[v], W.planes[v].data(), W.planes[v].size() * sizeof(int2));
+	}
+	W.d_counters = (int *)(base + W.o_counters);
+	return ok;
+}
+
+int device_cu_count()
+{
+	static int n = 0;
+	if (!n) {
+		int dev = 0, v = 0;
+		if (hipGetDevice(&dev) == hipSuccess &&
+			hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+			n = v;
+		else
+			n = 256;
+	}
+	return n;
+}
+
+// Queue the resampling kernels of a work set.  persistent: one launch per
+// band height sized to the device, workgroups taking bands from a counter;
+// otherwise one workgroup per band.
+void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
+{
+	if (persistent) hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 3);
+	for (int r = 0; r < 3; r++) {
+		const int n = (int)W.bands[r].size();
+		if (!n) continue;
+		const size_t lds = W.band_lds[r];
+		int grid = n;
+		if (persistent) {
+			const int per_cu = std::max(1, std::min(8, (int)(rt().band_lds_max / std::max<size_t>(lds, 1))));
+			grid = std::min(n, device_cu_count() * per_cu);
+		}
+		int *ctr = persistent ? W.d_counters + r : nullptr;
+		switch (kBandR[r]) {
+		case 1: hipLaunchKernelGGL(band_kernel<1>, dim3(grid), dim3(256), lds, st, W.d_plans, W.d_bands[r], n, ctr); break;
+		case 2: hipLaunchKernelGGL(band_kernel<2>, dim3(grid), dim3(256), lds, st, W.d_plans, W.d_bands[r], n, ctr); break;
+		default: hipLaunchKernelGGL(band_kernel<4>, dim3(grid), dim3(256), lds, st, W.d_plans, W.d_bands[r], n, ctr); break;
+		}
+	}
+	for (int jx : W.twopass) {
+		const DevPlan &D = W.plans[jx];
+		hipLaunchKernelGGL(vpass_kernel, dim3((D.W + 255) / 256, D.Hp), dim3(256), 0, st, D);
+		hipLaunchKernelGGL(hpass_kernel, dim3((D.out_w + 255) / 256, D.out_h), dim3(256), 0, st, D);
+	}
+}
+
+void ws_enqueue_errdiff(const WorkSet &W, hipStream_t st)
+{
+	for (int v = 0; v < 8; v++) {
+		const int n = (int)W.planes[v].size();
+		if (n) hipLaunchKernelGGL(kEdKernels[v], dim3(n), dim3(ED_THREADS), 0, st, W.d_plans, W.d_planes[v]);
+	}
+}
+
 bool job_upload(iwamd_job *J)
 {
 	iw_context *ctx = J->ctx;
@@ -1362,21 +1533,29 @@ bool job_upload(iwamd_job *J)
 	J->band_rows = S.band_rows;
 	J->out_off = S.o_out;
 	J->out_bytes = S.out_bytes;
-	J->block_bytes = S.bytes + 256;
+	// Work lists follow the job's data in the same block; their sizes do not
+	// depend on where the block lands.
+	WorkSet *W = new WorkSet();
+	J->ws = W;
+	ws_add(*W, S, p);
+	ws_layout(*W);
+	J->block_bytes = S.bytes + W->bytes;
 	J->block = dev_alloc(J->block_bytes);
 	if (!J->block) {
 		iw_set_error(ctx, "Out of device memory");
 		return false;
 	}
 	char *base = (char *)J->block;
+	fill_dev_plan(S, p, base, J->D);
+	W->plans[0] = J->D;
 	bool ok = upload_job(S, p, base, J->stream);
+	ok &= ws_upload(*W, base + S.bytes, J->stream);
 	// Host tables are freed when S goes out of scope: wait for the copies.
 	ok &= hipStreamSynchronize(J->stream) == hipSuccess;
 	if (!ok) {
 		iw_set_error(ctx, "Host-to-device copy failed");
 		return false;
 	}
-	fill_dev_plan(S, p, base, J->D);
 	J->out_pixels = (double)p.out_width * p.out_height;
 	J->alg_bytes = (double)S.in_bytes + (double)S.out_bytes;
 	return true;
@@ -1409,45 +1588,21 @@ extern "C" struct iwamd_job *iwamd_job_create(struct iw_context *ctx)
 extern "C" int iwamd_job_run(struct iwamd_job *J)
 {
 	if (J->skip_pixels) return 1;
-	const DevPlan &D = J->D;
+	const WorkSet &W = *(const WorkSet *)J->ws;
 	hipStream_t st = J->stream;
 	hipEventRecord(J->ev[0], st);
 	if (J->band_rows) {
 		// One fused kernel; the horizontal-pass slot of the timings stays 0.
-		const int R = J->band_rows;
-		const dim3 grid((D.Hp + R - 1) / R);
-		const size_t lds = band_lds_bytes(R, D.inter_nch, D.W);
-		if (R == 4) hipLaunchKernelGGL(band_kernel<4>, grid, dim3(256), lds, st, D);
-		else if (R == 2) hipLaunchKernelGGL(band_kernel<2>, grid, dim3(256), lds, st, D);
-		else hipLaunchKernelGGL(band_kernel<1>, grid, dim3(256), lds, st, D);
+		ws_enqueue_resample(W, st, false);
 		hipEventRecord(J->ev[1], st);
 	} else {
-		dim3 grid((D.W + 255) / 256, D.Hp);
-		hipLaunchKernelGGL(vpass_kernel, grid, dim3(256), 0, st, D);
+		const DevPlan &D = J->D;
+		hipLaunchKernelGGL(vpass_kernel, dim3((D.W + 255) / 256, D.Hp), dim3(256), 0, st, D);
 		hipEventRecord(J->ev[1], st);
-		dim3 grid2((D.out_w + 255) / 256, D.out_h);
-		hipLaunchKernelGGL(hpass_kernel, grid2, dim3(256), 0, st, D);
+		hipLaunchKernelGGL(hpass_kernel, dim3((D.out_w + 255) / 256, D.out_h), dim3(256), 0, st, D);
 	}
 	hipEventRecord(J->ev[2], st);
-	if (J->n_ed > 0) {
-		const int *l = J->ed_list;
-		// Decision tables fit in LDS for up to 257 levels (8-bit output); the
-		// x+2 carry is skipped where its weight is zero for every plane.
-		bool small = true, m1 = false;
-		for (int e = 0; e < J->n_ed; e++) {
-			const DevOChan &q = D.oc[l[e]];
-			small &= q.P <= ED_LDS_TABLE;
-			const int sub = (q.subtype >= 0 && q.subtype <= 7) ? q.subtype : 0;
-			m1 |= sub != 0 && sub != 6;  // Floyd-Steinberg and Sierra 4-2-1 have none
-		}
-		// Error rows in LDS up to ED_ROWS_LDS pixels wide.
-		const bool rows = D.out_w <= ED_ROWS_LDS;
-		auto k = small ? (m1 ? (rows ? errdiff_kernel<true, true, true> : errdiff_kernel<true, true, false>)
-		                     : (rows ? errdiff_kernel<true, false, true> : errdiff_kernel<true, false, false>))
-		               : (m1 ? (rows ? errdiff_kernel<false, true, true> : errdiff_kernel<false, true, false>)
-		                     : (rows ? errdiff_kernel<false, false, true> : errdiff_kernel<false, false, false>));
-		hipLaunchKernelGGL(k, dim3(J->n_ed), dim3(ED_THREADS), 0, st, D, l[0], l[1], l[2], l[3]);
-	}
+	ws_enqueue_errdiff(W, st);
 	hipEventRecord(J->ev[3], st);
 	if (hipGetLastError() != hipSuccess) {
 		iw_set_error(J->ctx, "Kernel launch failed");
@@ -1507,6 +1662,7 @@ extern "C" void iwamd_job_destroy(struct iwamd_job *J)
 	if (!J) return;
 	if (J->stream) hipStreamSynchronize(J->stream);
 	dev_release(J->block, J->block_bytes);
+	delete (WorkSet *)J->ws;
 	for (auto &e : J->ev)
 		if (e) hipEventDestroy(e);
 	if (J->stream) hipStreamDestroy(J->stream);
@@ -1551,6 +1707,201 @@ extern "C" int iwamd_copy_probe(size_t bytes, double *ms)
 	(void)hipFree(a);
 	(void)hipFree(b);
 	return ok ? 1 : 0;
+}
+
+// ---- batches --------------------------------------------------------------
+// Many contexts' pixel work in one device block: every job's input and
+// tables, one output arena (one copy back), a descriptor table of kernel
+// arguments, and work lists.  A run is one hipGraph: the persistent band
+// kernels (one per band height, workgroups taking bands from a counter),
+// the two-pass kernels of jobs too wide for a band, and the error-diffusion
+// planes grouped by kernel variant.  Each job's bytes depend only on its own
+// plan: band height, work order and the persistent schedule change which
+// workgroup computes a band, never how.
+
+struct iwamd_batch {
+	struct Job {
+		iw_context *ctx = nullptr;
+		iwamd_plan plan;
+		Staged S;
+		size_t off = 0, out_off = 0;
+		int index = -1;
+		bool skip_pixels = false;
+	};
+	std::vector<Job *> jobs;
+	WorkSet W;
+	void *block = nullptr;
+	size_t block_bytes = 0, out_base = 0, out_total = 0;
+	hipStream_t stream = nullptr;
+	hipEvent_t ev[2] = {nullptr, nullptr};
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	bool uploaded = false, ran = false, failed = false;
+	double out_pixels = 0.0, alg_bytes = 0.0;
+};
+
+extern "C" struct iwamd_batch *iwamd_batch_create(void)
+{
+	iwamd_batch *B = new iwamd_batch();
+	if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
+		delete B;
+		return nullptr;
+	}
+	for (auto &e : B->ev) hipEventCreate(&e);
+	return B;
+}
+
+extern "C" int iwamd_batch_add(struct iwamd_batch *B, struct iw_context *ctx)
+{
+	if (B->uploaded) {
+		iw_set_error(ctx, "Batch already uploaded");
+		return -1;
+	}
+	iwamd_batch::Job *J = new iwamd_batch::Job();
+	J->ctx = ctx;
+	if (!iwamd_prepare(ctx, &J->plan) || !runtime_init(ctx) ||
+		!stage_job(ctx, J->plan, J->S, iw_get_option(ctx, "amd:resample"), true)) {
+		delete J;
+		return -1;
+	}
+	J->index = (int)B->jobs.size();
+	J->skip_pixels = J->S.skip_pixels;
+	B->jobs.push_back(J);
+	return J->index;
+}
+
+extern "C" int iwamd_batch_upload(struct iwamd_batch *B)
+{
+	if (B->uploaded) return 1;
+	if (B->jobs.empty()) return 0;
+	// Layout: [jobs' data][output arena][work lists].
+	size_t off = 0;
+	for (auto *J : B->jobs) {
+		J->off = off;
+		off += J->S.bytes;
+	}
+	B->out_base = off;
+	size_t arena = 0;
+	for (auto *J : B->jobs) {
+		J->out_off = arena;
+		arena += (J->S.out_bytes + 255) & ~(size_t)255;
+		ws_add(B->W, J->S, J->plan);
+		B->out_pixels += (double)J->plan.out_width * J->plan.out_height;
+		B->alg_bytes += (double)J->S.in_bytes + (double)J->S.out_bytes;
+	}
+	B->out_total = arena;
+	ws_layout(B->W);
+	B->block_bytes = off + arena + B->W.bytes;
+	B->block = dev_alloc(B->block_bytes);
+	if (!B->block) {
+		for (auto *J : B->jobs) iw_set_error(J->ctx, "Out of device memory");
+		B->failed = true;
+		return 0;
+	}
+	char *base = (char *)B->block;
+	bool ok = true;
+	for (auto *J : B->jobs) {
+		DevPlan &D = B->W.plans[J->index];
+		fill_dev_plan(J->S, J->plan, base + J->off, D);
+		D.out = (uint8_t *)(base + B->out_base + J->out_off);
+		ok &= upload_job(J->S, J->plan, base + J->off, B->stream);
+	}
+	ok &= ws_upload(B->W, base + B->out_base + B->out_total, B->stream);
+	ok &= hipStreamSynchronize(B->stream) == hipSuccess;
+	// Host tables are no longer needed.
+	for (auto *J : B->jobs) J->S = Staged();
+	if (ok) {
+		ok = hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+		if (ok) {
+			ws_enqueue_resample(B->W, B->stream, true);
+			ws_enqueue_errdiff(B->W, B->stream);
+			ok = hipStreamEndCapture(B->stream, &B->graph) == hipSuccess;
+		}
+		ok = ok && hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0) == hipSuccess;
+	}
+	if (!ok) {
+		for (auto *J : B->jobs) iw_set_error(J->ctx, "Batch upload failed");
+		B->failed = true;
+		return 0;
+	}
+	B->uploaded = true;
+	return 1;
+}
+
+extern "C" int iwamd_batch_run(struct iwamd_batch *B)
+{
+	if (!B->uploaded && !iwamd_batch_upload(B)) return 0;
+	hipEventRecord(B->ev[0], B->stream);
+	const bool ok = hipGraphLaunch(B->exec, B->stream) == hipSuccess;
+	hipEventRecord(B->ev[1], B->stream);
+	if (!ok) {
+		for (auto *J : B->jobs) iw_set_error(J->ctx, "Batch launch failed");
+		return 0;
+	}
+	B->ran = true;
+	return 1;
+}
+
+extern "C" int iwamd_batch_sync(struct iwamd_batch *B)
+{
+	const hipError_t e = hipStreamSynchronize(B->stream);
+	if (e != hipSuccess) {
+		for (auto *J : B->jobs) iw_set_errorf(J->ctx, "GPU batch failed: %s", hipGetErrorString(e));
+		return 0;
+	}
+	return 1;
+}
+
+extern "C" int iwamd_batch_finish(struct iwamd_batch *B)
+{
+	if (!B->ran && !iwamd_batch_run(B)) return 0;
+	std::vector<uint8_t> host(B->out_total + 1, 0);
+	if (hipMemcpyAsync(host.data(), (char *)B->block + B->out_base, B->out_total, hipMemcpyDeviceToHost,
+			B->stream) != hipSuccess || !iwamd_batch_sync(B)) {
+		for (auto *J : B->jobs) iw_set_error(J->ctx, "Device-to-host copy failed");
+		return 0;
+	}
+	int ok = 1;
+	std::vector<uint8_t> zeros;
+	for (auto *J : B->jobs) {
+		const uint8_t *img2 = host.data() + J->out_off;
+		if (J->skip_pixels) {
+			zeros.assign(J->plan.out_bpr * (size_t)J->plan.out_height + 1, 0);
+			img2 = zeros.data();
+		}
+		ok &= iwamd_finish(J->ctx, img2);
+	}
+	return ok;
+}
+
+extern "C" int iwamd_batch_kernel_ms(struct iwamd_batch *B, double *ms, int n)
+{
+	if (!B->ran || n < 1) return 0;
+	float t = 0.0f;
+	hipEventElapsedTime(&t, B->ev[0], B->ev[1]);
+	ms[0] = t;
+	return 1;
+}
+
+extern "C" void iwamd_batch_sizes(struct iwamd_batch *B, double *out_pixels, double *alg_bytes, int *n_jobs)
+{
+	if (out_pixels) *out_pixels = B->out_pixels;
+	if (alg_bytes) *alg_bytes = B->alg_bytes;
+	if (n_jobs) *n_jobs = (int)B->jobs.size();
+}
+
+extern "C" void iwamd_batch_destroy(struct iwamd_batch *B)
+{
+	if (!B) return;
+	if (B->stream) hipStreamSynchronize(B->stream);
+	if (B->exec) hipGraphExecDestroy(B->exec);
+	if (B->graph) hipGraphDestroy(B->graph);
+	dev_release(B->block, B->block_bytes);
+	for (auto &e : B->ev)
+		if (e) hipEventDestroy(e);
+	if (B->stream) hipStreamDestroy(B->stream);
+	for (auto *J : B->jobs) delete J;
+	delete B;
 }
 
 int iwx_gpu_device_count(void)

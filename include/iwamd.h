@@ -169,6 +169,30 @@ IW_EXPORT(int) iwamd_job_kernel_ms(struct iwamd_job *job, double *ms, int n);
  * of one run. */
 IW_EXPORT(void) iwamd_job_sizes(struct iwamd_job *job, double *out_pixels, double *alg_bytes);
 
+/* Batches.  iwamd_batch_add() runs the host decisions of a context
+ * (consuming its single use, like iw_process_image) and stages its tables;
+ * iwamd_batch_upload() places every job in one device block (inputs, tables,
+ * one output arena, a descriptor table of kernel arguments and the work
+ * lists) and captures the launch sequence as a hipGraph: persistent band
+ * kernels taking work items from a device counter, then the error-diffusion
+ * planes.  iwamd_batch_run() replays the graph (asynchronously; repeatable),
+ * iwamd_batch_finish() copies the arena back once and completes every
+ * context so iw_get_output_image() works on each.  A job's output bytes do
+ * not depend on the other jobs in the batch.  iwamd_batch_add returns the
+ * job's index, or -1 with the context's error set; the others return 1 on
+ * success, 0 on failure (errors go to the jobs' contexts). */
+struct iwamd_batch;
+IW_EXPORT(struct iwamd_batch*) iwamd_batch_create(void);
+IW_EXPORT(int) iwamd_batch_add(struct iwamd_batch *batch, struct iw_context *ctx);
+IW_EXPORT(int) iwamd_batch_upload(struct iwamd_batch *batch);
+IW_EXPORT(int) iwamd_batch_run(struct iwamd_batch *batch);
+IW_EXPORT(int) iwamd_batch_sync(struct iwamd_batch *batch);
+IW_EXPORT(int) iwamd_batch_finish(struct iwamd_batch *batch);
+IW_EXPORT(void) iwamd_batch_destroy(struct iwamd_batch *batch);
+/* Device time of the last run (HIP events around the graph), ms in ms[0]. */
+IW_EXPORT(int) iwamd_batch_kernel_ms(struct iwamd_batch *batch, double *ms, int n);
+IW_EXPORT(void) iwamd_batch_sizes(struct iwamd_batch *batch, double *out_pixels, double *alg_bytes, int *n_jobs);
+
 /* Library/device information.  Returns the number of visible HIP devices
  * (0 when none), and fills name (may be NULL). */
 IW_EXPORT(int) iwamd_device_count(void);

@@ -90,6 +90,19 @@ def sweep_jobs():
     return jobs
 
 
+def c3_jobs(count=4096, seed=4, max_side=None):
+    """The batched configuration: RGBA8 images of log-uniform sides, each
+    resized by its own factors with its own filter, 8-bit, no dither.
+    max_side keeps only images whose sides are at most that (test subsets)."""
+    jobs = []
+    for (w, h, ow, oh, filt, iseed) in synth.c3_list(count, seed):
+        if max_side and (w > max_side or h > max_side):
+            continue
+        img = synth.c3_image(w, h, iseed)
+        jobs.append(api.image_job(img, w, h, RGBA, 8, canvas=(ow, oh), resize=api.resize_both(filt)))
+    return jobs
+
+
 def assert_same(a, b, what=""):
     assert a["ok"], (what, a["error"])
     assert b["ok"], (what, b["error"])
