@@ -550,6 +550,183 @@ __global__ void reset_counters_kernel(int *c, int n)
 	if (k < n) c[k] = 0;
 }
 
+// ---- chosen-precision form: FP16 MFMA, FP32 accumulation ------------------
+// A work item is a tile of FAST_ROWS output rows x up to FAST_MAX_TI output
+// columns.  Both passes are small dense matrix products on
+// v_mfma_f32_16x16x16f16:
+//   vertical:   inter[r][x] = sum_k Wv[r][k] * in[k][x]   (k: the band's input
+//               rows, x: the tile's intermediate columns)
+//   horizontal: out[r][i]   = sum_x inter[r][x] * Wh[x][i]
+// with the band's weights zero outside each row's list.  Inputs, weights and
+// the intermediate are FP16, sums FP32; everything after the resampled value
+// (unpremultiply, background, the linear-to-output transfer, quantisation
+// and dithering) is the plain form's FP64 code.  A tile's sums depend only on
+// its own plan (band and tile geometry come from the plan), never on the
+// batch around it.
+
+#define FAST_ROWS 16
+#define FAST_MAX_KV 256   // input rows of a band
+#define FAST_MAX_XT 288   // intermediate columns of a tile
+#define FAST_MAX_TI 64    // output columns of a tile
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+struct FastItem {
+	int job, j0, i0, ni;  // plan; first output row; first output column, count
+	int x_lo, nx;         // the tile's intermediate columns
+	int p_lo, kv;         // the band's input rows
+};
+
+struct FastLds {
+	_Float16 wv[FAST_ROWS][FAST_MAX_KV];
+	_Float16 inter[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_XT];
+	float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
+};
+
+// Operand layout of v_mfma_f32_16x16x16f16 for lane l: A[l%16][4*(l/16)+t],
+// B[4*(l/16)+t][l%16], D[4*(l/16)+t][l%16], t = 0..3.
+__device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, FastLds &S)
+{
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int C = P.inter_nch;
+	const int lr = lane & 15, lq = lane >> 4;
+	const int kv_pad = (w.kv + 15) & ~15;
+	const DevWeights &dv = P.ic[0].v;  // every channel shares the lists
+	const DevWeights &dh = P.ic[0].h;
+
+	// Vertical weights of the band.
+	for (int t = tid; t < FAST_ROWS * kv_pad; t += blockDim.x) {
+		const int r = t / kv_pad, k = t % kv_pad;
+		float v = 0.0f;
+		const int j = w.j0 + r;
+		if (j < P.Hp && k < w.kv) {
+			const int kk = w.p_lo + k - dv.dfirst[j];
+			if (kk >= 0 && kk < dv.dcount[j]) v = (float)dv.dw[dv.doff[j] + kk];
+		}
+		S.wv[r][k] = (_Float16)v;
+	}
+	__syncthreads();
+
+	// Vertical pass: 16-column tiles of the intermediate, one per wave in turn.
+	for (int ct = wave; ct * 16 < w.nx; ct += 4) {
+		const int col = ct * 16 + lr;
+		const int x = w.x_lo + col;
+		const bool outside = x < 0 || x >= P.W;
+		const int xc = x < 0 ? 0 : (x >= P.W ? P.W - 1 : x);
+		float4_t acc[IWAMD_MAX_CH];
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+		for (int kk = 0; kk < kv_pad; kk += 16) {
+			half4_t a;
+#pragma unroll
+			for (int t = 0; t < 4; t++) a[t] = S.wv[lr][kk + 4 * lq + t];
+			half4_t b[IWAMD_MAX_CH];
+#pragma unroll
+			for (int t = 0; t < 4; t++) {
+				const int k = kk + 4 * lq + t;
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++) {
+					float v = 0.0f;
+					if (c < C && k < w.kv && col < w.nx) v = (float)band_sample(P, P.ic[c], xc, w.p_lo + k, P.ih);
+					b[c][t] = (_Float16)v;
+				}
+			}
+#pragma unroll
+			for (int c = 0; c < IWAMD_MAX_CH; c++)
+				if (c < C) acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b[c], acc[c], 0, 0, 0);
+		}
+		if (col < w.nx) {
+#pragma unroll
+			for (int c = 0; c < IWAMD_MAX_CH; c++) {
+				if (c >= C) continue;
+#pragma unroll
+				for (int t = 0; t < 4; t++) {
+					float v = acc[c][t];
+					// A virtual column outside a TRANSPARENT edge is the edge value.
+					if (outside && P.ic[c].h.edge_transparent) v = (float)P.ic[c].h.edge;
+					if (P.intclamp) v = fminf(fmaxf(v, 0.0f), 1.0f);
+					S.inter[c][4 * lq + t][col] = (_Float16)v;
+				}
+			}
+		}
+	}
+	__syncthreads();
+
+	// Horizontal pass: 16-column output groups, one per wave.
+	for (int g = wave; g * 16 < w.ni; g += 4) {
+		const int i = w.i0 + g * 16 + lr;
+		const bool live = (g * 16 + lr) < w.ni;
+		int f = 0x7fffffff, e = -0x7fffffff;
+		if (live && dh.dcount[i] > 0) {
+			f = dh.dfirst[i];
+			e = f + dh.dcount[i] - 1;
+		}
+		for (int m = 1; m < 16; m <<= 1) {
+			f = min(f, __shfl_xor(f, m));
+			e = max(e, __shfl_xor(e, m));
+		}
+		float4_t acc[IWAMD_MAX_CH];
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+		if (e >= f) {
+			for (int x0 = f - w.x_lo; x0 <= e - w.x_lo; x0 += 16) {
+				half4_t b;
+				for (int t = 0; t < 4; t++) {
+					const int xk = x0 + 4 * lq + t;
+					float v = 0.0f;
+					if (live) {
+						const int kk = w.x_lo + xk - dh.dfirst[i];
+						if (kk >= 0 && kk < dh.dcount[i]) v = (float)dh.dw[dh.doff[i] + kk];
+					}
+					b[t] = (_Float16)v;
+				}
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++) {
+					if (c >= C) continue;
+					half4_t a;
+#pragma unroll
+					for (int t = 0; t < 4; t++) {
+						const int xk = x0 + 4 * lq + t;
+						a[t] = (xk >= 0 && xk < w.nx) ? S.inter[c][lr][xk] : (_Float16)0.0f;
+					}
+					acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, acc[c], 0, 0, 0);
+				}
+			}
+		}
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++)
+			if (c < C)
+#pragma unroll
+				for (int t = 0; t < 4; t++) S.hout[c][4 * lq + t][g * 16 + lr] = acc[c][t];
+	}
+	__syncthreads();
+
+	// Per-pixel work in FP64, as in the plain form.
+	const int rows = min(FAST_ROWS, P.Hp - w.j0);
+	for (int t = tid; t < rows * w.ni; t += blockDim.x) {
+		const int r = t / w.ni, ii = t % w.ni;
+		emit_pixel(P, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
+	}
+}
+
+__global__ void __launch_bounds__(256) fast_kernel(const DevPlan *__restrict__ plans,
+	const FastItem *__restrict__ items, int n_items, int *counter)
+{
+	__shared__ FastLds S;
+	__shared__ int next_item;
+	int it = blockIdx.x;
+	while (it < n_items) {
+		const FastItem w = items[it];
+		fast_body(plans[w.job], w, S);
+		if (!counter) break;
+		__syncthreads();
+		if (threadIdx.x == 0) next_item = (int)gridDim.x + atomicAdd(counter, 1);
+		__syncthreads();
+		it = next_item;
+	}
+}
+
 // ---- error diffusion ------------------------------------------------------
 //
 // A plane is one dependency chain: rows alternate direction (src/imagew-main.c:
@@ -1028,7 +1205,8 @@ struct QOff { size_t T, U, lin, code, nc, prow, pcol, M, pre, suf; };
 
 struct Staged {
 	std::vector<iwx::WeightList> wv, wh;
-	std::vector<iwx::DenseWeights> dv;
+	std::vector<iwx::DenseWeights> dv, dh;
+	std::vector<FastItem> fast;  // chosen-precision tiles (job index set when placed); empty: plain
 	std::vector<std::vector<double>> luts;
 	int ic_lut[4][3];
 	int alpha_lut = -1;
@@ -1079,6 +1257,68 @@ int choose_band_rows(const iwamd_plan &p, const std::vector<iwx::DenseWeights> &
 	return 0;
 }
 
+bool same_dense(const iwx::DenseWeights &a, const iwx::DenseWeights &b)
+{
+	return a.first == b.first && a.count == b.count && a.off == b.off && a.w == b.w;
+}
+
+// Tiles of the chosen-precision form, or none where it does not apply: it
+// covers outputs of up to 8 bits whose channels share one weight list per
+// dimension, with band and tile spans that fit its LDS; the widest tile
+// (64, 32 or 16 output columns) that fits is used for the whole image.
+void plan_fast(const iwamd_plan &p, Staged &S)
+{
+	S.fast.clear();
+	if (p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT || p.out_bit_depth > 8) return;
+	for (int c = 1; c < p.inter_nch; c++) {
+		if (!same_dense(S.dv[c], S.dv[0]) || !same_dense(S.dh[c], S.dh[0])) return;
+		if (S.wv[c].edge_value != S.wv[0].edge_value) return;
+	}
+	const iwx::DenseWeights &dv = S.dv[0], &dh = S.dh[0];
+	auto span = [](const iwx::DenseWeights &d, int a, int n, int &lo, int &cnt) {
+		int l = 0x7fffffff, h = -0x7fffffff;
+		for (int k = a; k < a + n; k++) {
+			if (!d.count[k]) continue;
+			l = std::min(l, d.first[k]);
+			h = std::max(h, d.first[k] + d.count[k] - 1);
+		}
+		if (h < l) { l = 0; h = -1; }
+		lo = l;
+		cnt = h - l + 1;
+	};
+	for (int j0 = 0; j0 < p.inter_height; j0 += FAST_ROWS) {
+		int lo, n;
+		span(dv, j0, std::min(FAST_ROWS, p.inter_height - j0), lo, n);
+		if (n > FAST_MAX_KV) return;
+	}
+	int ti = 0;
+	for (int cand : {64, 32, 16}) {
+		bool fits = true;
+		for (int i0 = 0; i0 < p.out_width && fits; i0 += cand) {
+			int lo, n;
+			span(dh, i0, std::min(cand, p.out_width - i0), lo, n);
+			fits = n <= FAST_MAX_XT;
+		}
+		if (fits) {
+			ti = cand;
+			break;
+		}
+	}
+	if (!ti) return;
+	for (int j0 = 0; j0 < p.inter_height; j0 += FAST_ROWS) {
+		FastItem it;
+		it.job = 0;
+		it.j0 = j0;
+		span(dv, j0, std::min(FAST_ROWS, p.inter_height - j0), it.p_lo, it.kv);
+		for (int i0 = 0; i0 < p.out_width; i0 += ti) {
+			it.i0 = i0;
+			it.ni = std::min(ti, p.out_width - i0);
+			span(dh, i0, it.ni, it.x_lo, it.nx);
+			S.fast.push_back(it);
+		}
+	}
+}
+
 // Build the host tables of a plan, choose its path and lay out its device
 // data (offsets from the job's base).  path: "band", "twopass" or null.
 bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path, bool out_elsewhere = false)
@@ -1099,6 +1339,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	S.wv.resize(p.inter_nch);
 	S.wh.resize(p.inter_nch);
 	S.dv.resize(p.inter_nch);
+	S.dh.resize(p.inter_nch);
 	for (int c = 0; c < p.inter_nch; c++) {
 		S.wv[c] = build_weight_list(p.rs[IW_DIMENSION_V], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_V]);
 		S.wh[c] = build_weight_list(p.rs[IW_DIMENSION_H], p.ich[c].channeltype, p.ich[c].edge_sample_value[IW_DIMENSION_H]);
@@ -1109,6 +1350,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 			S.skip_pixels = true;
 		}
 		S.dv[c] = dense_weights(S.wv[c]);
+		S.dh[c] = dense_weights(S.wh[c]);
 	}
 	S.in_int = p.in_sampletype != IW_SAMPLETYPE_FLOATINGPOINT;
 	const int nvalues = S.in_int ? (1 << p.in_bit_depth) : 0;
@@ -1184,8 +1426,11 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	// two-pass kernels with the intermediate in HBM.
 	const bool force_two = path && !strcmp(path, "twopass");
 	const bool force_band = path && !strcmp(path, "band");
-	S.band_rows = force_two ? 0 : choose_band_rows(p, S.dv);
-	if (!force_band && S.band_rows && p.inter_width > 2048) S.band_rows = 0;
+	if (p.precision == IWAMD_PRECISION_FAST && !S.skip_pixels) plan_fast(p, S);
+	if (S.fast.empty()) {
+		S.band_rows = force_two ? 0 : choose_band_rows(p, S.dv);
+		if (!force_band && S.band_rows && p.inter_width > 2048) S.band_rows = 0;
+	}
 
 	Layout L;
 	S.in_bytes = p.in_bpr * (size_t)phys_rows;
@@ -1212,6 +1457,10 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 		S.ov[c].dcount = L.add<int>(S.dv[c].count.size());
 		S.ov[c].doff = L.add<int>(S.dv[c].off.size());
 		S.ov[c].dw = L.add<double>(S.dv[c].w.size());
+		S.oh[c].dfirst = L.add<int>(S.dh[c].first.size());
+		S.oh[c].dcount = L.add<int>(S.dh[c].count.size());
+		S.oh[c].doff = L.add<int>(S.dh[c].off.size());
+		S.oh[c].dw = L.add<double>(S.dh[c].w.size());
 	}
 	S.oq.resize(p.out_nch);
 	for (int o = 0; o < (int)S.qt.size(); o++) {
@@ -1254,6 +1503,10 @@ bool upload_job(const Staged &S, const iwamd_plan &p, char *base, hipStream_t st
 		up(S.ov[c].dcount, S.dv[c].count.data(), S.dv[c].count.size() * 4);
 		up(S.ov[c].doff, S.dv[c].off.data(), S.dv[c].off.size() * 4);
 		up(S.ov[c].dw, S.dv[c].w.data(), S.dv[c].w.size() * 8);
+		up(S.oh[c].dfirst, S.dh[c].first.data(), S.dh[c].first.size() * 4);
+		up(S.oh[c].dcount, S.dh[c].count.data(), S.dh[c].count.size() * 4);
+		up(S.oh[c].doff, S.dh[c].off.data(), S.dh[c].off.size() * 4);
+		up(S.oh[c].dw, S.dh[c].w.data(), S.dh[c].w.size() * 8);
 	}
 	for (int o = 0; o < (int)S.qt.size(); o++) {
 		const QOff &q = S.oq[o];
@@ -1328,6 +1581,11 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 		d.v.doff = (const int *)(base + S.ov[c].doff);
 		d.v.dw = (const double *)(base + S.ov[c].dw);
 		d.v.dmax = S.dv[c].max_count;
+		d.h.dfirst = (const int *)(base + S.oh[c].dfirst);
+		d.h.dcount = (const int *)(base + S.oh[c].dcount);
+		d.h.doff = (const int *)(base + S.oh[c].doff);
+		d.h.dw = (const double *)(base + S.oh[c].dw);
+		d.h.dmax = S.dh[c].max_count;
 	}
 	D.inter = (float *)(base + S.o_inter);
 	D.intclamp = p.intclamp;
@@ -1410,7 +1668,9 @@ struct WorkSet {
 	size_t band_lds[3] = {0, 0, 0};   // largest LDS of a band of each height
 	std::vector<int2> planes[8];      // (plan, output channel), by ed_variant
 	std::vector<int> twopass;         // plans resampled by vpass + hpass
-	size_t o_plans = 0, o_bands[3] = {0, 0, 0}, o_planes[8] = {0}, o_counters = 0, bytes = 0;
+	std::vector<FastItem> fast;       // chosen-precision tiles
+	size_t o_plans = 0, o_bands[3] = {0, 0, 0}, o_planes[8] = {0}, o_counters = 0, o_fast = 0, bytes = 0;
+	const FastItem *d_fast = nullptr;
 	const DevPlan *d_plans = nullptr;
 	const int2 *d_bands[3] = {nullptr, nullptr, nullptr};
 	const int2 *d_planes[8] = {nullptr};
@@ -1424,7 +1684,12 @@ int ws_add(WorkSet &W, const Staged &S, const iwamd_plan &p)
 	const int jx = (int)W.plans.size();
 	W.plans.push_back(DevPlan());
 	if (S.skip_pixels) return jx;
-	if (S.band_rows) {
+	if (!S.fast.empty()) {
+		for (FastItem it : S.fast) {
+			it.job = jx;
+			W.fast.push_back(it);
+		}
+	} else if (S.band_rows) {
 		const int ri = S.band_rows == 4 ? 2 : (S.band_rows == 2 ? 1 : 0);
 		for (int j0 = 0; j0 < p.inter_height; j0 += S.band_rows) W.bands[ri].push_back(make_int2(jx, j0));
 		W.band_lds[ri] = std::max(W.band_lds[ri], band_lds_bytes(S.band_rows, p.inter_nch, p.inter_width));
@@ -1444,7 +1709,8 @@ void ws_layout(WorkSet &W)
 	W.o_plans = L.add<DevPlan>(W.plans.size());
 	for (int r = 0; r < 3; r++) W.o_bands[r] = L.add<int2>(W.bands[r].size());
 	for (int v = 0; v < 8; v++) W.o_planes[v] = L.add<int2>(W.planes[v].size());
-	W.o_counters = L.add<int>(3);
+	W.o_counters = L.add<int>(4);
+	W.o_fast = L.add<FastItem>(W.fast.size());
 	W.bytes = (L.off + 255) & ~(size_t)255;
 }
 
@@ -1467,6 +1733,8 @@ bool ws_upload(WorkSet &W, char *base, hipStream_t st)
 		up(W.o_planes[v], W.planes[v].data(), W.planes[v].size() * sizeof(int2));
 	}
 	W.d_counters = (int *)(base + W.o_counters);
+	W.d_fast = (const FastItem *)(base + W.o_fast);
+	up(W.o_fast, W.fast.data(), W.fast.size() * sizeof(FastItem));
 	return ok;
 }
 
@@ -1489,7 +1757,13 @@ int device_cu_count()
 // otherwise one workgroup per band.
 void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 {
-	if (persistent) hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 3);
+	if (persistent) hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 4);
+	if (!W.fast.empty()) {
+		const int n = (int)W.fast.size();
+		const int grid = persistent ? std::min(n, device_cu_count() * 2) : n;
+		hipLaunchKernelGGL(fast_kernel, dim3(grid), dim3(256), 0, st, W.d_plans, W.d_fast, n,
+			persistent ? W.d_counters + 3 : nullptr);
+	}
 	for (int r = 0; r < 3; r++) {
 		const int n = (int)W.bands[r].size();
 		if (!n) continue;
@@ -1591,7 +1865,7 @@ extern "C" int iwamd_job_run(struct iwamd_job *J)
 	const WorkSet &W = *(const WorkSet *)J->ws;
 	hipStream_t st = J->stream;
 	hipEventRecord(J->ev[0], st);
-	if (J->band_rows) {
+	if (W.twopass.empty()) {
 		// One fused kernel; the horizontal-pass slot of the timings stays 0.
 		ws_enqueue_resample(W, st, false);
 		hipEventRecord(J->ev[1], st);

@@ -1,0 +1,67 @@
+"""The chosen-precision form (IWAMD_PRECISION_FAST: FP16 MFMA resampling,
+FP32 sums, FP64 quantisation) against the plain FP64 form on the GPU, whose
+bytes equal the reference's.  Bar: at most 1 LSB on 8-bit outputs without
+error diffusion (ordered and random dither included: a decision can move by
+one level); with error diffusion a small difference reroutes the error, so
+only the mean is bounded there."""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from imageworsener_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def fast(job):
+    return dict(job, precision=api.PRECISION_FAST)
+
+
+def lsb_stats(a, b):
+    assert a["ok"] and b["ok"], (a["error"], b["error"])
+    assert (a["width"], a["height"], a["bit_depth"]) == (b["width"], b["height"], b["bit_depth"])
+    d = np.abs(a["pixels"].astype(np.int32) - b["pixels"].astype(np.int32))
+    return int(d.max()) if d.size else 0, float(d.mean()) if d.size else 0.0
+
+
+def test_fast_c0(prod):
+    job = cases.c0_job()
+    mx, mean = lsb_stats(prod.run(job), prod.run(fast(job)))
+    say("C0 fast vs plain: max %d LSB, mean %.5f LSB" % (mx, mean))
+    assert mx <= 1 and mean < 0.05
+
+
+@pytest.mark.parametrize("name,job", [(n, j) for n, j in cases.sweep_jobs()
+                                      if j.get("depth", 8) == 8 and "-f" not in n and "jjn" not in n
+                                      and "stucki" not in n and "burkes" not in n and "sierra" not in n
+                                      and "atkinson" not in n and "wide-ed" not in n],
+                         ids=lambda v: v if isinstance(v, str) else "")
+def test_fast_sweep(prod, name, job):
+    mx, mean = lsb_stats(prod.run(job), prod.run(fast(job)))
+    assert mx <= 1, (name, mx, mean)
+
+
+def test_fast_c3_batch(prod):
+    jobs = cases.c3_jobs(count=300, max_side=256)
+    plain = prod.run_batch(jobs)
+    quick = prod.run_batch([fast(j) for j in jobs])
+    worst = 0
+    for k, (a, b) in enumerate(zip(plain, quick)):
+        mx, _ = lsb_stats(a, b)
+        worst = max(worst, mx)
+    say("C3 subset (%d jobs) fast vs plain: max %d LSB" % (len(jobs), worst))
+    assert worst <= 1
+
+
+def test_fast_error_diffusion_mean(prod):
+    job = cases.c1_job(size=512, out=125)
+    mx, mean = lsb_stats(prod.run(job), prod.run(fast(job)))
+    say("C1/8 FS fast vs plain: max %d LSB, mean %.4f LSB" % (mx, mean))
+    assert mean < 0.5
