@@ -128,6 +128,13 @@ _EXT_SIGS = {
     "iwamd_batch_kernel_ms": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_double), _c.c_int]),
     "iwamd_batch_sizes": (None, [_c.c_void_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double),
                                  _c.POINTER(_c.c_int)]),
+    "iwamd_server_create": (_c.c_void_p, [_c.c_size_t, _c.c_int]),
+    "iwamd_server_submit": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
+    "iwamd_server_wait": (_c.c_int, [_c.c_void_p, _c.c_int]),
+    "iwamd_server_drain": (_c.c_int, [_c.c_void_p]),
+    "iwamd_server_request_ms": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(_c.c_double),
+                                           _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "iwamd_server_destroy": (None, [_c.c_void_p]),
 }
 
 # Byte offsets into struct iwamd_plan used by executors driven from Python.
@@ -358,6 +365,54 @@ class Batch:
     def close(self):
         if self.h:
             self.lib.dll.iwamd_batch_destroy(self.h)
+            self.h = None
+        for ctx in self.ctxs:
+            self.lib.dll.iw_destroy_context(ctx)
+        self.ctxs = []
+
+
+class Server:
+    """The request loop (iwamd_server_*): submit() queues a job and returns at
+    once; the library's worker admits jobs by HBM budget and runs them in
+    overlapping batches.  results() returns every job's result in submit
+    order, with its submit/start/done times (ms)."""
+
+    def __init__(self, lib, hbm_budget=0, streams=4):
+        self.lib = lib
+        self.h = lib.dll.iwamd_server_create(hbm_budget, streams)
+        if not self.h:
+            raise RuntimeError("iwamd_server_create failed")
+        self.ctxs, self.ids = [], []
+
+    def submit(self, job):
+        d = self.lib.dll
+        ctx = d.iw_create_context(None)
+        if not ctx:
+            raise MemoryError("iw_create_context failed")
+        self.lib._setup(ctx, job)
+        rid = d.iwamd_server_submit(self.h, ctx)
+        self.ctxs.append(ctx)
+        self.ids.append(rid)
+        return rid
+
+    def drain(self):
+        return bool(self.lib.dll.iwamd_server_drain(self.h))
+
+    def results(self):
+        self.drain()
+        out = []
+        for ctx, rid in zip(self.ctxs, self.ids):
+            ok = rid >= 0 and bool(self.lib.dll.iwamd_server_wait(self.h, rid))
+            r = self.lib._collect(ctx, ok)
+            t = [ctypes.c_double() for _ in range(3)]
+            if rid >= 0 and self.lib.dll.iwamd_server_request_ms(self.h, rid, *[ctypes.byref(v) for v in t]):
+                r["times_ms"] = tuple(v.value for v in t)
+            out.append(r)
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.dll.iwamd_server_destroy(self.h)
             self.h = None
         for ctx in self.ctxs:
             self.lib.dll.iw_destroy_context(ctx)

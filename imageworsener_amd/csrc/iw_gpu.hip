@@ -25,9 +25,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -2176,6 +2179,279 @@ extern "C" void iwamd_batch_destroy(struct iwamd_batch *B)
 	if (B->stream) hipStreamDestroy(B->stream);
 	for (auto *J : B->jobs) delete J;
 	delete B;
+}
+
+// ---- request loop (continuous batching) -------------------------------------
+// iwamd_server_submit() runs a request's host decisions and table building on
+// the caller's thread and queues it.  One worker thread admits queued
+// requests while their device bytes fit the HBM budget and launches every
+// request admitted at the same moment as one batch on a free stream (one
+// device block, persistent band kernels, error-diffusion planes, the outputs
+// copied back on the same stream).  Up to `streams` batches run at once, so a
+// request that arrives while others are on the GPU starts as soon as memory
+// and a stream free up, and its work overlaps theirs.  A finished batch
+// releases its memory and completes its requests' contexts.  Each request's
+// bytes depend only on its own plan.
+
+struct iwamd_server {
+	struct Req {
+		int id = 0, widx = 0;
+		iw_context *ctx = nullptr;
+		iwamd_plan plan;
+		Staged S;
+		size_t off = 0, out_off = 0;
+		double t_submit = 0, t_start = 0, t_done = 0;
+		double out_pixels = 0;
+		bool done = false, ok = false, skip = false;
+	};
+	struct Flight {
+		std::vector<Req *> reqs;
+		WorkSet W;
+		void *block = nullptr;
+		size_t bytes = 0, out_base = 0, out_total = 0;
+		uint8_t *host = nullptr;
+		hipStream_t st = nullptr;
+		hipEvent_t ev = nullptr;
+		bool failed = false;
+	};
+	std::mutex mu;
+	std::condition_variable cv_work, cv_done;
+	std::thread worker;
+	bool stop = false;
+	size_t budget = 0, used = 0;
+	std::vector<hipStream_t> streams_free;
+	std::vector<hipStream_t> streams_all;
+	std::deque<Req *> queue;
+	std::vector<Flight *> flights;
+	std::map<int, Req *> reqs;
+	int next_id = 0;
+	int device = 0;
+	std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+	double now() const
+	{
+		return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+	}
+};
+
+namespace {
+
+// Bytes a request holds on the device while it runs.
+size_t req_bytes(const iwamd_server::Req &R)
+{
+	return R.S.bytes + ((R.S.out_bytes + 255) & ~(size_t)255) + 64 * 1024;
+}
+
+void server_launch(iwamd_server::Flight *F)
+{
+	size_t off = 0;
+	for (auto *R : F->reqs) {
+		R->off = off;
+		off += R->S.bytes;
+	}
+	F->out_base = off;
+	size_t arena = 0;
+	for (auto *R : F->reqs) {
+		R->out_off = arena;
+		arena += (R->S.out_bytes + 255) & ~(size_t)255;
+		R->widx = ws_add(F->W, R->S, R->plan);
+	}
+	F->out_total = arena;
+	ws_layout(F->W);
+	F->bytes = off + arena + F->W.bytes;
+	F->block = dev_alloc(F->bytes);
+	bool ok = F->block && hipHostMalloc((void **)&F->host, arena + 1, hipHostMallocDefault) == hipSuccess;
+	if (ok) {
+		char *base = (char *)F->block;
+		for (auto *R : F->reqs) {
+			DevPlan &D = F->W.plans[R->widx];
+			fill_dev_plan(R->S, R->plan, base + R->off, D);
+			D.out = (uint8_t *)(base + F->out_base + R->out_off);
+			ok &= upload_job(R->S, R->plan, base + R->off, F->st);
+		}
+		ok &= ws_upload(F->W, base + F->out_base + F->out_total, F->st);
+	}
+	if (ok) {
+		ws_enqueue_resample(F->W, F->st, true);
+		ws_enqueue_errdiff(F->W, F->st);
+		ok = hipMemcpyAsync(F->host, (char *)F->block + F->out_base, arena, hipMemcpyDeviceToHost, F->st) ==
+			hipSuccess;
+	}
+	F->failed = !ok;
+	hipEventRecord(F->ev, F->st);
+}
+
+// Completes a finished flight's requests; returns its bytes and stream.
+void server_complete(iwamd_server *V, iwamd_server::Flight *F)
+{
+	(void)hipEventSynchronize(F->ev);
+	const double t = V->now();
+	std::vector<uint8_t> zeros;
+	for (auto *R : F->reqs) {
+		bool ok = !F->failed;
+		if (ok) {
+			const uint8_t *img2 = F->host + R->out_off;
+			if (R->skip) {
+				zeros.assign(R->plan.out_bpr * (size_t)R->plan.out_height + 1, 0);
+				img2 = zeros.data();
+			}
+			ok = iwamd_finish(R->ctx, img2) != 0;
+		} else {
+			iw_set_error(R->ctx, "Request could not be run on the GPU");
+		}
+		R->S = Staged();
+		std::lock_guard<std::mutex> lock(V->mu);
+		R->ok = ok;
+		R->done = true;
+		R->t_done = t;
+	}
+	dev_release(F->block, F->bytes);
+	if (F->host) (void)hipHostFree(F->host);
+	hipEventDestroy(F->ev);
+}
+
+void server_loop(iwamd_server *V)
+{
+	(void)hipSetDevice(V->device);
+	std::unique_lock<std::mutex> lock(V->mu);
+	for (;;) {
+		// Completed flights first: they free memory and streams.
+		for (size_t k = 0; k < V->flights.size();) {
+			iwamd_server::Flight *F = V->flights[k];
+			if (hipEventQuery(F->ev) == hipSuccess) {
+				V->flights.erase(V->flights.begin() + k);
+				lock.unlock();
+				server_complete(V, F);
+				lock.lock();
+				size_t b = 0;
+				for (auto *R : F->reqs) b += req_bytes(*R);
+				V->used -= std::min(V->used, b);
+				V->streams_free.push_back(F->st);
+				delete F;
+				V->cv_done.notify_all();
+			} else {
+				k++;
+			}
+		}
+		// Admit what fits, as one batch on one stream.
+		if (!V->queue.empty() && !V->streams_free.empty()) {
+			iwamd_server::Flight *F = new iwamd_server::Flight();
+			while (!V->queue.empty()) {
+				iwamd_server::Req *R = V->queue.front();
+				const size_t b = req_bytes(*R);
+				// An idle device takes a request even above the budget.
+				if (V->used + b > V->budget && !(V->used == 0 && F->reqs.empty())) break;
+				V->queue.pop_front();
+				V->used += b;
+				R->t_start = V->now();
+				F->reqs.push_back(R);
+			}
+			if (F->reqs.empty()) {
+				delete F;
+			} else {
+				F->st = V->streams_free.back();
+				V->streams_free.pop_back();
+				hipEventCreateWithFlags(&F->ev, hipEventDisableTiming);
+				V->flights.push_back(F);
+				lock.unlock();
+				server_launch(F);
+				lock.lock();
+				continue;
+			}
+		}
+		if (V->stop && V->queue.empty() && V->flights.empty()) break;
+		// Wait for work or poll running flights.
+		if (V->flights.empty())
+			V->cv_work.wait(lock, [&] { return V->stop || !V->queue.empty(); });
+		else
+			V->cv_work.wait_for(lock, std::chrono::microseconds(200));
+	}
+}
+
+}  // namespace
+
+extern "C" struct iwamd_server *iwamd_server_create(size_t hbm_budget_bytes, int streams)
+{
+	iwamd_server *V = new iwamd_server();
+	V->budget = hbm_budget_bytes ? hbm_budget_bytes : ((size_t)64 << 30);
+	if (streams < 1) streams = 1;
+	if (streams > 16) streams = 16;
+	if (hipGetDevice(&V->device) != hipSuccess) V->device = 0;
+	for (int k = 0; k < streams; k++) {
+		hipStream_t st = nullptr;
+		if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) break;
+		V->streams_all.push_back(st);
+		V->streams_free.push_back(st);
+	}
+	if (V->streams_all.empty()) {
+		delete V;
+		return nullptr;
+	}
+	V->worker = std::thread(server_loop, V);
+	return V;
+}
+
+extern "C" int iwamd_server_submit(struct iwamd_server *V, struct iw_context *ctx)
+{
+	iwamd_server::Req *R = new iwamd_server::Req();
+	R->ctx = ctx;
+	R->t_submit = V->now();
+	if (!iwamd_prepare(ctx, &R->plan) || !runtime_init(ctx) ||
+		!stage_job(ctx, R->plan, R->S, iw_get_option(ctx, "amd:resample"), true)) {
+		delete R;
+		return -1;
+	}
+	R->skip = R->S.skip_pixels;
+	R->out_pixels = (double)R->plan.out_width * R->plan.out_height;
+	std::lock_guard<std::mutex> lock(V->mu);
+	R->id = V->next_id++;
+	V->reqs[R->id] = R;
+	V->queue.push_back(R);
+	V->cv_work.notify_one();
+	return R->id;
+}
+
+extern "C" int iwamd_server_wait(struct iwamd_server *V, int id)
+{
+	std::unique_lock<std::mutex> lock(V->mu);
+	auto it = V->reqs.find(id);
+	if (it == V->reqs.end()) return 0;
+	iwamd_server::Req *R = it->second;
+	V->cv_done.wait(lock, [&] { return R->done; });
+	return R->ok ? 1 : 0;
+}
+
+extern "C" int iwamd_server_drain(struct iwamd_server *V)
+{
+	std::unique_lock<std::mutex> lock(V->mu);
+	V->cv_done.wait(lock, [&] { return V->queue.empty() && V->flights.empty(); });
+	int ok = 1;
+	for (auto &kv : V->reqs) ok &= kv.second->ok ? 1 : 0;
+	return ok;
+}
+
+extern "C" int iwamd_server_request_ms(struct iwamd_server *V, int id, double *submit, double *start, double *done)
+{
+	std::lock_guard<std::mutex> lock(V->mu);
+	auto it = V->reqs.find(id);
+	if (it == V->reqs.end() || !it->second->done) return 0;
+	if (submit) *submit = it->second->t_submit;
+	if (start) *start = it->second->t_start;
+	if (done) *done = it->second->t_done;
+	return 1;
+}
+
+extern "C" void iwamd_server_destroy(struct iwamd_server *V)
+{
+	if (!V) return;
+	{
+		std::lock_guard<std::mutex> lock(V->mu);
+		V->stop = true;
+	}
+	V->cv_work.notify_all();
+	if (V->worker.joinable()) V->worker.join();
+	for (hipStream_t st : V->streams_all) hipStreamDestroy(st);
+	for (auto &kv : V->reqs) delete kv.second;
+	delete V;
 }
 
 int iwx_gpu_device_count(void)

@@ -15,6 +15,9 @@ the same bytes on any host.  Draw order is part of each recipe.
   c3_list()/c3_image(): 4096 RGBA8 images, seed 4, sides log-uniform in
         [64, 1024], per-axis scale U[0.25, 2], filter from {triangle, catrom,
         lanczos2, lanczos3}
+  c4_list()/c4_image(): serving requests, seed 5: RGB 8- or 16-bit (p=0.5),
+        sides log-uniform in [256, 8192], C1's RGB recipe, one scale U[0.1,
+        1.5], Lanczos3 or Mitchell (p=0.5), Poisson arrival gaps
 """
 import numpy as np
 
@@ -176,3 +179,53 @@ def c3_image(w, h, seed):
     vals[u[:, 1] < 0.1] = 0.0
     a = np.repeat(np.repeat(vals.reshape(th, tw), 16, 0), 16, 1)[:h, :w]
     return np.concatenate([rgb, a[..., None].astype(np.uint8)], axis=2)
+
+
+def gratings_rgb(w, h, depth, seed):
+    """C1's RGB recipe at w x h: three sinusoid gratings of amplitude 1/6 of
+    full scale around mid-grey, plus uniform noise of 1/32 of full scale,
+    clamped; uint8 or uint16 samples, shape (h, w, 3)."""
+    rng = SplitMix64(seed)
+    full = 255.0 if depth == 8 else 65535.0
+    mid, amp = (full + 1.0) / 2.0, (full + 1.0) / 6.0
+    nz = int(full + 1.0) // 32
+    xs, ys = np.arange(w, dtype=np.float64), np.arange(h, dtype=np.float64)
+    size = float(max(w, h))
+    out = np.empty((h, w, 3), np.uint8 if depth == 8 else np.uint16)
+    for ch in range(3):
+        params = rng.uniform(9).reshape(3, 3)
+        v = np.full((h, w), mid)
+        for f_u, th_u, ph_u in params:
+            f = 1.0 + f_u * 255.0
+            th, ph = th_u * PI, ph_u * TWO_PI
+            cth, sth = sin_poly(th + PI / 2), sin_poly(th)
+            ax = (TWO_PI * f / size) * cth * xs + ph
+            by = (TWO_PI * f / size) * sth * ys
+            sx, cx = sin_poly(ax), sin_poly(ax + PI / 2)
+            sy, cy = sin_poly(by), sin_poly(by + PI / 2)
+            v += amp * (sy[:, None] * cx[None, :] + cy[:, None] * sx[None, :])
+        v += rng.ints(w * h, -nz, nz).reshape(h, w)
+        out[..., ch] = _round_clip(v, full).astype(out.dtype)
+    return out
+
+
+C4_FILTERS = ["lanczos3", "mitchell"]
+
+
+def c4_list(count=1000, seed=5, lo_side=256, hi_side=8192):
+    """[(w, h, depth, out_w, out_h, filter, image_seed, gap)] for the serving
+    config; gap is the exponential inter-arrival time in units of 1/rate."""
+    rng = SplitMix64(seed)
+    u = rng.uniform(count * 6).reshape(count, 6)
+    lo, hi = np.log(float(lo_side)), np.log(float(hi_side))
+    out = []
+    for i in range(count):
+        w = int(np.floor(np.exp(lo + u[i, 0] * (hi - lo)) + 0.5))
+        h = int(np.floor(np.exp(lo + u[i, 1] * (hi - lo)) + 0.5))
+        depth = 8 if u[i, 2] < 0.5 else 16
+        sc = 0.1 + u[i, 3] * 1.4
+        ow, oh = max(1, int(np.floor(w * sc + 0.5))), max(1, int(np.floor(h * sc + 0.5)))
+        filt = C4_FILTERS[0 if u[i, 4] < 0.5 else 1]
+        gap = -np.log(1.0 - u[i, 5])
+        out.append((w, h, depth, ow, oh, filt, 2000 + i, gap))
+    return out

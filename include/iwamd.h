@@ -193,6 +193,28 @@ IW_EXPORT(void) iwamd_batch_destroy(struct iwamd_batch *batch);
 IW_EXPORT(int) iwamd_batch_kernel_ms(struct iwamd_batch *batch, double *ms, int n);
 IW_EXPORT(void) iwamd_batch_sizes(struct iwamd_batch *batch, double *out_pixels, double *alg_bytes, int *n_jobs);
 
+/* Request loop with continuous batching.  iwamd_server_create() starts a
+ * worker thread on the current device with `streams` streams and an HBM
+ * budget in bytes (0: 64 GiB).  iwamd_server_submit() runs a context's host
+ * decisions on the caller's thread (consuming its single use) and queues it;
+ * it returns a request id, or -1 with the context's error set.  The worker
+ * admits queued requests while their device bytes fit the budget and
+ * launches all requests admitted together as one batch on a free stream;
+ * up to `streams` batches overlap on the device.  iwamd_server_wait() blocks
+ * until a request is done (1: its context holds the output, 0: failed, with
+ * the context's error set); iwamd_server_drain() waits for every request.
+ * iwamd_server_request_ms() gives a finished request's submit, start and
+ * done times in ms since the server started.  Contexts must outlive the
+ * server's use of them (until their request is done). */
+struct iwamd_server;
+IW_EXPORT(struct iwamd_server*) iwamd_server_create(size_t hbm_budget_bytes, int streams);
+IW_EXPORT(int) iwamd_server_submit(struct iwamd_server *server, struct iw_context *ctx);
+IW_EXPORT(int) iwamd_server_wait(struct iwamd_server *server, int id);
+IW_EXPORT(int) iwamd_server_drain(struct iwamd_server *server);
+IW_EXPORT(int) iwamd_server_request_ms(struct iwamd_server *server, int id, double *submit, double *start,
+	double *done);
+IW_EXPORT(void) iwamd_server_destroy(struct iwamd_server *server);
+
 /* Library/device information.  Returns the number of visible HIP devices
  * (0 when none), and fills name (may be NULL). */
 IW_EXPORT(int) iwamd_device_count(void);

@@ -103,6 +103,19 @@ def c3_jobs(count=4096, seed=4, max_side=None):
     return jobs
 
 
+def c4_requests(count=1000, seed=5, lo_side=256, hi_side=8192):
+    """Serving requests: (job, gap) with RGB 8/16-bit input, one scale per
+    request, Lanczos3 or Mitchell, 8-bit output with Floyd-Steinberg."""
+    out = []
+    for (w, h, depth, ow, oh, filt, iseed, gap) in synth.c4_list(count, seed, lo_side, hi_side):
+        img = synth.gratings_rgb(w, h, depth, iseed)
+        pix = img if depth == 8 else synth.be16(img)
+        job = api.image_job(pix, w, h, RGB, depth, canvas=(ow, oh), resize=api.resize_both(filt),
+                            dither=[(api.CH_ALL, 2, 0)], depth=8)
+        out.append((job, gap))
+    return out
+
+
 def assert_same(a, b, what=""):
     assert a["ok"], (what, a["error"])
     assert b["ok"], (what, b["error"])
