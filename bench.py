@@ -24,7 +24,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "output Mpix/s on 1 GPU, chosen vs FP64 plain at <=1 LSB err; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["vpass", "hpass", "errdiff"]
+# Per-kernel slots of iwamd_job_kernel_ms: the vertical pass (or the whole
+# fused resample: band kernel or the FP16-MFMA kernel), the horizontal pass
+# (0 when fused), error diffusion.
+KERNELS = ["resample", "hpass", "errdiff"]
 
 
 def parse():
@@ -36,8 +39,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--out", type=int, default=1000)
-    p.add_argument("--workload", choices=["c1", "c3"], default="c1",
-                   help="c1: the headline single image; c3: the 4096-image batch")
+    p.add_argument("--workload", choices=["c1", "c3", "c4"], default="c1",
+                   help="c1: the headline single image; c3: the 4096-image batch; c4: the request loop")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="after timing, run one plain device-to-device copy of MIB MiB (calibrates HBM counters)")
     return p.parse_args()
@@ -84,6 +87,22 @@ def measure(lib, job, steps, warmup):
             "out_pixels": dj.out_pixels, "alg_bytes": dj.alg_bytes, "result": res}
 
 
+def lsb_error(plain, fast):
+    """Max / mean |code difference| (output LSBs), the share of samples that
+    differ, and the same in linear light (sRGB codes decoded)."""
+    import numpy as np
+    if not (plain["ok"] and fast["ok"]):
+        return None
+    a = plain["pixels"].astype(np.int32)
+    b = fast["pixels"].astype(np.int32)
+    d = np.abs(a - b)
+    lin = lambda c: np.where(c / 255.0 <= 0.04045, c / 255.0 / 12.92, ((c / 255.0 + 0.055) / 1.055) ** 2.4)
+    dl = np.abs(lin(a.astype(np.float64)) - lin(b.astype(np.float64)))
+    return {"max_lsb": int(d.max()), "mean_lsb": round(float(d.mean()), 5),
+            "frac_differ": round(float((d > 0).mean()), 5),
+            "max_linear": round(float(dl.max()), 6), "mean_linear": round(float(dl.mean()), 7)}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy
@@ -120,27 +139,16 @@ def cpu_baseline(job, out_pixels):
             "ok": r["ok"]}, r
 
 
-def bench_c3(a):
-    """4096 RGBA8 images (sides log-uniform 64..1024, factors 0.25..2 per axis,
-    triangle / Catmull-Rom / Lanczos2 / Lanczos3, 8-bit, no dither) as one
-    batch: one device block, one hipGraph replay per step."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from imageworsener_amd import api
-    import cases
-    lib = api.Library()
-    log("generating the C3 batch (4096 images)")
-    t0 = time.perf_counter()
-    jobs = cases.c3_jobs(4096)
-    log("generated in %.1f s; staging and uploading" % (time.perf_counter() - t0))
+def run_batch_timed(lib, jobs, steps, warmup):
     b = lib.batch(jobs)
     try:
         out_px, alg_bytes, n = b.sizes()
-        for _ in range(a.warmup):
+        for _ in range(warmup):
             b.run()
         b.sync()
         lat, kern = [], []
         t0 = time.perf_counter()
-        for _ in range(a.steps):
+        for _ in range(steps):
             s0 = time.perf_counter()
             b.run()
             b.sync()
@@ -150,14 +158,42 @@ def bench_c3(a):
         res = b.finish()
     finally:
         b.close()
-    ms = wall * 1e3 / a.steps
-    kms = sum(kern) / len(kern)
+    return {"ms": wall * 1e3 / steps, "kernel_ms": sum(kern) / len(kern), "lat": sorted(lat), "out_px": out_px,
+            "alg_bytes": alg_bytes, "n": n, "res": res}
+
+
+def bench_c3(a):
+    """4096 RGBA8 images (sides log-uniform 64..1024, factors 0.25..2 per axis,
+    triangle / Catmull-Rom / Lanczos2 / Lanczos3, 8-bit, no dither) as one
+    batch: one device block, one hipGraph replay per step."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from imageworsener_amd import api
+    import cases
+    lib = api.Library()
+    log("generating the C3 batch (4096 images)")
+    t0 = time.perf_counter()
+    jobs = cases.c3_jobs(4096)
+    log("generated in %.1f s" % (time.perf_counter() - t0))
+    forms = {}
+    want = ["plain", "fast"] if a.precision == "both" else [a.precision]
+    for form in want:
+        prec = api.PRECISION_PLAIN if form == "plain" else api.PRECISION_FAST
+        log("C3 %s: staging, uploading, %d warmup + %d timed replays" % (form, a.warmup, a.steps))
+        forms[form] = run_batch_timed(lib, [dict(j, precision=prec) for j in jobs], a.steps, a.warmup)
+        log("C3 %s: %.3f ms/step" % (form, forms[form]["ms"]))
+    chosen = "fast" if "fast" in forms else "plain"
+    m = forms[chosen]
+    err = None
+    if "fast" in forms and "plain" in forms:
+        cat = lambda rs: {"ok": all(r["ok"] for r in rs), "pixels": np.concatenate([r["pixels"] for r in rs])}
+        err = lsb_error(cat(forms["plain"]["res"]), cat(forms["fast"]["res"]))
     base = None
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and "plain" in forms:
         ref_path = os.path.join(ROOT, "oracle", "_ref", "libiwref.so")
-        kind = "reference" if os.path.exists(ref_path) else None
-        if kind:
+        if os.path.exists(ref_path):
             ref = api.Library(ref_path, extensions=False)
+            res = forms["plain"]["res"]
             done_px, t_cpu, same, k = 0.0, 0.0, True, 0
             while k < len(jobs) and t_cpu < 10.0:
                 c0 = time.perf_counter()
@@ -166,22 +202,82 @@ def bench_c3(a):
                 done_px += r["width"] * r["height"]
                 same &= bool(r["ok"] and res[k]["ok"] and (r["pixels"] == res[k]["pixels"]).all())
                 k += 1
-            base = {"value": round(done_px / 1e6 / t_cpu, 4), "unit": "Mpix/s", "cores": 1, "kind": kind,
+            base = {"value": round(done_px / 1e6 / t_cpu, 4), "unit": "Mpix/s", "cores": 1, "kind": "reference",
                     "sample": "first %d of the 4096 images through iw_process_image on one host thread (%.1f s)"
                               % (k, t_cpu), "plain_bytes_identical": same}
-    line = {"metric": METRIC, "value": round(out_px / 1e6 / (ms / 1e3), 3), "unit": "Mpix/s", "n_gpus": 1,
-            "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+    ach = m["alg_bytes"] / (m["kernel_ms"] * 1e-3) / 1e9
+    line = {"metric": METRIC, "value": round(m["out_px"] / 1e6 / (m["ms"] / 1e3), 3), "unit": "Mpix/s",
+            "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(m["ms"], 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f16-mfma/f32" if chosen == "fast" else "f64", "data": "synthetic",
             "config": {"workload": "C3: 4096 RGBA8 images, sides 64..1024, x0.25..2 per axis, "
                                    "triangle/catrom/lanczos2/lanczos3, 8-bit, no dither; one batch",
-                       "global_batch": n, "precision": "plain"},
-            "roofline": {"kernel": "band (persistent) + graph", "bound": "hbm",
-                         "achieved": round(alg_bytes / (kms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(alg_bytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                         "traffic": None, "alg_bytes_per_launch": alg_bytes},
+                       "global_batch": m["n"], "precision": chosen},
+            "roofline": {"kernel": "%s (persistent, hipGraph)" % ("fast_kernel" if chosen == "fast" else "band_kernel"),
+                         "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_launch": m["alg_bytes"]},
             "cpu_baseline": base,
-            "latency_ms": {"p50": round(sorted(lat)[len(lat) // 2], 4), "p99": round(max(lat), 4)},
-            "kernel_ms": round(kms, 4), "out_mpix": round(out_px / 1e6, 3)}
+            "latency_ms": {"p50": round(m["lat"][len(m["lat"]) // 2], 4), "p99": round(m["lat"][-1], 4)},
+            "fast_vs_plain": err,
+            "forms": {f: {"mpix_s": round(v["out_px"] / 1e6 / (v["ms"] / 1e3), 3), "ms_per_step": round(v["ms"], 4),
+                          "kernel_ms": round(v["kernel_ms"], 4)} for f, v in forms.items()},
+            "out_mpix": round(m["out_px"] / 1e6, 3)}
+    print(json.dumps(line), flush=True)
+
+
+def bench_c4(a):
+    """Serving: requests (RGB 8/16-bit, sides log-uniform 256..2048, one scale
+    0.1..1.5, Lanczos3 or Mitchell, 8-bit output with Floyd-Steinberg) through
+    the request loop.  Saturation throughput first (all submitted at once),
+    then Poisson arrivals at 70% of it for p50/p99 latency."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from imageworsener_amd import api
+    import cases
+    lib = api.Library()
+    n = max(a.steps, 8) * 12
+    log("generating %d serving requests" % n)
+    reqs = cases.c4_requests(count=n, lo_side=256, hi_side=2048)
+    prec = api.PRECISION_FAST if a.precision == "fast" else api.PRECISION_PLAIN
+    reqs = [(dict(j, precision=prec), g) for j, g in reqs]
+    out_px = sum(j["canvas"][0] * j["canvas"][1] for j, _ in reqs)
+
+    def serve(rate):
+        srv = api.Server(lib, hbm_budget=32 << 30, streams=8)
+        try:
+            t0 = time.perf_counter()
+            for job, gap in reqs:
+                if rate:
+                    due = t0 + gap / rate
+                    time.sleep(max(0.0, due - time.perf_counter()))
+                    t0 = due
+                srv.submit(job)
+            res = srv.results()
+        finally:
+            srv.close()
+        lat = sorted(r["times_ms"][2] - r["times_ms"][0] for r in res if "times_ms" in r)
+        span = max(r["times_ms"][2] for r in res) - min(r["times_ms"][0] for r in res)
+        return res, lat, span
+
+    log("saturation run (all %d requests at once)" % n)
+    res, lat_sat, span = serve(0.0)
+    sat_rate = n / (span / 1e3)
+    log("saturation: %.1f requests/s, %.1f Mpix/s" % (sat_rate, out_px / 1e6 / (span / 1e3)))
+    rate = 0.7 * sat_rate
+    log("Poisson arrivals at %.1f requests/s" % rate)
+    res2, lat, span2 = serve(rate)
+    ok = all(r["ok"] for r in res) and all(r["ok"] for r in res2)
+    same = all((x["pixels"] == y["pixels"]).all() for x, y in zip(res, res2))
+    line = {"metric": METRIC, "value": round(out_px / 1e6 / (span / 1e3), 3), "unit": "Mpix/s", "n_gpus": 1,
+            "steps": n, "warmup": 0, "ms_per_step": round(span / n, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == api.PRECISION_PLAIN else "f16-mfma/f32",
+            "data": "synthetic",
+            "config": {"workload": "C4 (reduced sides 256..2048): %d requests, RGB 8/16-bit, x0.1..1.5, "
+                                   "lanczos3/mitchell, 8-bit FS; request loop with continuous batching" % n,
+                       "global_batch": n, "precision": "plain" if prec == api.PRECISION_PLAIN else "fast"},
+            "latency_ms": {"rate_per_s": round(rate, 2), "p50": round(lat[len(lat) // 2], 3),
+                           "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3), "max": round(lat[-1], 3)},
+            "saturation": {"requests_per_s": round(sat_rate, 2), "p50": round(lat_sat[len(lat_sat) // 2], 3)},
+            "all_ok": ok, "same_bytes_in_both_runs": bool(same), "out_mpix": round(out_px / 1e6, 3)}
     print(json.dumps(line), flush=True)
 
 
@@ -193,6 +289,8 @@ def main():
         sys.exit(2)
     if a.workload == "c3":
         return bench_c3(a)
+    if a.workload == "c4":
+        return bench_c4(a)
     from imageworsener_amd import api, synth
     lib = api.Library()
     log("generating %dx%d RGBA16 input" % (a.size, a.size))
@@ -213,6 +311,12 @@ def main():
     chosen = "fast" if "fast" in forms else "plain"
     m = forms[chosen]
     value = m["out_pixels"] / 1e6 / (m["ms_per_step"] / 1e3)
+    err = None
+    if "fast" in forms and "plain" in forms:
+        log("error of the fast form against the plain form (with the dither, and without it)")
+        nodither = dict(job, dither=[(api.CH_ALL, 0, 0)])
+        err = {"with_fs": lsb_error(forms["plain"]["result"], forms["fast"]["result"]),
+               "pre_dither": lsb_error(lib.run(nodither), lib.run(dict(nodither, precision=api.PRECISION_FAST)))}
 
     # Roofline of the dominant (longest) kernel, in algorithmic bytes per
     # launch: vpass reads the input image and writes the FP32 intermediate;
@@ -227,17 +331,20 @@ def main():
     out_bytes = a.out * a.out * C
     in_bytes = H_in * W_in * C * 2
     samp_bytes = C * a.out * a.out * 8
-    alg_of = {"vpass": in_bytes + inter_bytes, "hpass": inter_bytes + samp_bytes, "errdiff": samp_bytes + out_bytes}
+    fused = k.get("hpass", 0.0) == 0.0
+    alg_of = {"resample": in_bytes + (samp_bytes if fused else inter_bytes), "hpass": inter_bytes + samp_bytes,
+              "errdiff": samp_bytes + out_bytes}
     alg = alg_of[dom]
     achieved = alg / (k[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(dom) if (a.size, a.out, chosen) == (4096, 1000, "plain") else None
+    traffic = measured_traffic({"resample": "vpass"}.get(dom, dom)) if (
+        (a.size, a.out, chosen) == (4096, 1000, "plain") and not fused) or dom == "errdiff" else None
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "alg_bytes_per_launch": alg,
             "note": "errdiff is a serial chain per plane; step time %.1f ns/pixel" % (
                 k["errdiff"] * 1e6 / (a.out * a.out)) if dom == "errdiff" else ""}
     # The data-parallel part of the same step (input image in, codes out).
-    rs_ms = k.get("vpass", 0.0) + k.get("hpass", 0.0)
-    resample = {"kernels": "vpass+hpass", "ms": round(rs_ms, 4),
+    rs_ms = k.get("resample", 0.0) + k.get("hpass", 0.0)
+    resample = {"kernels": "fused" if fused else "vpass+hpass", "ms": round(rs_ms, 4),
                 "alg_bytes": in_bytes + out_bytes,
                 "achieved_GBs": round((in_bytes + out_bytes) / (rs_ms * 1e-3) / 1e9, 1) if rs_ms > 0 else None}
 
@@ -262,6 +369,7 @@ def main():
         "resample": resample,
         "cpu_baseline": base,
         "latency_ms": {"p50": round(m["p50_ms"], 4), "p99": round(m["p99_ms"], 4)},
+        "fast_vs_plain": err,
         "forms": {f: {"mpix_s": round(v["out_pixels"] / 1e6 / (v["ms_per_step"] / 1e3), 3),
                       "ms_per_step": round(v["ms_per_step"], 4),
                       "kernels_ms": {kk: round(vv, 4) for kk, vv in v["kernels_ms"].items()}}
