@@ -830,6 +830,20 @@ __device__ __forceinline__ double ed_error(const EdTab &t, int k, double s, int 
 	return exact ? 0.0 : s - t.lin[level];
 }
 
+// Level of a wide pixel's sample: the thresholds of its window are compared
+// eight at a time (independent LDS reads), the rest by bisection.
+__device__ __forceinline__ int wide_level(const double *T, int klo, int khi, double s)
+{
+	int lv = klo;
+#pragma unroll
+	for (int q = 0; q < 8; q++) {
+		const int k = min(klo + q, khi - 1);
+		lv += (klo + q < khi && T[k] <= s) ? 1 : 0;
+	}
+	if (lv == klo + 8 && khi > klo + 8) lv = level_in(T, klo + 8, khi, s);
+	return lv;
+}
+
 template <bool HAS_M1>
 __device__ __forceinline__ double ed_step(const EdPix &p, double &e1, double &e2, double m0, double m1)
 {
@@ -914,7 +928,8 @@ __device__ __forceinline__ void ed_finish(EdLds<LDS_ROWS> &S, const EdCtx &c, co
 	if (z >= c.W) return;
 	const int x = (j % 2) ? c.W - 1 - z : z;
 	const double s = S.s[ch % 3][i];
-	const int lv = level_in(c.t.T, S.klo[ch % 3][i], S.khi[ch % 3][i], s);
+	const int klo = S.klo[ch % 3][i], khi = S.khi[ch % 3][i];
+	const int lv = (khi > klo) ? wide_level(c.t.T, klo, khi, s) : klo;
 	int level;
 	c.ERR[x] = ed_error(c.t, lv, s, level);
 	store_sample(P, x, j, c.o, c.t.cd[level]);
@@ -926,22 +941,24 @@ __device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int 
 	const int n = min(ED_CH, c.W - ch * ED_CH);
 	const EdPix *px = S.px[ch % 2];
 	double *so = S.s[ch % 3];
+	const unsigned long long *wm = S.wide[ch % 2];
 	const double m0 = c.m0, m1 = c.m1;
-	int a = 0;
-	while (a < n) {
-		// Run of ordinary pixels up to the next wide one.
-		int stop = n;
+	// First wide pixel at or after a (n: none).
+	auto next_wide = [&](int a) {
 		for (int w = a >> 6; w < (n + 63) >> 6; w++) {
-			unsigned long long mk = S.wide[ch % 2][w];
+			unsigned long long mk = wm[w];
 			if (w == (a >> 6)) mk &= ~0ull << (a & 63);
-			if (mk) {
-				stop = min(n, w * 64 + __ffsll((unsigned long long)mk) - 1);
-				break;
-			}
+			if (mk) return min(n, w * 64 + __ffsll((unsigned long long)mk) - 1);
 		}
-		// Four pixels per turn, operands loaded two pixels ahead
-		// (px has ED_CH + 2 entries, so index stop + 1 is in bounds).
-		EdPix pa = px[a], pb = px[a + 1];
+		return n;
+	};
+	// Runs of ordinary pixels between wide ones, four pixels per turn with
+	// operands loaded two pixels ahead; pa, pb hold px[k], px[k+1] and stay
+	// loaded across a wide pixel.  px has ED_CH + 2 entries, so every index
+	// used (at most n + 1) is in bounds.
+	int a = 0, stop = next_wide(0);
+	EdPix pa = px[0], pb = px[1];
+	for (;;) {
 		int k = a;
 		for (; k + 4 <= stop; k += 4) {
 			const EdPix pc = px[k + 2], pd = px[k + 3];
@@ -952,21 +969,32 @@ __device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int 
 			so[k + 2] = ed_step<HAS_M1>(pc, e1, e2, m0, m1);
 			so[k + 3] = ed_step<HAS_M1>(pd, e1, e2, m0, m1);
 		}
-		if (k < stop) so[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
-		if (k + 1 < stop) so[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
-		if (k + 2 < stop) so[k + 2] = ed_step<HAS_M1>(px[k + 2], e1, e2, m0, m1);
-		if (stop < n) {
-			const EdPix &p = px[stop];
-			const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
-			const double s = fmin(fmax(p.b + cell, 0.0), 1.0);
-			const int lv = level_in(c.t.T, S.klo[ch % 3][stop], S.khi[ch % 3][stop], s);
-			int level;
-			const double err = ed_error(c.t, lv, s, level);
-			so[stop] = s;
-			e2 = e1;
-			e1 = err;
+		const int r = stop - k;
+		if (r >= 1) so[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
+		if (r >= 2) so[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
+		if (r >= 3) so[k + 2] = ed_step<HAS_M1>(px[k + 2], e1, e2, m0, m1);
+		if (r == 1) {
+			pa = pb;
+			pb = px[stop + 1];
+		} else if (r >= 2) {
+			pa = px[stop];
+			pb = px[stop + 1];
 		}
+		if (stop >= n) break;
+		// The wide pixel at stop (operands in pa).
+		const int next = next_wide(stop + 1);
+		const double cell = HAS_M1 ? (pa.e + e2 * m1) + e1 * m0 : pa.e + e1 * m0;
+		const double s = fmin(fmax(pa.b + cell, 0.0), 1.0);
+		const int lv = wide_level(c.t.T, S.klo[ch % 3][stop], S.khi[ch % 3][stop], s);
+		int level;
+		const double err = ed_error(c.t, lv, s, level);
+		so[stop] = s;
+		e2 = e1;
+		e1 = err;
+		pa = pb;
+		pb = px[stop + 2];
 		a = stop + 1;
+		stop = next;
 	}
 }
 
