@@ -27,8 +27,7 @@ ARCH = os.environ.get("IWAMD_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
           "-I" + INCLUDE, "-I" + CSRC, "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value"]
 HIPFLAGS = ["--offload-arch=" + ARCH, "-fno-gpu-flush-denormals-to-zero",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-munsafe-fp-atomics"]
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def hipcc():

@@ -1,22 +1,29 @@
 // GPU executor of the ImageWorsener pixel pipeline for gfx950 (MI355X).
 //
-// One iw_process_image() is three kernels on one HIP stream:
-//   vpass  -- colour-in + premultiply/early background + vertical resample,
-//             rounded to FP32 into the intermediate image
-//             (src/imagew-main.c:831-927)
-//   hpass  -- horizontal resample of every channel of one output pixel
-//             (alpha first), unpremultiply, late background, quantise and
-//             ordered/random dither, store (src/imagew-main.c:929-1155,
-//             669-765); error-diffusion channels stop after the clamp and
-//             hand their linear samples to
-//   errdiff -- one serial walk per error-diffused plane in the reference's
-//             serpentine order (src/imagew-main.c:497-567, 1072-1136).
+// One iw_process_image() resamples, then diffuses errors, on one HIP stream:
+//   band_kernel  -- a band of output rows per workgroup: colour-in +
+//                   premultiply/early background + vertical resample into LDS
+//                   (FP32, as the reference stores it), then the horizontal
+//                   resample and the per-pixel work of every channel (alpha
+//                   first; unpremultiply, late background, quantise, ordered /
+//                   random dither, store) (src/imagew-main.c:831-1155,
+//                   669-765).  Error-diffusion channels stop after the clamp
+//                   and hand their linear samples on.
+//   vpass/hpass  -- the same work as two kernels with the intermediate in HBM,
+//                   for rows too wide for a band in LDS.
+//   fast_kernel  -- the chosen-precision form: both resamples as FP16 MFMA
+//                   band products with FP32 sums, the per-pixel work in FP64.
+//   errdiff      -- one workgroup per error-diffused plane walking the
+//                   reference's serpentine order (src/imagew-main.c:497-567,
+//                   1072-1136).
+// Batches and the request loop run the same kernels over a descriptor table
+// of many plans (persistent band kernels, hipGraph replay).
 //
-// Byte parity with the reference: every product and sum is a separately
-// rounded FP64 operation in the reference's order (contraction is off for
-// this file and the compile line), no libm is called on the device (all
-// transfer functions are host tables), and the intermediate and final alpha
-// are rounded to FP32 where the reference stores them as float.
+// Byte parity of the plain form with the reference: every product and sum is
+// a separately rounded FP64 operation in the reference's order (contraction
+// is off for this file and the compile line), no libm is called on the device
+// (all transfer functions are host tables), and the intermediate and final
+// alpha are rounded to FP32 where the reference stores them as float.
 #include <hip/hip_runtime.h>
 
 #include "iw_ctx.h"
