@@ -70,6 +70,15 @@ def test_sweep_oracle_equals_reference(prod, ref, oracle_exec, name, job):
 A, M = 0xfff0bf23, 0xfff0bf2300000001
 
 
+def test_c3_c4_shapes_oracle_equals_reference(prod, ref, oracle_exec):
+    """Small jobs of the batched and serving configurations (RGBA8 with
+    per-axis factors; RGB 8/16-bit with Floyd-Steinberg)."""
+    jobs = cases.c3_jobs(count=300, max_side=120)
+    jobs += [j for j, _ in cases.c4_requests(count=6, lo_side=64, hi_side=160)]
+    for k, job in enumerate(jobs):
+        cases.assert_same(ref.run(job), prod.run(job, executor=oracle_exec), "shape %d" % k)
+
+
 @pytest.mark.parametrize("seed", [0, 3, 12345, -7, 0x7fffffff])
 def test_prng_jump_ahead_closed_form(oracle, seed):
     """Q_n = a*x_n + c_n + 1 satisfies b*Q_n == Q_(n-1) (mod m), so
