@@ -875,12 +875,13 @@ struct EdCtx {
 };
 
 template <bool LDS_ROWS>
-__device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int ch, int i)
+__device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int ch, int i,
+	bool prefetched = false)
 {
 	const int z = ch * ED_CH + i;
 	if (z >= c.W) return;
 	const int x = (j % 2) ? c.W - 1 - z : z;
-	const double b = c.samp[(size_t)j * c.W + x];
+	const double b = prefetched ? S.b[ch % 2][i] : c.samp[(size_t)j * c.W + x];
 	const double e = c.E[x];
 	const int kg0 = level_in(c.t.T, 0, c.Pq, clamp01(b + ((e - c.G2) - c.G1)));
 	const int kg1 = level_in(c.t.T, kg0, c.Pq, clamp01(b + ((e + c.G2) + c.G1)));
@@ -1089,8 +1090,10 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 			__syncthreads();
 		}
 		if (helper) {
-			ed_stage_a(S, c, j, 0, h);
-			if (nch > 1) ed_stage_a(S, c, j, 1, h);
+			// The samples of these two chunks were loaded during the
+			// previous row's last step.
+			ed_stage_a(S, c, j, 0, h, j > 0);
+			if (nch > 1) ed_stage_a(S, c, j, 1, h, j > 0);
 		}
 		__syncthreads();
 		if (helper) ed_stage_b(S, c, 0, h);
@@ -1104,6 +1107,16 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 				if (ch >= 1) ed_finish(S, c, P, j, ch - 1, h);
 				if (ch + 1 < nch) ed_stage_b(S, c, ch + 1, h);
 				if (ch + 2 < nch) ed_stage_a(S, c, j, ch + 2, h);
+				if (ch == nch - 1 && j + 1 < H) {
+					// No stage reads S.b during a row's last step: load the
+					// next row's first two chunks of samples ahead of the
+					// row boundary.
+					const int jn = j + 1;
+					for (int q = 0; q < 2 && q < nch; q++) {
+						const int z = q * ED_CH + h;
+						if (z < W) S.b[q][h] = c.samp[(size_t)jn * W + ((jn % 2) ? W - 1 - z : z)];
+					}
+				}
 			}
 			__syncthreads();
 		}
