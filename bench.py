@@ -88,19 +88,8 @@ def measure(lib, job, steps, warmup):
 
 
 def lsb_error(plain, fast):
-    """Max / mean |code difference| (output LSBs), the share of samples that
-    differ, and the same in linear light (sRGB codes decoded)."""
-    import numpy as np
-    if not (plain["ok"] and fast["ok"]):
-        return None
-    a = plain["pixels"].astype(np.int32)
-    b = fast["pixels"].astype(np.int32)
-    d = np.abs(a - b)
-    lin = lambda c: np.where(c / 255.0 <= 0.04045, c / 255.0 / 12.92, ((c / 255.0 + 0.055) / 1.055) ** 2.4)
-    dl = np.abs(lin(a.astype(np.float64)) - lin(b.astype(np.float64)))
-    return {"max_lsb": int(d.max()), "mean_lsb": round(float(d.mean()), 5),
-            "frac_differ": round(float((d > 0).mean()), 5),
-            "max_linear": round(float(dl.max()), 6), "mean_linear": round(float(dl.mean()), 7)}
+    from imageworsener_amd import api
+    return api.code_error(plain, fast)
 
 
 def measured_traffic(kernel):
@@ -186,7 +175,8 @@ def bench_c3(a):
     m = forms[chosen]
     err = None
     if "fast" in forms and "plain" in forms:
-        cat = lambda rs: {"ok": all(r["ok"] for r in rs), "pixels": np.concatenate([r["pixels"] for r in rs])}
+        cat = lambda rs: {"ok": all(r["ok"] for r in rs), "imgtype": rs[0]["imgtype"],
+                          "pixels": np.concatenate([r["pixels"] for r in rs])}
         err = lsb_error(cat(forms["plain"]["res"]), cat(forms["fast"]["res"]))
     base = None
     if not a.no_cpu_baseline and "plain" in forms:

@@ -524,6 +524,38 @@ class _PlanLayout:
         return p.out_bpr, p.out_height
 
 
+def code_error(plain, fast):
+    """Difference of two 8-bit results of the same job, in output codes (LSBs)
+    and in linear light (sRGB codes decoded): max, mean, share of samples that
+    differ.  Colour samples of pixels whose alpha is 0 in `plain` are left
+    out: their colour is the quotient of two near-zero sums and carries no
+    visible value."""
+    if not (plain["ok"] and fast["ok"]):
+        return None
+    if (plain["imgtype"], plain["bpr"], plain["height"], plain.get("bit_depth")) != (
+            fast["imgtype"], fast["bpr"], fast["height"], fast.get("bit_depth")):
+        return {"comparable": False}  # the output optimiser chose different formats
+    nch = NCH[plain["imgtype"]]
+    a = plain["pixels"].astype(np.int64).reshape(-1, nch)
+    b = fast["pixels"].astype(np.int64).reshape(-1, nch)
+    d = np.abs(a - b)
+    has_alpha = plain["imgtype"] in (IMGTYPE["RGBA"], IMGTYPE["GRAYA"])
+    masked = 0
+    if has_alpha:
+        clear = a[:, -1] == 0
+        masked = int(clear.sum())
+        d[clear, :-1] = 0
+    lin = lambda c: np.where(c / 255.0 <= 0.04045, c / 255.0 / 12.92, ((c / 255.0 + 0.055) / 1.055) ** 2.4)
+    dl = np.abs(lin(a.astype(np.float64)) - lin(b.astype(np.float64)))
+    if has_alpha:
+        dl[a[:, -1] == 0, :-1] = 0.0
+    return {"max_lsb": int(d.max()) if d.size else 0, "mean_lsb": round(float(d.mean()), 5) if d.size else 0.0,
+            "frac_differ": round(float((d > 0).mean()), 5) if d.size else 0.0,
+            "max_linear": round(float(dl.max()), 6) if dl.size else 0.0,
+            "mean_linear": round(float(dl.mean()), 7) if dl.size else 0.0,
+            "transparent_pixels_left_out": masked}
+
+
 def image_job(pixels, width, height, imgtype, bit_depth, **settings):
     """Convenience: a job dict for an interleaved image buffer."""
     nch = NCH[imgtype]
