@@ -175,9 +175,14 @@ def bench_c3(a):
     m = forms[chosen]
     err = None
     if "fast" in forms and "plain" in forms:
-        cat = lambda rs: {"ok": all(r["ok"] for r in rs), "imgtype": rs[0]["imgtype"],
-                          "pixels": np.concatenate([r["pixels"] for r in rs])}
-        err = lsb_error(cat(forms["plain"]["res"]), cat(forms["fast"]["res"]))
+        per = [lsb_error(x, y) for x, y in zip(forms["plain"]["res"], forms["fast"]["res"])]
+        per = [e for e in per if e and e.get("comparable", True)]
+        w = [r["pixels"].size for r in forms["plain"]["res"]]
+        err = {"images": len(per),
+               "max_lsb": max(e["max_lsb"] for e in per),
+               "mean_lsb": round(sum(e["mean_lsb"] * n for e, n in zip(per, w)) / max(1, sum(w)), 5),
+               "max_linear": max(e["max_linear"] for e in per),
+               "images_over_1_lsb": sum(1 for e in per if e["max_lsb"] > 1)}
     base = None
     if not a.no_cpu_baseline and "plain" in forms:
         ref_path = os.path.join(ROOT, "oracle", "_ref", "libiwref.so")

@@ -44,6 +44,12 @@
 
 namespace {
 
+// Global-memory views of device pointers: accesses through them compile to
+// global (not flat) loads and stores.  Flat accesses share the LDS counter,
+// and with one outstanding every LDS wait in its reach becomes a full drain.
+typedef __attribute__((address_space(1))) unsigned char g_u8;
+typedef const __attribute__((address_space(1))) double g_cf64;
+
 // ---- device-side plan ----------------------------------------------------
 
 struct DevWeights {
@@ -868,11 +874,29 @@ __device__ __forceinline__ double ed_step(const EdPix &p, double &e1, double &e2
 
 struct EdCtx {
 	EdTab t;
-	const double *samp;
+	g_cf64 *samp;
 	double *E, *R1, *ERR;
 	double m0, m1, G1, G2;
 	int W, Pq, o;
+	// Output image, read once from the plan.
+	g_u8 *out;
+	size_t out_bpr;
+	int out_nch, out_depth;
 };
+
+// store_sample() through the context's global view of the output.
+__device__ __forceinline__ void ed_store(const EdCtx &c, int i, int j, double s_full)
+{
+	g_u8 *row = c.out + (size_t)j * c.out_bpr;
+	if (c.out_depth == 8) {
+		row[c.out_nch * i + c.o] = (uint8_t)(0.5 + s_full);
+	} else {
+		const unsigned int v = (unsigned short)(0.5 + s_full);
+		g_u8 *q = row + (size_t)(c.out_nch * i + c.o) * 2;
+		q[0] = (uint8_t)(v >> 8);
+		q[1] = (uint8_t)(v & 0xff);
+	}
+}
 
 template <bool LDS_ROWS>
 __device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int ch, int i,
@@ -940,7 +964,7 @@ __device__ __forceinline__ void ed_finish(EdLds<LDS_ROWS> &S, const EdCtx &c, co
 	const int lv = (khi > klo) ? wide_level(c.t.T, klo, khi, s) : klo;
 	int level;
 	c.ERR[x] = ed_error(c.t, lv, s, level);
-	store_sample(P, x, j, c.o, c.t.cd[level]);
+	ed_store(c, x, j, c.t.cd[level]);
 }
 
 template <bool LDS_ROWS, bool HAS_M1>
@@ -961,12 +985,14 @@ __device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int 
 		return n;
 	};
 	// Runs of ordinary pixels between wide ones, four pixels per turn with
-	// operands loaded two pixels ahead; pa, pb hold px[k], px[k+1] and stay
-	// loaded across a wide pixel.  px has ED_CH + 2 entries, so every index
-	// used (at most n + 1) is in bounds.
-	int a = 0, stop = next_wide(0);
-	EdPix pa = px[0], pb = px[1];
-	for (;;) {
+	// operands loaded two pixels ahead (px has ED_CH + 2 entries, so index
+	// stop + 1 is in bounds).  Each run loads its first operands itself: the
+	// loop's loads then always have the same pending pattern, which keeps
+	// the compiler's LDS waits partial.
+	int a = 0;
+	while (a < n) {
+		const int stop = next_wide(a);
+		EdPix pa = px[a], pb = px[a + 1];
 		int k = a;
 		for (; k + 4 <= stop; k += 4) {
 			const EdPix pc = px[k + 2], pd = px[k + 3];
@@ -977,32 +1003,21 @@ __device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int 
 			so[k + 2] = ed_step<HAS_M1>(pc, e1, e2, m0, m1);
 			so[k + 3] = ed_step<HAS_M1>(pd, e1, e2, m0, m1);
 		}
-		const int r = stop - k;
-		if (r >= 1) so[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
-		if (r >= 2) so[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
-		if (r >= 3) so[k + 2] = ed_step<HAS_M1>(px[k + 2], e1, e2, m0, m1);
-		if (r == 1) {
-			pa = pb;
-			pb = px[stop + 1];
-		} else if (r >= 2) {
-			pa = px[stop];
-			pb = px[stop + 1];
+		if (k < stop) so[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
+		if (k + 1 < stop) so[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
+		if (k + 2 < stop) so[k + 2] = ed_step<HAS_M1>(px[k + 2], e1, e2, m0, m1);
+		if (stop < n) {
+			const EdPix &p = px[stop];
+			const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
+			const double s = fmin(fmax(p.b + cell, 0.0), 1.0);
+			const int lv = wide_level(c.t.T, S.klo[ch % 3][stop], S.khi[ch % 3][stop], s);
+			int level;
+			const double err = ed_error(c.t, lv, s, level);
+			so[stop] = s;
+			e2 = e1;
+			e1 = err;
 		}
-		if (stop >= n) break;
-		// The wide pixel at stop (operands in pa).
-		const int next = next_wide(stop + 1);
-		const double cell = HAS_M1 ? (pa.e + e2 * m1) + e1 * m0 : pa.e + e1 * m0;
-		const double s = fmin(fmax(pa.b + cell, 0.0), 1.0);
-		const int lv = wide_level(c.t.T, S.klo[ch % 3][stop], S.khi[ch % 3][stop], s);
-		int level;
-		const double err = ed_error(c.t, lv, s, level);
-		so[stop] = s;
-		e2 = e1;
-		e1 = err;
-		pa = pb;
-		pb = px[stop + 2];
 		a = stop + 1;
-		stop = next;
 	}
 }
 
@@ -1029,7 +1044,11 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	// Global bounds on the in-row carries, for the first window.
 	c.G1 = q.eb_max * c.m0;
 	c.G2 = q.eb_max * c.m1;
-	c.samp = P.ed_samples + (size_t)q.ed_index * H * W;
+	c.samp = (g_cf64 *)(P.ed_samples + (size_t)q.ed_index * H * W);
+	c.out = (g_u8 *)P.out;
+	c.out_bpr = P.out_bpr;
+	c.out_nch = P.out_nch;
+	c.out_depth = P.out_depth;
 	if (LDS_ROWS) {
 		c.E = S.rows;
 		c.R1 = S.rows + ED_ROWS_LDS;
@@ -1042,13 +1061,13 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	c.t.P = Pq;
 	if (LDS_TAB) {
 		for (int k = tid; k <= Pq; k += ED_THREADS) {
-			if (k < Pq) S.T[k] = q.T[k];
-			S.U[k] = q.U[k];
-			S.M[k] = q.M[k];
-			S.lin[k] = q.lin[k];
-			S.cd[k] = q.code[k];
-			S.pre[k] = q.ebpre[k];
-			S.suf[k] = q.ebsuf[k];
+			if (k < Pq) S.T[k] = ((g_cf64 *)q.T)[k];
+			S.U[k] = ((g_cf64 *)q.U)[k];
+			S.M[k] = ((g_cf64 *)q.M)[k];
+			S.lin[k] = ((g_cf64 *)q.lin)[k];
+			S.cd[k] = ((g_cf64 *)q.code)[k];
+			S.pre[k] = ((g_cf64 *)q.ebpre)[k];
+			S.suf[k] = ((g_cf64 *)q.ebsuf)[k];
 		}
 		c.t.T = S.T; c.t.U = S.U; c.t.M = S.M; c.t.lin = S.lin; c.t.cd = S.cd; c.t.pre = S.pre; c.t.suf = S.suf;
 	} else {
