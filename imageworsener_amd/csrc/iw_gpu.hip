@@ -582,8 +582,8 @@ __global__ void reset_counters_kernel(int *c, int n)
 
 #define FAST_ROWS 16
 #define FAST_MAX_KV 256   // input rows of a band
-#define FAST_MAX_XT 288   // intermediate columns of a tile
-#define FAST_MAX_TI 64    // output columns of a tile
+#define FAST_MAX_XT 160   // intermediate columns of a tile
+#define FAST_MAX_TI 32    // output columns of a tile
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -594,11 +594,27 @@ struct FastItem {
 	int p_lo, kv;         // the band's input rows
 };
 
+// Operands are split: v = hi + lo/2048 with hi = fp16(v) and lo =
+// fp16((v - hi)*2048), so a product is hi*hi + (hi*lo + lo*hi)/2048 to about
+// 22 bits -- three MFMAs, the two cross terms in their own FP32 sum.  (A
+// single FP16 product carries 11 bits, which the division by a small
+// resampled alpha would magnify past one output level.)
 struct FastLds {
-	_Float16 wv[FAST_ROWS][FAST_MAX_KV];
-	_Float16 inter[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_XT];
+	_Float16 wv[2][FAST_ROWS][FAST_MAX_KV];                 // hi, lo
+	_Float16 inter[2][IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_XT];  // hi, lo
 	float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
 };
+
+__device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo)
+{
+	hi = (_Float16)v;
+	lo = (_Float16)((v - (float)hi) * 2048.0f);
+}
+
+__device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c)
+{
+	return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
 
 // Operand layout of v_mfma_f32_16x16x16f16 for lane l: A[l%16][4*(l/16)+t],
 // B[4*(l/16)+t][l%16], D[4*(l/16)+t][l%16], t = 0..3.
@@ -610,8 +626,9 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 	const int kv_pad = (w.kv + 15) & ~15;
 	const DevWeights &dv = P.ic[0].v;  // every channel shares the lists
 	const DevWeights &dh = P.ic[0].h;
+	const float inv = 1.0f / 2048.0f;
 
-	// Vertical weights of the band.
+	// Vertical weights of the band, split.
 	for (int t = tid; t < FAST_ROWS * kv_pad; t += blockDim.x) {
 		const int r = t / kv_pad, k = t % kv_pad;
 		float v = 0.0f;
@@ -620,7 +637,7 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 			const int kk = w.p_lo + k - dv.dfirst[j];
 			if (kk >= 0 && kk < dv.dcount[j]) v = (float)dv.dw[dv.doff[j] + kk];
 		}
-		S.wv[r][k] = (_Float16)v;
+		split16(v, S.wv[0][r][k], S.wv[1][r][k]);
 	}
 	__syncthreads();
 
@@ -630,14 +647,20 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 		const int x = w.x_lo + col;
 		const bool outside = x < 0 || x >= P.W;
 		const int xc = x < 0 ? 0 : (x >= P.W ? P.W - 1 : x);
-		float4_t acc[IWAMD_MAX_CH];
+		float4_t acc[IWAMD_MAX_CH], acl[IWAMD_MAX_CH];
 #pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++) acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+		for (int c = 0; c < IWAMD_MAX_CH; c++) {
+			acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+			acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+		}
 		for (int kk = 0; kk < kv_pad; kk += 16) {
-			half4_t a;
+			half4_t ah, al;
 #pragma unroll
-			for (int t = 0; t < 4; t++) a[t] = S.wv[lr][kk + 4 * lq + t];
-			half4_t b[IWAMD_MAX_CH];
+			for (int t = 0; t < 4; t++) {
+				ah[t] = S.wv[0][lr][kk + 4 * lq + t];
+				al[t] = S.wv[1][lr][kk + 4 * lq + t];
+			}
+			half4_t bh[IWAMD_MAX_CH], bl[IWAMD_MAX_CH];
 #pragma unroll
 			for (int t = 0; t < 4; t++) {
 				const int k = kk + 4 * lq + t;
@@ -645,12 +668,19 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 				for (int c = 0; c < IWAMD_MAX_CH; c++) {
 					float v = 0.0f;
 					if (c < C && k < w.kv && col < w.nx) v = (float)band_sample(P, P.ic[c], xc, w.p_lo + k, P.ih);
-					b[c][t] = (_Float16)v;
+					_Float16 h, l;
+					split16(v, h, l);
+					bh[c][t] = h;
+					bl[c][t] = l;
 				}
 			}
 #pragma unroll
-			for (int c = 0; c < IWAMD_MAX_CH; c++)
-				if (c < C) acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b[c], acc[c], 0, 0, 0);
+			for (int c = 0; c < IWAMD_MAX_CH; c++) {
+				if (c >= C) continue;
+				acc[c] = mfma16(ah, bh[c], acc[c]);
+				acl[c] = mfma16(ah, bl[c], acl[c]);
+				acl[c] = mfma16(al, bh[c], acl[c]);
+			}
 		}
 		if (col < w.nx) {
 #pragma unroll
@@ -658,11 +688,11 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 				if (c >= C) continue;
 #pragma unroll
 				for (int t = 0; t < 4; t++) {
-					float v = acc[c][t];
+					float v = acc[c][t] + acl[c][t] * inv;
 					// A virtual column outside a TRANSPARENT edge is the edge value.
 					if (outside && P.ic[c].h.edge_transparent) v = (float)P.ic[c].h.edge;
 					if (P.intclamp) v = fminf(fmaxf(v, 0.0f), 1.0f);
-					S.inter[c][4 * lq + t][col] = (_Float16)v;
+					split16(v, S.inter[0][c][4 * lq + t][col], S.inter[1][c][4 * lq + t][col]);
 				}
 			}
 		}
@@ -682,12 +712,16 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 			f = min(f, __shfl_xor(f, m));
 			e = max(e, __shfl_xor(e, m));
 		}
-		float4_t acc[IWAMD_MAX_CH];
+		float4_t acc[IWAMD_MAX_CH], acl[IWAMD_MAX_CH];
 #pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++) acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+		for (int c = 0; c < IWAMD_MAX_CH; c++) {
+			acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+			acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+		}
 		if (e >= f) {
 			for (int x0 = f - w.x_lo; x0 <= e - w.x_lo; x0 += 16) {
-				half4_t b;
+				half4_t bh, bl;
+#pragma unroll
 				for (int t = 0; t < 4; t++) {
 					const int xk = x0 + 4 * lq + t;
 					float v = 0.0f;
@@ -695,18 +729,25 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 						const int kk = w.x_lo + xk - dh.dfirst[i];
 						if (kk >= 0 && kk < dh.dcount[i]) v = (float)dh.dw[dh.doff[i] + kk];
 					}
-					b[t] = (_Float16)v;
+					_Float16 h, l;
+					split16(v, h, l);
+					bh[t] = h;
+					bl[t] = l;
 				}
 #pragma unroll
 				for (int c = 0; c < IWAMD_MAX_CH; c++) {
 					if (c >= C) continue;
-					half4_t a;
+					half4_t ah, al;
 #pragma unroll
 					for (int t = 0; t < 4; t++) {
 						const int xk = x0 + 4 * lq + t;
-						a[t] = (xk >= 0 && xk < w.nx) ? S.inter[c][lr][xk] : (_Float16)0.0f;
+						const bool in = xk >= 0 && xk < w.nx;
+						ah[t] = in ? S.inter[0][c][lr][xk] : (_Float16)0.0f;
+						al[t] = in ? S.inter[1][c][lr][xk] : (_Float16)0.0f;
 					}
-					acc[c] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, acc[c], 0, 0, 0);
+					acc[c] = mfma16(ah, bh, acc[c]);
+					acl[c] = mfma16(ah, bl, acl[c]);
+					acl[c] = mfma16(al, bh, acl[c]);
 				}
 			}
 		}
@@ -714,7 +755,7 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 		for (int c = 0; c < IWAMD_MAX_CH; c++)
 			if (c < C)
 #pragma unroll
-				for (int t = 0; t < 4; t++) S.hout[c][4 * lq + t][g * 16 + lr] = acc[c][t];
+				for (int t = 0; t < 4; t++) S.hout[c][4 * lq + t][g * 16 + lr] = acc[c][t] + acl[c][t] * inv;
 	}
 	__syncthreads();
 
@@ -1335,7 +1376,7 @@ bool same_dense(const iwx::DenseWeights &a, const iwx::DenseWeights &b)
 // Tiles of the chosen-precision form, or none where it does not apply: it
 // covers outputs of up to 8 bits whose channels share one weight list per
 // dimension, with band and tile spans that fit its LDS; the widest tile
-// (64, 32 or 16 output columns) that fits is used for the whole image.
+// (32 or 16 output columns) that fits is used for the whole image.
 void plan_fast(const iwamd_plan &p, Staged &S)
 {
 	S.fast.clear();
@@ -1362,7 +1403,7 @@ void plan_fast(const iwamd_plan &p, Staged &S)
 		if (n > FAST_MAX_KV) return;
 	}
 	int ti = 0;
-	for (int cand : {64, 32, 16}) {
+	for (int cand : {32, 16}) {
 		bool fits = true;
 		for (int i0 = 0; i0 < p.out_width && fits; i0 += cand) {
 			int lo, n;
