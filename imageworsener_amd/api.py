@@ -527,9 +527,10 @@ class _PlanLayout:
 def code_error(plain, fast):
     """Difference of two 8-bit results of the same job, in output codes (LSBs)
     and in linear light (sRGB codes decoded): max, mean, share of samples that
-    differ.  Colour samples of pixels whose alpha is 0 in `plain` are left
-    out: their colour is the quotient of two near-zero sums and carries no
-    visible value."""
+    differ.  Colour samples of pixels whose alpha is 0 in either result are
+    left out: the output optimiser blackens fully transparent pixels
+    (src/imagew-opt.c:1147-1200), so a one-level alpha change to 0 clears
+    the colour, and such colour carries no visible value."""
     if not (plain["ok"] and fast["ok"]):
         return None
     if (plain["imgtype"], plain["bpr"], plain["height"], plain.get("bit_depth")) != (
@@ -541,14 +542,15 @@ def code_error(plain, fast):
     d = np.abs(a - b)
     has_alpha = plain["imgtype"] in (IMGTYPE["RGBA"], IMGTYPE["GRAYA"])
     masked = 0
+    clear = None
     if has_alpha:
-        clear = a[:, -1] == 0
+        clear = (a[:, -1] == 0) | (b[:, -1] == 0)
         masked = int(clear.sum())
         d[clear, :-1] = 0
     lin = lambda c: np.where(c / 255.0 <= 0.04045, c / 255.0 / 12.92, ((c / 255.0 + 0.055) / 1.055) ** 2.4)
     dl = np.abs(lin(a.astype(np.float64)) - lin(b.astype(np.float64)))
     if has_alpha:
-        dl[a[:, -1] == 0, :-1] = 0.0
+        dl[clear, :-1] = 0.0
     return {"max_lsb": int(d.max()) if d.size else 0, "mean_lsb": round(float(d.mean()), 5) if d.size else 0.0,
             "frac_differ": round(float((d > 0).mean()), 5) if d.size else 0.0,
             "max_linear": round(float(dl.max()), 6) if dl.size else 0.0,

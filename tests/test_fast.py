@@ -3,8 +3,9 @@ FP32 sums, FP64 quantisation) against the plain FP64 form on the GPU, whose
 bytes equal the reference's.  Bar: at most 1 LSB on 8-bit outputs without
 error diffusion (ordered and random dither included: a decision can move by
 one level); with error diffusion a small difference reroutes the error, so
-only the mean is bounded there.  The colour of a pixel whose alpha is 0 is a
-quotient of two near-zero sums and is not compared."""
+only the mean is bounded there.  The colour of a pixel whose alpha is 0 in
+either result is not compared (the output optimiser blackens fully
+transparent pixels)."""
 import sys
 
 import numpy as np
