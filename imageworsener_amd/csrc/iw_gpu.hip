@@ -2440,15 +2440,24 @@ void server_loop(iwamd_server *V)
 				k++;
 			}
 		}
-		// Admit what fits, as one batch on one stream.
+		// Admit what fits, as one batch on one stream.  A batch completes as
+		// a unit, so it takes requests of like size: the oldest queued
+		// request, and any others whose serial work (output pixels) is
+		// within 4x of it.  The rest wait for the next stream.
 		if (!V->queue.empty() && !V->streams_free.empty()) {
 			iwamd_server::Flight *F = new iwamd_server::Flight();
-			while (!V->queue.empty()) {
-				iwamd_server::Req *R = V->queue.front();
+			const double anchor = std::max(1.0, V->queue.front()->out_pixels);
+			for (auto it = V->queue.begin(); it != V->queue.end();) {
+				iwamd_server::Req *R = *it;
 				const size_t b = req_bytes(*R);
 				// An idle device takes a request even above the budget.
 				if (V->used + b > V->budget && !(V->used == 0 && F->reqs.empty())) break;
-				V->queue.pop_front();
+				const double px = std::max(1.0, R->out_pixels);
+				if (!F->reqs.empty() && (px > 4.0 * anchor || 4.0 * px < anchor)) {
+					++it;
+					continue;
+				}
+				it = V->queue.erase(it);
 				V->used += b;
 				R->t_start = V->now();
 				F->reqs.push_back(R);
