@@ -107,6 +107,13 @@ struct DevPlan {
 	double *ed_samples;
 	double *ed_err;
 	int n_ed;
+	// Per-request completion (request loop only; null otherwise): the last
+	// error-diffusion plane of a request to finish copies the output image to
+	// host memory and raises the flag there.
+	int *ed_done;           // planes finished (device memory, zeroed before the run)
+	uint8_t *host_out;      // device-visible view of pinned, coherent host memory
+	int *host_flag;         // the same, set to 1 when host_out holds the output
+	size_t out_bytes;
 	int apply_bkgd, late, checker, check_size, ox, oy, bk_has_trans;
 	double bk1a, bk2a;
 };
@@ -1182,6 +1189,31 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 		}
 	}
 	if (helper) ed_finish(S, c, P, H - 1, nch - 1, h);
+	if (P.ed_done) {
+		// Completion of the request: the last of its planes to finish copies
+		// the whole output to the host and raises the flag.
+		__shared__ int last;
+		__syncthreads();
+		if (tid == 0) {
+			__threadfence();
+			last = atomicAdd(P.ed_done, 1) == P.n_ed - 1;
+			__threadfence();
+		}
+		__syncthreads();
+		if (last) {
+			const size_t n16 = P.out_bytes / 16;
+			const uint4 *src = (const uint4 *)P.out;
+			uint4 *dst = (uint4 *)P.host_out;
+			for (size_t k = tid; k < n16; k += ED_THREADS) dst[k] = src[k];
+			for (size_t k = n16 * 16 + tid; k < P.out_bytes; k += ED_THREADS) P.host_out[k] = P.out[k];
+			__threadfence_system();
+			__syncthreads();
+			if (tid == 0) {
+				__threadfence_system();
+				__hip_atomic_store(P.host_flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+			}
+		}
+	}
 }
 
 // ---- device memory and stream -------------------------------------------
@@ -1780,8 +1812,10 @@ struct WorkSet {
 	std::vector<int2> planes[8];      // (plan, output channel), by ed_variant
 	std::vector<int> twopass;         // plans resampled by vpass + hpass
 	std::vector<FastItem> fast;       // chosen-precision tiles
-	size_t o_plans = 0, o_bands[3] = {0, 0, 0}, o_planes[8] = {0}, o_counters = 0, o_fast = 0, bytes = 0;
+	size_t o_plans = 0, o_bands[3] = {0, 0, 0}, o_planes[8] = {0}, o_counters = 0, o_fast = 0, o_done = 0;
+	size_t bytes = 0;
 	const FastItem *d_fast = nullptr;
+	int *d_done = nullptr;  // one error-diffusion completion counter per plan
 	const DevPlan *d_plans = nullptr;
 	const int2 *d_bands[3] = {nullptr, nullptr, nullptr};
 	const int2 *d_planes[8] = {nullptr};
@@ -1822,6 +1856,7 @@ void ws_layout(WorkSet &W)
 	for (int v = 0; v < 8; v++) W.o_planes[v] = L.add<int2>(W.planes[v].size());
 	W.o_counters = L.add<int>(4);
 	W.o_fast = L.add<FastItem>(W.fast.size());
+	W.o_done = L.add<int>(W.plans.size());
 	W.bytes = (L.off + 255) & ~(size_t)255;
 }
 
@@ -1846,6 +1881,7 @@ bool ws_upload(WorkSet &W, char *base, hipStream_t st)
 	W.d_counters = (int *)(base + W.o_counters);
 	W.d_fast = (const FastItem *)(base + W.o_fast);
 	up(W.o_fast, W.fast.data(), W.fast.size() * sizeof(FastItem));
+	W.d_done = (int *)(base + W.o_done);
 	return ok;
 }
 
@@ -1868,7 +1904,11 @@ int device_cu_count()
 // otherwise one workgroup per band.
 void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 {
-	if (persistent) hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 4);
+	if (persistent) {
+		hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 4);
+		const int np = (int)W.plans.size();
+		hipLaunchKernelGGL(reset_counters_kernel, dim3((np + 255) / 256), dim3(256), 0, st, W.d_done, np);
+	}
 	if (!W.fast.empty()) {
 		const int n = (int)W.fast.size();
 		const int grid = persistent ? std::min(n, device_cu_count() * 2) : n;
@@ -2310,14 +2350,16 @@ struct iwamd_server {
 		size_t off = 0, out_off = 0;
 		double t_submit = 0, t_start = 0, t_done = 0;
 		double out_pixels = 0;
-		bool done = false, ok = false, skip = false;
+		int flag = -1;  // index of its completion flag in the flight's host memory; -1: none
+		bool done = false, ok = false, skip = false, finished = false;
 	};
 	struct Flight {
 		std::vector<Req *> reqs;
 		WorkSet W;
 		void *block = nullptr;
 		size_t bytes = 0, out_base = 0, out_total = 0;
-		uint8_t *host = nullptr;
+		uint8_t *host = nullptr;  // pinned, coherent: outputs, then one flag per request
+		int *flags = nullptr;
 		hipStream_t st = nullptr;
 		hipEvent_t ev = nullptr;
 		bool failed = false;
@@ -2367,13 +2409,30 @@ void server_launch(iwamd_server::Flight *F)
 	ws_layout(F->W);
 	F->bytes = off + arena + F->W.bytes;
 	F->block = dev_alloc(F->bytes);
-	bool ok = F->block && hipHostMalloc((void **)&F->host, arena + 1, hipHostMallocDefault) == hipSuccess;
+	const size_t host_bytes = arena + sizeof(int) * F->reqs.size() + 16;
+	bool ok = F->block &&
+		hipHostMalloc((void **)&F->host, host_bytes, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess;
+	uint8_t *dhost = nullptr;
+	if (ok) {
+		F->flags = (int *)(F->host + arena);
+		for (size_t k = 0; k < F->reqs.size(); k++) F->flags[k] = 0;
+		ok = hipHostGetDevicePointer((void **)&dhost, F->host, 0) == hipSuccess;
+	}
 	if (ok) {
 		char *base = (char *)F->block;
-		for (auto *R : F->reqs) {
+		for (size_t k = 0; k < F->reqs.size(); k++) {
+			iwamd_server::Req *R = F->reqs[k];
 			DevPlan &D = F->W.plans[R->widx];
 			fill_dev_plan(R->S, R->plan, base + R->off, D);
 			D.out = (uint8_t *)(base + F->out_base + R->out_off);
+			if (R->S.n_ed > 0 && !R->skip) {
+				// The request completes on its own when its last plane ends.
+				D.ed_done = (int *)(base + F->out_base + F->out_total + F->W.o_done) + R->widx;
+				D.host_out = dhost + R->out_off;
+				D.host_flag = (int *)(dhost + arena) + k;
+				D.out_bytes = R->S.out_bytes;
+				R->flag = (int)k;
+			}
 			ok &= upload_job(R->S, R->plan, base + R->off, F->st);
 		}
 		ok &= ws_upload(F->W, base + F->out_base + F->out_total, F->st);
@@ -2381,37 +2440,47 @@ void server_launch(iwamd_server::Flight *F)
 	if (ok) {
 		ws_enqueue_resample(F->W, F->st, true);
 		ws_enqueue_errdiff(F->W, F->st);
-		ok = hipMemcpyAsync(F->host, (char *)F->block + F->out_base, arena, hipMemcpyDeviceToHost, F->st) ==
-			hipSuccess;
+		// Requests without error diffusion complete with the flight.
+		for (auto *R : F->reqs)
+			if (R->flag < 0 && !R->skip)
+				ok &= hipMemcpyAsync(F->host + R->out_off, (char *)F->block + F->out_base + R->out_off,
+					R->S.out_bytes, hipMemcpyDeviceToHost, F->st) == hipSuccess;
 	}
 	F->failed = !ok;
 	hipEventRecord(F->ev, F->st);
 }
 
-// Completes a finished flight's requests; returns its bytes and stream.
+// Completes one request from the flight's host memory.
+void server_finish_req(iwamd_server *V, iwamd_server::Flight *F, iwamd_server::Req *R)
+{
+	bool ok = !F->failed;
+	if (ok) {
+		std::vector<uint8_t> zeros;
+		const uint8_t *img2 = F->host + R->out_off;
+		if (R->skip) {
+			zeros.assign(R->plan.out_bpr * (size_t)R->plan.out_height + 1, 0);
+			img2 = zeros.data();
+		}
+		ok = iwamd_finish(R->ctx, img2) != 0;
+	} else {
+		iw_set_error(R->ctx, "Request could not be run on the GPU");
+	}
+	R->finished = true;
+	const double t = V->now();
+	std::lock_guard<std::mutex> lock(V->mu);
+	R->ok = ok;
+	R->done = true;
+	R->t_done = t;
+	V->cv_done.notify_all();
+}
+
+// Completes a finished flight's remaining requests and frees it.
 void server_complete(iwamd_server *V, iwamd_server::Flight *F)
 {
 	(void)hipEventSynchronize(F->ev);
-	const double t = V->now();
-	std::vector<uint8_t> zeros;
-	for (auto *R : F->reqs) {
-		bool ok = !F->failed;
-		if (ok) {
-			const uint8_t *img2 = F->host + R->out_off;
-			if (R->skip) {
-				zeros.assign(R->plan.out_bpr * (size_t)R->plan.out_height + 1, 0);
-				img2 = zeros.data();
-			}
-			ok = iwamd_finish(R->ctx, img2) != 0;
-		} else {
-			iw_set_error(R->ctx, "Request could not be run on the GPU");
-		}
-		R->S = Staged();
-		std::lock_guard<std::mutex> lock(V->mu);
-		R->ok = ok;
-		R->done = true;
-		R->t_done = t;
-	}
+	for (auto *R : F->reqs)
+		if (!R->finished) server_finish_req(V, F, R);
+	for (auto *R : F->reqs) R->S = Staged();
 	dev_release(F->block, F->bytes);
 	if (F->host) (void)hipHostFree(F->host);
 	hipEventDestroy(F->ev);
@@ -2422,7 +2491,19 @@ void server_loop(iwamd_server *V)
 	(void)hipSetDevice(V->device);
 	std::unique_lock<std::mutex> lock(V->mu);
 	for (;;) {
-		// Completed flights first: they free memory and streams.
+		// Requests whose error diffusion has ended complete at once.
+		for (iwamd_server::Flight *F : V->flights) {
+			if (F->failed) continue;
+			for (auto *R : F->reqs) {
+				if (R->finished || R->flag < 0) continue;
+				if (__atomic_load_n(&F->flags[R->flag], __ATOMIC_ACQUIRE) == 1) {
+					lock.unlock();
+					server_finish_req(V, F, R);
+					lock.lock();
+				}
+			}
+		}
+		// Completed flights: they free memory and streams.
 		for (size_t k = 0; k < V->flights.size();) {
 			iwamd_server::Flight *F = V->flights[k];
 			if (hipEventQuery(F->ev) == hipSuccess) {
@@ -2440,24 +2521,17 @@ void server_loop(iwamd_server *V)
 				k++;
 			}
 		}
-		// Admit what fits, as one batch on one stream.  A batch completes as
-		// a unit, so it takes requests of like size: the oldest queued
-		// request, and any others whose serial work (output pixels) is
-		// within 4x of it.  The rest wait for the next stream.
+		// Admit what fits, as one batch on one stream.  Requests with error
+		// diffusion complete individually (their last plane copies the
+		// output to host memory and raises a flag), so sizes may mix.
 		if (!V->queue.empty() && !V->streams_free.empty()) {
 			iwamd_server::Flight *F = new iwamd_server::Flight();
-			const double anchor = std::max(1.0, V->queue.front()->out_pixels);
-			for (auto it = V->queue.begin(); it != V->queue.end();) {
-				iwamd_server::Req *R = *it;
+			while (!V->queue.empty()) {
+				iwamd_server::Req *R = V->queue.front();
 				const size_t b = req_bytes(*R);
 				// An idle device takes a request even above the budget.
 				if (V->used + b > V->budget && !(V->used == 0 && F->reqs.empty())) break;
-				const double px = std::max(1.0, R->out_pixels);
-				if (!F->reqs.empty() && (px > 4.0 * anchor || 4.0 * px < anchor)) {
-					++it;
-					continue;
-				}
-				it = V->queue.erase(it);
+				V->queue.pop_front();
 				V->used += b;
 				R->t_start = V->now();
 				F->reqs.push_back(R);
