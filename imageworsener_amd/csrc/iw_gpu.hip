@@ -80,6 +80,7 @@ struct DevOChan {
 	const double *T, *U, *lin, *code, *nc;
 	const double *M, *ebpre, *ebsuf;  // error-diffusion decision data (EdTables)
 	double eb_max;
+	int ed_ordered;                    // EdTables::ordered
 	double bk1, bk2;
 	const uint64_t *prng_row, *prng_col;
 };
@@ -846,9 +847,79 @@ struct __align__(16) EdPix {
 	double l1, l2;  // lin[klo+1], lin[klo+2]
 };
 
+// One pixel's operands for the fast walk (weights without an x+2 carry):
+// the reference's decision for every s its window can reach, as one
+// round-up threshold between two bases and one exact interval [sa, sb).
+struct __align__(16) EdFast {
+	double b, e;             // clamped sample; error carried from earlier rows
+	double sm, lo;           // base: s >= sm ? hi : lo
+	double hi;
+	long long nsa;           // -bits(sa)
+	unsigned long long wdt;  // bits(sb) - bits(sa), 0 when the interval is empty
+	double pad;
+};
+
+__host__ __device__ inline long long ed_bits(double d) { return __builtin_bit_cast(long long, d); }
+
+// The decision data for a window [sL, sH] whose levels are klo..khi (the
+// levels of sL and sH).  Samples are >= +0.0, so comparing bit patterns as
+// integers orders them like doubles.  Level k's decision (src/imagew-main.c:
+// 712-748): exact below U[k] (error 0), else the base is lin[k + 1] from
+// M[k] on and lin[k] below it, and the error is s - base.
+//   one level k:  base by M[k]; exact iff s < U[k] (s >= T[k-1] holds);
+//   two levels k, k+1: the window must not reach U[k] (no exact sample of
+//      level k) nor hold both M[k] and M[k+1].  With M[k] <= T[k] <= M[k+1]
+//      ("ordered"), a sample of level k+1 is past M[k] and one of level k is
+//      below M[k+1], so one threshold -- M[k] if the window starts below it,
+//      else M[k+1] -- picks the base, and the exact samples are [T[k],
+//      U[k+1]).
+// False for any other window: the walk then searches the tables.
+__host__ __device__ inline bool ed_fast_form(const double *T, const double *U, const double *M,
+	const double *lin, int P, bool ordered, int klo, int khi, double sL, double sH, EdFast &f)
+{
+	const int k = klo;
+	double sa, sb;
+	if (khi == klo) {
+		f.sm = M[k];
+		f.lo = lin[k];
+		f.hi = lin[k < P ? k + 1 : P];
+		sa = 0.0;
+		sb = U[k];
+	} else if (khi == klo + 1 && ordered && sL >= U[k] && !(sL < M[k] && sH >= M[k + 1])) {
+		if (sL < M[k]) {
+			f.sm = M[k];
+			f.lo = lin[k];
+			f.hi = lin[k + 1];
+		} else {
+			f.sm = M[k + 1];
+			f.lo = lin[k + 1];
+			f.hi = lin[k + 2 <= P ? k + 2 : P];
+		}
+		sa = T[k];
+		sb = U[k + 1];
+	} else {
+		return false;
+	}
+	const long long ba = ed_bits(sa), bb = ed_bits(sb);
+	f.nsa = -ba;
+	f.wdt = bb > ba ? (unsigned long long)(bb - ba) : 0ull;
+	return true;
+}
+
+// The error of sample s under ed_fast_form's data (the walk's step).
+__host__ __device__ inline double ed_fast_err(const EdFast &f, double s)
+{
+	const double base = s >= f.sm ? f.hi : f.lo;
+	const bool exact = (unsigned long long)ed_bits(s) + (unsigned long long)f.nsa < f.wdt;
+	return exact ? 0.0 : s - base;
+}
+
 template <bool LDS_ROWS>
 struct EdLds {
-	EdPix px[2][ED_CH + 2];                 // chunk c in px[c % 2]
+	union {
+		EdPix px[2][ED_CH + 2];             // chunk c in px[c % 2]
+		EdFast fpx[2][ED_CH + 2];           // ... for the fast walk
+	};
 	double s[3][ED_CH];                     // walked samples, chunk c in [c % 3]
 	double eb[3][ED_CH];                    // bound on |error| per pixel
 	double b[2][ED_CH], e[2][ED_CH];        // stage A -> stage B
@@ -925,7 +996,7 @@ struct EdCtx {
 	g_cf64 *samp;
 	double *E, *R1, *ERR;
 	double m0, m1, G1, G2;
-	int W, Pq, o;
+	int W, Pq, o, ordered;
 	// Output image, read once from the plan.
 	g_u8 *out;
 	size_t out_bpr;
@@ -964,8 +1035,11 @@ __device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, i
 	S.kg1[ch % 2][i] = (unsigned short)kg1;
 }
 
-// Run by all 64 lanes of a helper wave (the wide flags are a ballot).
-template <bool LDS_ROWS>
+// Run by all 64 lanes of a helper wave (the wide flags are a ballot).  FAST:
+// the pixel's data for the fast walk (EdFast); a pixel is wide when
+// ed_fast_form has no data for its window.  Else EdPix; wide = three or more
+// levels.
+template <bool LDS_ROWS, bool FAST>
 __device__ __forceinline__ void ed_stage_b(EdLds<LDS_ROWS> &S, const EdCtx &c, int ch, int i)
 {
 	const int z = ch * ED_CH + i;
@@ -978,23 +1052,31 @@ __device__ __forceinline__ void ed_stage_b(EdLds<LDS_ROWS> &S, const EdCtx &c, i
 		const double eb1 = (i >= 1) ? S.eb[ch % 3][i - 1] : (z >= 1 ? S.eb[(ch + 2) % 3][ED_CH - 1] : 0.0);
 		const double eb2 = (i >= 2) ? S.eb[ch % 3][i - 2] : (z >= 2 ? S.eb[(ch + 2) % 3][ED_CH - 2 + i] : 0.0);
 		const double B1 = eb1 * c.m0, B2 = eb2 * c.m1;
-		const int klo = level_in(c.t.T, kg0, kg1, clamp01(b + ((e - B2) - B1)));
-		const int khi = level_in(c.t.T, klo, kg1, clamp01(b + ((e + B2) + B1)));
-		const int k1 = min(klo + 1, c.Pq), k2 = min(klo + 2, c.Pq);
-		EdPix &p = S.px[ch % 2][i];
-		p.b = b;
-		p.e = e;
-		p.t1 = (khi > klo) ? c.t.T[klo] : 2.0;
-		p.u0 = c.t.U[klo];
-		p.m0 = c.t.M[klo];
-		p.u1 = c.t.U[k1];
-		p.m1 = c.t.M[k1];
-		p.l0 = c.t.lin[klo];
-		p.l1 = c.t.lin[k1];
-		p.l2 = c.t.lin[k2];
+		const double sL = clamp01(b + ((e - B2) - B1)), sH = clamp01(b + ((e + B2) + B1));
+		const int klo = level_in(c.t.T, kg0, kg1, sL);
+		const int khi = level_in(c.t.T, klo, kg1, sH);
+		if (FAST) {
+			EdFast &f = S.fpx[ch % 2][i];
+			f.b = b;
+			f.e = e;
+			wide = !ed_fast_form(c.t.T, c.t.U, c.t.M, c.t.lin, c.Pq, c.ordered != 0, klo, khi, sL, sH, f);
+		} else {
+			const int k1 = min(klo + 1, c.Pq), k2 = min(klo + 2, c.Pq);
+			EdPix &p = S.px[ch % 2][i];
+			p.b = b;
+			p.e = e;
+			p.t1 = (khi > klo) ? c.t.T[klo] : 2.0;
+			p.u0 = c.t.U[klo];
+			p.m0 = c.t.M[klo];
+			p.u1 = c.t.U[k1];
+			p.m1 = c.t.M[k1];
+			p.l0 = c.t.lin[klo];
+			p.l1 = c.t.lin[k1];
+			p.l2 = c.t.lin[k2];
+			wide = khi > klo + 1;
+		}
 		S.klo[ch % 3][i] = (unsigned short)klo;
 		S.khi[ch % 3][i] = (unsigned short)khi;
-		wide = khi > klo + 1;
 	}
 	const unsigned long long wm = __ballot(wide);
 	if ((i & 63) == 0) S.wide[ch % 2][i >> 6] = wm;
@@ -1069,6 +1151,149 @@ __device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int 
 	}
 }
 
+__device__ __forceinline__ double lane_f64(double v, int k)
+{
+	const unsigned long long u = __double_as_longlong(v);
+	const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, k);
+	const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), k);
+	return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+
+// One walk step of ed_fast_run (assembler text).
+#define ED_STEP \
+	"v_mul_f64 v[100:101], s[40:41], %[m0]\n\t" \
+	"v_add_f64 v[100:101], %[e], v[100:101]\n\t" \
+	"v_add_f64 %[s], %[b], v[100:101] clamp\n\t" \
+	"v_cmp_ge_f64 vcc, %[s], %[sm]\n\t" \
+	"v_lshl_add_u64 %[q], %[s], 0, %[nsa]\n\t" \
+	"s_nop 0\n\t" \
+	"v_cndmask_b32 v100, v104, v106, vcc\n\t" \
+	"v_cndmask_b32 v101, v105, v107, vcc\n\t" \
+	"v_cmp_gt_u64 %[mask], %[wdt], %[q]\n\t" \
+	"v_add_f64 v[102:103], %[s], -v[100:101]\n\t" \
+	"s_nop 0\n\t" \
+	"v_cndmask_b32 v102, v102, 0, %[mask]\n\t" \
+	"v_cndmask_b32 v103, v103, 0, %[mask]\n\t" \
+	"v_readfirstlane_b32 s40, v102\n\t" \
+	"v_readfirstlane_b32 s41, v103\n\t" \
+	"s_ff1_i32_b64 %[lane], exec\n\t" \
+	"s_bitset0_b64 exec, %[lane]\n\t"
+
+// Steps 2^bit of ed_fast_run when bit `bit` of the run length is set.
+#define ED_STEPS(bit, n, skip) \
+	"s_bitcmp1_b32 %[len], " #bit "\n\t" \
+	"s_cbranch_scc0 " #skip "f\n\t" \
+	".rept " #n "\n\t" ED_STEP ".endr\n\t" \
+	#skip ":\n\t"
+
+// The walk steps of a run of `len` ordinary pixels (1..64), one per lane:
+// exec is the run's lanes, each step walks the first active lane -- its
+// error becomes the carried x (v_readfirstlane) -- and then clears that lane
+// from exec, so the lane keeps its s and error.  The steps are straight-line
+// groups of 64, 32, ..., 1 taken by the bits of len, so exactly len steps
+// run and x ends as the run's last error.  Per step:
+//   s = clamp(b + (e + x*m0)); base = s >= sm ? hi : lo;
+//   err = (bits(s) - bits(sa) < wdt, unsigned) ? 0 : s - base.
+// The products and sums are separately rounded FP64 operations in the
+// reference's order (src/imagew-main.c:702-709); v_add_f64's clamp is the
+// [0,1] clamp (the sum is never NaN or -0.0).  Two wait states separate each
+// VALU write of an SGPR (compare, v_readfirstlane) from its VALU reader.
+// Fixed registers: x in s[40:41]; v[100:101] the base, v[102:103] the error,
+// v[104:105] lo, v[106:107] hi.
+__device__ __forceinline__ void ed_fast_run(const EdFast &f, double m0, unsigned long long run, int len, double &x,
+	double &s, double &err)
+{
+	unsigned long long save, q, mask;
+	unsigned lane;
+	double t;
+	{
+		// x is the same in every lane; say so, for the SGPR operand.
+		const unsigned long long u = __double_as_longlong(x);
+		const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+		const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+		x = __longlong_as_double(((unsigned long long)hi << 32) | lo);
+	}
+	asm volatile(
+		"s_mov_b64 %[save], exec\n\t"
+		"s_and_b64 exec, exec, %[run]\n\t"
+		"s_nop 0\n\t"
+		ED_STEPS(6, 64, 2)
+		ED_STEPS(5, 32, 3)
+		ED_STEPS(4, 16, 4)
+		ED_STEPS(3, 8, 5)
+		ED_STEPS(2, 4, 6)
+		ED_STEPS(1, 2, 7)
+		ED_STEPS(0, 1, 8)
+		"s_mov_b64 exec, %[save]\n\t"
+		"s_nop 1"
+		: [x] "+{s[40:41]}"(x), [s] "+v"(s), "+{v[102:103]}"(err), "=&{v[100:101]}"(t), [save] "=&s"(save),
+		  [q] "=&v"(q), [mask] "=&s"(mask), [lane] "=&s"(lane)
+		: [m0] "v"(m0), [e] "v"(f.e), [b] "v"(f.b), [sm] "v"(f.sm), [nsa] "v"(f.nsa), [wdt] "v"(f.wdt),
+		  "{v[104:105]}"(f.lo), "{v[106:107]}"(f.hi), [run] "s"(run), [len] "s"(len)
+		: "vcc", "scc");
+	(void)t;
+}
+
+#undef ED_STEPS
+#undef ED_STEP
+
+// The walk of chunk ch for weights without an x+2 carry, by all 64 lanes of
+// wave 0 (lane l: pixel 64*blk + l of the chunk).  Runs of ordinary pixels
+// go through ed_fast_run.  A wide pixel between them is stepped by all
+// lanes: they compare its sample with 64 thresholds of its window at a time,
+// and the count of those <= s (thresholds increase) gives its level.
+template <bool LDS_ROWS>
+__device__ __forceinline__ void ed_walk_fast(EdLds<LDS_ROWS> &S, const EdCtx &c, int ch, double &x)
+{
+	const int lane = threadIdx.x;
+	const int n = min(ED_CH, c.W - ch * ED_CH);
+	const EdFast *px = S.fpx[ch % 2];
+	EdFast f = px[lane];
+	for (int blk = 0; blk * 64 < n; blk++) {
+		const int base = blk * 64, nb = min(64, n - base);
+		// Operands of the next block, loaded while this one is walked (px has
+		// ED_CH + 2 entries; base + 64 + lane is in bounds for blk < 2).
+		EdFast fn = f;
+		if (base + 64 < n) fn = px[base + 64 + lane];
+		const unsigned long long wm = S.wide[ch % 2][blk] & (nb == 64 ? ~0ull : ((1ull << nb) - 1));
+		int klo = 0, khi = 0;
+		if (wm) {
+			klo = S.klo[ch % 3][base + lane];
+			khi = S.khi[ch % 3][base + lane];
+		}
+		double s = 0.0, err = 0.0;
+		int a = 0;
+		while (a < nb) {
+			const unsigned long long rest = wm >> a;
+			const int stop = rest ? min(nb, a + __ffsll((unsigned long long)rest) - 1) : nb;
+			if (stop > a) {
+				const int len = stop - a;
+				const unsigned long long run = (len == 64) ? ~0ull : (((1ull << len) - 1) << a);
+				ed_fast_run(f, c.m0, run, len, x, s, err);
+			}
+			if (stop < nb) {
+				const double bw = lane_f64(f.b, stop), ew = lane_f64(f.e, stop);
+				const int klw = __builtin_amdgcn_readlane(klo, stop), khw = __builtin_amdgcn_readlane(khi, stop);
+				const double sk = fmin(fmax(bw + (ew + x * c.m0), 0.0), 1.0);
+				int lv = klw;
+				for (int j0 = klw; j0 < khw; j0 += 64) {
+					const int j = j0 + lane;
+					const int cnt = __popcll(__ballot(j < khw && c.t.T[j] <= sk));
+					lv += cnt;
+					if (cnt < 64) break;
+				}
+				// The reference's decision at level lv (as ed_error).
+				const double u = c.t.U[lv], m = c.t.M[lv], l0 = c.t.lin[lv], l1 = c.t.lin[min(lv + 1, c.Pq)];
+				x = (sk < u) ? 0.0 : sk - ((sk >= m) ? l1 : l0);
+				s = (lane == stop) ? sk : s;
+			}
+			a = stop + 1;
+		}
+		if (lane < nb) S.s[ch % 3][base + lane] = s;
+		f = fn;
+	}
+}
+
 // One workgroup per plane: planes[b] = (plan, output channel).
 template <bool LDS_TAB, bool HAS_M1, bool LDS_ROWS>
 __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__restrict__ plans,
@@ -1089,6 +1314,7 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	c.o = o;
 	c.m0 = m[0];
 	c.m1 = m[1];
+	c.ordered = q.ed_ordered;
 	// Global bounds on the in-row carries, for the first window.
 	c.G1 = q.eb_max * c.m0;
 	c.G2 = q.eb_max * c.m1;
@@ -1163,16 +1389,17 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 			if (nch > 1) ed_stage_a(S, c, j, 1, h, j > 0);
 		}
 		__syncthreads();
-		if (helper) ed_stage_b(S, c, 0, h);
+		if (helper) ed_stage_b<LDS_ROWS, !HAS_M1>(S, c, 0, h);
 		__syncthreads();
 		e1 = 0.0;
 		e2 = 0.0;
 		for (int ch = 0; ch < nch; ch++) {
-			if (tid == 0) {
-				ed_walk<LDS_ROWS, HAS_M1>(S, c, ch, e1, e2);
+			if (HAS_M1 ? tid == 0 : tid < 64) {
+				if constexpr (HAS_M1) ed_walk<LDS_ROWS, HAS_M1>(S, c, ch, e1, e2);
+				else ed_walk_fast(S, c, ch, e1);
 			} else if (helper) {
 				if (ch >= 1) ed_finish(S, c, P, j, ch - 1, h);
-				if (ch + 1 < nch) ed_stage_b(S, c, ch + 1, h);
+				if (ch + 1 < nch) ed_stage_b<LDS_ROWS, !HAS_M1>(S, c, ch + 1, h);
 				if (ch + 2 < nch) ed_stage_a(S, c, j, ch + 2, h);
 				if (ch == nch - 1 && j + 1 < H) {
 					// No stage reads S.b during a row's last step: load the
@@ -1763,6 +1990,7 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 			d.ebpre = (const double *)(base + q.pre);
 			d.ebsuf = (const double *)(base + q.suf);
 			d.eb_max = S.et[o].eb_max;
+			d.ed_ordered = S.et[o].ordered ? 1 : 0;
 		}
 	}
 	D.ed_samples = (double *)(base + S.o_eds);
@@ -2101,6 +2329,25 @@ __global__ void __launch_bounds__(256) copy_probe_kernel(const uint4 *__restrict
 {
 	for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
 		dst[k] = src[k];
+}
+
+// ed_fast_form and ed_fast_err on the host, for tests: the error of each
+// sample s[i] in [sL, sH] under the fast walk's data for that window.
+// Returns 1, 0 when the window needs the table search, -1 for a
+// nearest-colour plan.
+extern "C" int iwamd_ed_fast_check(const struct iwamd_ochan *och, double sL, double sH, int n, const double *s,
+	double *err)
+{
+	if (och->use_nc_table) return -1;
+	const iwx::QuantTable q = iwx::build_quant_table(*och, false);
+	const iwx::EdTables t = iwx::build_ed_tables(q);
+	auto level = [&](double v) { return (int)(std::upper_bound(q.T.begin(), q.T.end(), v) - q.T.begin()); };
+	EdFast f;
+	if (!ed_fast_form(q.T.data(), q.U.data(), t.M.data(), q.lin.data(), q.P, t.ordered, level(sL), level(sH), sL, sH,
+		    f))
+		return 0;
+	for (int i = 0; i < n; i++) err[i] = ed_fast_err(f, s[i]);
+	return 1;
 }
 
 extern "C" int iwamd_copy_probe(size_t bytes, double *ms)

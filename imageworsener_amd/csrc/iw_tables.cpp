@@ -388,6 +388,9 @@ EdTables build_ed_tables(const QuantTable &q)
 	run = 0.0;
 	for (int k = P; k >= 0; k--) t.suf[k] = run = std::max(run, t.eb[k]);
 	t.eb_max = t.pre[P];
+	t.ordered = true;
+	for (int k = 0; k < P; k++)
+		if (!(t.M[k] <= q.T[k] && q.T[k] <= t.M[k + 1])) t.ordered = false;
 	return t;
 }
 

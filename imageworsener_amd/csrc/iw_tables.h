@@ -76,6 +76,9 @@ QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc);
 struct EdTables {
 	std::vector<double> M, eb, pre, suf;
 	double eb_max = 0.0;
+	// M[k] <= T[k] <= M[k+1] for every k < P: a sample past level k's top
+	// rounds up from k, and one below it does not round up from k+1.
+	bool ordered = false;
 };
 
 EdTables build_ed_tables(const QuantTable &q);

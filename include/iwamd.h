@@ -146,6 +146,13 @@ IW_EXPORT(int) iwamd_quant_thresholds(int cstype, double gamma, int P, double *T
  * largest |error| a sample of level k can leave.  Arrays hold P+1 entries;
  * returns P, or -1 for a floating-point or nearest-colour channel. */
 IW_EXPORT(int) iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, double *M, double *eb);
+/* The fast error-diffusion walk's decision for samples s[0..n) of a window
+ * [sL, sH]: err[i] as the walk computes it.  Returns 1, 0 when the window
+ * needs the table search, -1 for a nearest-colour plan.  No counterpart in
+ * the reference: it checks the device's restatement of the decision at
+ * src/imagew-main.c:712-748. */
+IW_EXPORT(int) iwamd_ed_fast_check(const struct iwamd_ochan *och, double sL, double sH, int n, const double *s,
+	double *err);
 
 /* Device-resident jobs.  iwamd_job_create() runs the host decisions on ctx
  * (consuming its single use), builds the device tables and uploads the input

@@ -94,3 +94,68 @@ def test_ed_tables(prod, name, cstype, gamma, maxcc, cc):
             assert abs(err) <= eb[k], (k, s, err, eb[k])
             worst = max(worst, abs(err))
         assert worst == eb[k], (k, worst, eb[k])
+
+
+def _next(d):
+    return struct.unpack("<d", struct.pack("<q", struct.unpack("<q", struct.pack("<d", d))[0] + 1))[0]
+
+
+def _level(s, T):
+    import bisect
+    return bisect.bisect_right(T, s)
+
+
+@pytest.mark.parametrize("name,cstype,gamma,maxcc,cc", CASES, ids=[c[0] for c in CASES])
+def test_ed_fast_form(prod, name, cstype, gamma, maxcc, cc):
+    """The fast walk's decision form (one round-up threshold, one exact
+    interval per window) gives the reference's error for every sample of
+    every window it accepts; windows of one level, and of two levels that
+    stay clear of the lower level's exact samples and of a second round-up
+    threshold, are accepted."""
+    P, T, U, lin, M, eb = _tables(prod, cstype, gamma, maxcc, cc)
+    lay = api._PlanLayout.get()
+    och = lay.OChan()
+    och.ditherfamily, och.dithersubtype = api.DITHER["f"]
+    och.color_count = cc
+    och.maxcolorcode_int = maxcc
+    och.maxcolorcode_dbl = float(maxcc)
+    och.out_cs = api.CsDescr(cstype, 0, gamma)
+    fn = prod.dll.iwamd_ed_fast_check
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    fn.restype = ctypes.c_int
+    ordered = all(M[k] <= T[k] <= M[k + 1] for k in range(P))
+    rng = random.Random(11)
+    levels = range(P + 1) if P < 1000 else list(range(40)) + rng.sample(range(40, P - 40), 100) + list(range(P - 40, P + 1))
+    accepted = windows = 0
+    for k in levels:
+        lo_edge = T[k - 1] if k > 0 else 0.0
+        hi_edge = T[k] if k < P else 1.0
+        span = max(hi_edge - lo_edge, 1e-12)
+        for _ in range(12):
+            # Windows inside level k and windows reaching into level k+1.
+            sL = min(max(lo_edge + span * rng.random(), 0.0), 1.0)
+            sH = min(sL + span * rng.choice([0.05, 0.3, 0.6, 1.2]) * rng.random(), 1.0)
+            pts = [sL, sH] + [sL + (sH - sL) * rng.random() for _ in range(24)]
+            for th in ([T[k]] if k < P else []) + [U[k], M[k]] + ([U[k + 1], M[k + 1]] if k < P else []):
+                for v in (_prev(th), th, _next(th)):
+                    if sL <= v <= sH:
+                        pts.append(v)
+            n = len(pts)
+            s_arr = (ctypes.c_double * n)(*pts)
+            e_arr = (ctypes.c_double * n)()
+            r = fn(ctypes.byref(och), sL, sH, n, s_arr, e_arr)
+            windows += 1
+            kl, kh = _level(sL, T), _level(sH, T)
+            if kl == kh:
+                assert r == 1, (k, sL, sH)
+            elif kh == kl + 1 and ordered and sL >= U[kl] and not (sL < M[kl] and sH >= M[kl + 1]):
+                assert r == 1, (k, sL, sH)
+            else:
+                assert r == 0, (k, sL, sH)
+            if r != 1:
+                continue
+            accepted += 1
+            for s, got in zip(pts, e_arr):
+                want, _ = _ref_error(s, _level(s, T), U, lin)
+                assert got == want, (name, k, sL, sH, s, got, want)
+    assert accepted > windows // 2
