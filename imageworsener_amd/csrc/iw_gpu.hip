@@ -837,6 +837,7 @@ __constant__ double c_ed_matrix[8][12] = {
 #define ED_CH 192          // pixels per chunk = helper threads
 #define ED_LDS_TABLE 256   // largest level count - 1 whose tables live in LDS
 #define ED_ROWS_LDS 4096   // widest row whose error rows live in LDS
+#define ED_QBINS 2048      // bins of the level index
 
 // One pixel's operands for the walk, in the order it consumes them.
 struct __align__(16) EdPix {
@@ -928,11 +929,13 @@ struct EdLds {
 	unsigned short kg0[2][ED_CH], kg1[2][ED_CH];  // ... under the global bound
 	double T[ED_LDS_TABLE], U[ED_LDS_TABLE + 1], M[ED_LDS_TABLE + 1], lin[ED_LDS_TABLE + 1];
 	double cd[ED_LDS_TABLE + 1], pre[ED_LDS_TABLE + 1], suf[ED_LDS_TABLE + 1];
-	double rows[LDS_ROWS ? 3 * ED_ROWS_LDS : 1];  // E, R1, ERR
+	unsigned short Q[ED_QBINS + 1];         // level index (ed_level)
+	double rows[LDS_ROWS ? 2 * ED_ROWS_LDS : 1];  // R1, ERR
 };
 
 struct EdTab {
 	const double *T, *U, *M, *lin, *cd, *pre, *suf;
+	const unsigned short *Q;
 	int P;
 };
 
@@ -950,6 +953,24 @@ __device__ __forceinline__ int level_in(const double *T, int a, int b, double s)
 		}
 	}
 	return a;
+}
+
+// Level of s in [0, 1] (the number of thresholds <= s, as level_in over all
+// of T).  Q[q] is the level of q / ED_QBINS, so a sample of bin q has a level
+// in [Q[q], Q[q+1]]: up to four thresholds are compared at once, more are
+// bisected.
+__device__ __forceinline__ int ed_level(const EdTab &t, double s)
+{
+	const int q = min((int)(s * ED_QBINS), ED_QBINS - 1);  // s * 2^11 is exact
+	const int lo = t.Q[q], hi = t.Q[q + 1];
+	if (hi == lo) return lo;
+	if (hi - lo <= 4) {
+		const double t0 = t.T[lo], t1 = t.T[min(lo + 1, hi - 1)];
+		const double t2 = t.T[min(lo + 2, hi - 1)], t3 = t.T[min(lo + 3, hi - 1)];
+		return lo + (t0 <= s ? 1 : 0) + ((lo + 1 < hi && t1 <= s) ? 1 : 0) + ((lo + 2 < hi && t2 <= s) ? 1 : 0) +
+		       ((lo + 3 < hi && t3 <= s) ? 1 : 0);
+	}
+	return level_in(t.T, lo, hi, s);
 }
 
 // The reference's decision for a sample s whose level is k: exact -> error
@@ -994,7 +1015,8 @@ __device__ __forceinline__ double ed_step(const EdPix &p, double &e1, double &e2
 struct EdCtx {
 	EdTab t;
 	g_cf64 *samp;
-	double *E, *R1, *ERR;
+	const double *m;     // the weights (c_ed_matrix row)
+	double *R1, *ERR;
 	double m0, m1, G1, G2;
 	int W, Pq, o, ordered;
 	// Output image, read once from the plan.
@@ -1018,21 +1040,40 @@ __device__ __forceinline__ void ed_store(const EdCtx &c, int i, int j, double s_
 }
 
 template <bool LDS_ROWS>
-__device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int ch, int i,
+__device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int g, int ch, int i,
 	bool prefetched = false)
 {
 	const int z = ch * ED_CH + i;
 	if (z >= c.W) return;
 	const int x = (j % 2) ? c.W - 1 - z : z;
-	const double b = prefetched ? S.b[ch % 2][i] : c.samp[(size_t)j * c.W + x];
-	const double e = c.E[x];
-	const int kg0 = level_in(c.t.T, 0, c.Pq, clamp01(b + ((e - c.G2) - c.G1)));
-	const int kg1 = level_in(c.t.T, kg0, c.Pq, clamp01(b + ((e + c.G2) + c.G1)));
-	S.eb[ch % 3][i] = fmin(c.t.pre[kg1], c.t.suf[kg0]);
-	S.b[ch % 2][i] = b;
-	S.e[ch % 2][i] = e;
-	S.kg0[ch % 2][i] = (unsigned short)kg0;
-	S.kg1[ch % 2][i] = (unsigned short)kg1;
+	const double b = prefetched ? S.b[g % 2][i] : c.samp[(size_t)j * c.W + x];
+	// The error carried to x from the rows above: what row j-2 left in R1,
+	// then row j-1's sources x-2f, x-f, x, x+f, x+2f in its scan order with
+	// weights m[6..2] -- the reference's per-pixel scatter (src/imagew-main.c:
+	// 497-567) as an ordered gather.  R1[x] then takes row j-1's share for
+	// row j+1 (weights m[11..7]).  Row j's own errors reach ERR only at
+	// chunks already walked, two chunks behind this stage.
+	double e = 0.0;
+	if (j > 0) {
+		const int fwd = ((j - 1) % 2) ? -1 : 1;
+		double a2 = 0.0;
+		e = c.R1[x];
+		for (int d = -2; d <= 2; d++) {
+			const int p = x + d * fwd;
+			if (p < 0 || p >= c.W) continue;
+			const double er = c.ERR[p];
+			e = e + er * c.m[4 - d];
+			a2 = a2 + er * c.m[9 - d];
+		}
+		c.R1[x] = a2;
+	}
+	const int kg0 = ed_level(c.t, clamp01(b + ((e - c.G2) - c.G1)));
+	const int kg1 = max(ed_level(c.t, clamp01(b + ((e + c.G2) + c.G1))), kg0);
+	S.eb[g % 3][i] = fmin(c.t.pre[kg1], c.t.suf[kg0]);
+	S.b[g % 2][i] = b;
+	S.e[g % 2][i] = e;
+	S.kg0[g % 2][i] = (unsigned short)kg0;
+	S.kg1[g % 2][i] = (unsigned short)kg1;
 }
 
 // Run by all 64 lanes of a helper wave (the wide flags are a ballot).  FAST:
@@ -1040,29 +1081,30 @@ __device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, i
 // ed_fast_form has no data for its window.  Else EdPix; wide = three or more
 // levels.
 template <bool LDS_ROWS, bool FAST>
-__device__ __forceinline__ void ed_stage_b(EdLds<LDS_ROWS> &S, const EdCtx &c, int ch, int i)
+__device__ __forceinline__ void ed_stage_b(EdLds<LDS_ROWS> &S, const EdCtx &c, int g, int ch, int i)
 {
 	const int z = ch * ED_CH + i;
 	bool wide = false;
 	if (z < c.W) {
-		const double b = S.b[ch % 2][i], e = S.e[ch % 2][i];
-		const int kg0 = S.kg0[ch % 2][i], kg1 = S.kg1[ch % 2][i];
+		const double b = S.b[g % 2][i], e = S.e[g % 2][i];
+		const int kg0 = S.kg0[g % 2][i], kg1 = S.kg1[g % 2][i];
 		// Bounds of the previous two pixels in scan order (none before the
 		// row's first pixel); chunk ch-1 is full when ch > 0.
-		const double eb1 = (i >= 1) ? S.eb[ch % 3][i - 1] : (z >= 1 ? S.eb[(ch + 2) % 3][ED_CH - 1] : 0.0);
-		const double eb2 = (i >= 2) ? S.eb[ch % 3][i - 2] : (z >= 2 ? S.eb[(ch + 2) % 3][ED_CH - 2 + i] : 0.0);
+		const double eb1 = (i >= 1) ? S.eb[g % 3][i - 1] : (z >= 1 ? S.eb[(g + 2) % 3][ED_CH - 1] : 0.0);
+		const double eb2 = (i >= 2) ? S.eb[g % 3][i - 2] : (z >= 2 ? S.eb[(g + 2) % 3][ED_CH - 2 + i] : 0.0);
 		const double B1 = eb1 * c.m0, B2 = eb2 * c.m1;
 		const double sL = clamp01(b + ((e - B2) - B1)), sH = clamp01(b + ((e + B2) + B1));
-		const int klo = level_in(c.t.T, kg0, kg1, sL);
-		const int khi = level_in(c.t.T, klo, kg1, sH);
+		// level_in(T, a, b, s) is level(s) clamped to [a, b].
+		const int klo = min(max(ed_level(c.t, sL), kg0), kg1);
+		const int khi = min(max(ed_level(c.t, sH), klo), kg1);
 		if (FAST) {
-			EdFast &f = S.fpx[ch % 2][i];
+			EdFast &f = S.fpx[g % 2][i];
 			f.b = b;
 			f.e = e;
 			wide = !ed_fast_form(c.t.T, c.t.U, c.t.M, c.t.lin, c.Pq, c.ordered != 0, klo, khi, sL, sH, f);
 		} else {
 			const int k1 = min(klo + 1, c.Pq), k2 = min(klo + 2, c.Pq);
-			EdPix &p = S.px[ch % 2][i];
+			EdPix &p = S.px[g % 2][i];
 			p.b = b;
 			p.e = e;
 			p.t1 = (khi > klo) ? c.t.T[klo] : 2.0;
@@ -1075,35 +1117,39 @@ __device__ __forceinline__ void ed_stage_b(EdLds<LDS_ROWS> &S, const EdCtx &c, i
 			p.l2 = c.t.lin[k2];
 			wide = khi > klo + 1;
 		}
-		S.klo[ch % 3][i] = (unsigned short)klo;
-		S.khi[ch % 3][i] = (unsigned short)khi;
+		S.klo[g % 3][i] = (unsigned short)klo;
+		S.khi[g % 3][i] = (unsigned short)khi;
 	}
 	const unsigned long long wm = __ballot(wide);
-	if ((i & 63) == 0) S.wide[ch % 2][i >> 6] = wm;
+	if ((i & 63) == 0) S.wide[g % 2][i >> 6] = wm;
 }
 
-template <bool LDS_ROWS>
-__device__ __forceinline__ void ed_finish(EdLds<LDS_ROWS> &S, const EdCtx &c, const DevPlan &P, int j, int ch,
-	int i)
+// Completes pixel i of chunk (row j, chunk ch; slot g): the output code from
+// the walked s, and with WRITE_ERR the error (the fast walk writes it
+// itself).
+template <bool LDS_ROWS, bool WRITE_ERR>
+__device__ __forceinline__ void ed_finish(EdLds<LDS_ROWS> &S, const EdCtx &c, const DevPlan &P, int j, int g,
+	int ch, int i)
 {
 	const int z = ch * ED_CH + i;
 	if (z >= c.W) return;
 	const int x = (j % 2) ? c.W - 1 - z : z;
-	const double s = S.s[ch % 3][i];
-	const int klo = S.klo[ch % 3][i], khi = S.khi[ch % 3][i];
+	const double s = S.s[g % 3][i];
+	const int klo = S.klo[g % 3][i], khi = S.khi[g % 3][i];
 	const int lv = (khi > klo) ? wide_level(c.t.T, klo, khi, s) : klo;
 	int level;
-	c.ERR[x] = ed_error(c.t, lv, s, level);
+	const double err = ed_error(c.t, lv, s, level);
+	if (WRITE_ERR) c.ERR[x] = err;
 	ed_store(c, x, j, c.t.cd[level]);
 }
 
 template <bool LDS_ROWS, bool HAS_M1>
-__device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int ch, double &e1, double &e2)
+__device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int g, int ch, double &e1, double &e2)
 {
 	const int n = min(ED_CH, c.W - ch * ED_CH);
-	const EdPix *px = S.px[ch % 2];
-	double *so = S.s[ch % 3];
-	const unsigned long long *wm = S.wide[ch % 2];
+	const EdPix *px = S.px[g % 2];
+	double *so = S.s[g % 3];
+	const unsigned long long *wm = S.wide[g % 2];
 	const double m0 = c.m0, m1 = c.m1;
 	// First wide pixel at or after a (n: none).
 	auto next_wide = [&](int a) {
@@ -1140,7 +1186,7 @@ __device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int 
 			const EdPix &p = px[stop];
 			const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
 			const double s = fmin(fmax(p.b + cell, 0.0), 1.0);
-			const int lv = wide_level(c.t.T, S.klo[ch % 3][stop], S.khi[ch % 3][stop], s);
+			const int lv = wide_level(c.t.T, S.klo[g % 3][stop], S.khi[g % 3][stop], s);
 			int level;
 			const double err = ed_error(c.t, lv, s, level);
 			so[stop] = s;
@@ -1237,17 +1283,21 @@ __device__ __forceinline__ void ed_fast_run(const EdFast &f, double m0, unsigned
 #undef ED_STEPS
 #undef ED_STEP
 
-// The walk of chunk ch for weights without an x+2 carry, by all 64 lanes of
-// wave 0 (lane l: pixel 64*blk + l of the chunk).  Runs of ordinary pixels
+// The walk of chunk ch of row j (slot g) for weights without an x+2 carry,
+// by all 64 lanes of wave 0 (lane l: pixel 64*blk + l of the chunk).  Runs of ordinary pixels
 // go through ed_fast_run.  A wide pixel between them is stepped by all
 // lanes: they compare its sample with 64 thresholds of its window at a time,
 // and the count of those <= s (thresholds increase) gives its level.
 template <bool LDS_ROWS>
-__device__ __forceinline__ void ed_walk_fast(EdLds<LDS_ROWS> &S, const EdCtx &c, int ch, double &x)
+__device__ __forceinline__ void ed_walk_fast(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int g, int ch, double &x)
 {
+	static_assert(ED_CH == 3 * 64, "three blocks per chunk");
 	const int lane = threadIdx.x;
 	const int n = min(ED_CH, c.W - ch * ED_CH);
-	const EdFast *px = S.fpx[ch % 2];
+	const EdFast *px = S.fpx[g % 2];
+	// The chunk's wide masks, then the first block's operands: LDS returns
+	// in order, so each later wait covers only what it needs.
+	const unsigned long long wide[3] = {S.wide[g % 2][0], S.wide[g % 2][1], S.wide[g % 2][2]};
 	EdFast f = px[lane];
 	for (int blk = 0; blk * 64 < n; blk++) {
 		const int base = blk * 64, nb = min(64, n - base);
@@ -1255,11 +1305,11 @@ __device__ __forceinline__ void ed_walk_fast(EdLds<LDS_ROWS> &S, const EdCtx &c,
 		// ED_CH + 2 entries; base + 64 + lane is in bounds for blk < 2).
 		EdFast fn = f;
 		if (base + 64 < n) fn = px[base + 64 + lane];
-		const unsigned long long wm = S.wide[ch % 2][blk] & (nb == 64 ? ~0ull : ((1ull << nb) - 1));
+		const unsigned long long wm = wide[blk] & (nb == 64 ? ~0ull : ((1ull << nb) - 1));
 		int klo = 0, khi = 0;
 		if (wm) {
-			klo = S.klo[ch % 3][base + lane];
-			khi = S.khi[ch % 3][base + lane];
+			klo = S.klo[g % 3][base + lane];
+			khi = S.khi[g % 3][base + lane];
 		}
 		double s = 0.0, err = 0.0;
 		int a = 0;
@@ -1272,9 +1322,9 @@ __device__ __forceinline__ void ed_walk_fast(EdLds<LDS_ROWS> &S, const EdCtx &c,
 				ed_fast_run(f, c.m0, run, len, x, s, err);
 			}
 			if (stop < nb) {
-				const double bw = lane_f64(f.b, stop), ew = lane_f64(f.e, stop);
+				const double sl = fmin(fmax(f.b + (f.e + x * c.m0), 0.0), 1.0);
+				const double sk = lane_f64(sl, stop);
 				const int klw = __builtin_amdgcn_readlane(klo, stop), khw = __builtin_amdgcn_readlane(khi, stop);
-				const double sk = fmin(fmax(bw + (ew + x * c.m0), 0.0), 1.0);
 				int lv = klw;
 				for (int j0 = klw; j0 < khw; j0 += 64) {
 					const int j = j0 + lane;
@@ -1285,11 +1335,18 @@ __device__ __forceinline__ void ed_walk_fast(EdLds<LDS_ROWS> &S, const EdCtx &c,
 				// The reference's decision at level lv (as ed_error).
 				const double u = c.t.U[lv], m = c.t.M[lv], l0 = c.t.lin[lv], l1 = c.t.lin[min(lv + 1, c.Pq)];
 				x = (sk < u) ? 0.0 : sk - ((sk >= m) ? l1 : l0);
-				s = (lane == stop) ? sk : s;
+				s = (lane == stop) ? sl : s;
+				err = (lane == stop) ? x : err;
 			}
 			a = stop + 1;
 		}
-		if (lane < nb) S.s[ch % 3][base + lane] = s;
+		// Each lane's s for the finish and its error for the rows below
+		// (stage A reads ERR two chunks ahead of the walk).
+		if (lane < nb) {
+			S.s[g % 3][base + lane] = s;
+			const int z = ch * ED_CH + base + lane;
+			c.ERR[(j % 2) ? c.W - 1 - z : z] = err;
+		}
 		f = fn;
 	}
 }
@@ -1323,12 +1380,11 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	c.out_bpr = P.out_bpr;
 	c.out_nch = P.out_nch;
 	c.out_depth = P.out_depth;
+	c.m = m;
 	if (LDS_ROWS) {
-		c.E = S.rows;
-		c.R1 = S.rows + ED_ROWS_LDS;
-		c.ERR = S.rows + 2 * ED_ROWS_LDS;
+		c.R1 = S.rows;
+		c.ERR = S.rows + ED_ROWS_LDS;
 	} else {
-		c.E = P.ed_err + ((size_t)q.ed_index * 4 + 0) * W;
 		c.R1 = P.ed_err + ((size_t)q.ed_index * 4 + 1) * W;
 		c.ERR = P.ed_err + ((size_t)q.ed_index * 4 + 2) * W;
 	}
@@ -1347,10 +1403,10 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	} else {
 		c.t.T = q.T; c.t.U = q.U; c.t.M = q.M; c.t.lin = q.lin; c.t.cd = q.code; c.t.pre = q.ebpre; c.t.suf = q.ebsuf;
 	}
-	for (int x = tid; x < W; x += ED_THREADS) {
-		c.E[x] = 0.0;
-		c.R1[x] = 0.0;
-	}
+	for (int x = tid; x < W; x += ED_THREADS) c.R1[x] = 0.0;
+	__syncthreads();
+	for (int k = tid; k <= ED_QBINS; k += ED_THREADS) S.Q[k] = (unsigned short)level_in(c.t.T, 0, Pq, k / (double)ED_QBINS);
+	c.t.Q = S.Q;
 	__syncthreads();
 
 	const bool helper = tid >= 64;
@@ -1358,64 +1414,54 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	const int nch = (W + ED_CH - 1) / ED_CH;
 	double e1 = 0.0, e2 = 0.0;  // walker: errors of the previous two pixels
 
+	// Chunks are numbered across rows (g = j*nch + ch) and take their
+	// buffers by g, so the pipeline runs through row ends: at a row's start
+	// only stage A of its first chunk, then stage B of it (with stage A of
+	// the second) precede the walk.  The fast walk writes the errors itself,
+	// so the first chunk's stage A needs only the previous row's walk, and
+	// that row's last chunk is completed in the row's first step; the x+2
+	// walk completes it first.
 	for (int j = 0; j < H; j++) {
-		// Row boundary: complete the previous row, carry its errors down,
-		// stage this row's first chunks.
-		if (j > 0) {
-			if (helper) ed_finish(S, c, P, j - 1, nch - 1, h);
-			__syncthreads();
-			const int fwd = ((j - 1) % 2) ? -1 : 1;
-			// Target x receives from sources x-2f, x-f, x, x+f, x+2f (scan
-			// order) with next-row weights m[6], m[5], m[4], m[3], m[2] and
-			// row-after weights m[11..7].
-			for (int x = tid; x < W; x += ED_THREADS) {
-				double a1 = c.R1[x], a2 = 0.0;
-				for (int d = -2; d <= 2; d++) {
-					const int p = x + d * fwd;
-					if (p < 0 || p >= W) continue;
-					const double er = c.ERR[p];
-					a1 = a1 + er * m[4 - d];
-					a2 = a2 + er * m[9 - d];
-				}
-				c.E[x] = a1;
-				c.R1[x] = a2;
-			}
+		const int g0 = j * nch;
+		if (HAS_M1 && j > 0) {
+			// The x+2 walk leaves the errors to the finish.
+			if (helper) ed_finish<LDS_ROWS, true>(S, c, P, j - 1, g0 - 1, nch - 1, h);
 			__syncthreads();
 		}
-		if (helper) {
-			// The samples of these two chunks were loaded during the
-			// previous row's last step.
-			ed_stage_a(S, c, j, 0, h, j > 0);
-			if (nch > 1) ed_stage_a(S, c, j, 1, h, j > 0);
-		}
+		if (helper) ed_stage_a(S, c, j, g0, 0, h, j > 0);
 		__syncthreads();
-		if (helper) ed_stage_b<LDS_ROWS, !HAS_M1>(S, c, 0, h);
+		if (helper) {
+			ed_stage_b<LDS_ROWS, !HAS_M1>(S, c, g0, 0, h);
+			if (nch > 1) ed_stage_a(S, c, j, g0 + 1, 1, h, j > 0);
+		}
 		__syncthreads();
 		e1 = 0.0;
 		e2 = 0.0;
 		for (int ch = 0; ch < nch; ch++) {
+			const int g = g0 + ch;
 			if (HAS_M1 ? tid == 0 : tid < 64) {
-				if constexpr (HAS_M1) ed_walk<LDS_ROWS, HAS_M1>(S, c, ch, e1, e2);
-				else ed_walk_fast(S, c, ch, e1);
+				if constexpr (HAS_M1) ed_walk<LDS_ROWS, HAS_M1>(S, c, g, ch, e1, e2);
+				else ed_walk_fast(S, c, j, g, ch, e1);
 			} else if (helper) {
-				if (ch >= 1) ed_finish(S, c, P, j, ch - 1, h);
-				if (ch + 1 < nch) ed_stage_b<LDS_ROWS, !HAS_M1>(S, c, ch + 1, h);
-				if (ch + 2 < nch) ed_stage_a(S, c, j, ch + 2, h);
+				if (ch >= 1) ed_finish<LDS_ROWS, HAS_M1>(S, c, P, j, g - 1, ch - 1, h);
+				else if (!HAS_M1 && j >= 1) ed_finish<LDS_ROWS, false>(S, c, P, j - 1, g - 1, nch - 1, h);
+				if (ch + 1 < nch) ed_stage_b<LDS_ROWS, !HAS_M1>(S, c, g + 1, ch + 1, h);
+				if (ch + 2 < nch) ed_stage_a(S, c, j, g + 2, ch + 2, h);
 				if (ch == nch - 1 && j + 1 < H) {
 					// No stage reads S.b during a row's last step: load the
-					// next row's first two chunks of samples ahead of the
-					// row boundary.
+					// next row's first two chunks of samples (slots g+1,
+					// g+2) ahead of the row's end.
 					const int jn = j + 1;
 					for (int q = 0; q < 2 && q < nch; q++) {
 						const int z = q * ED_CH + h;
-						if (z < W) S.b[q][h] = c.samp[(size_t)jn * W + ((jn % 2) ? W - 1 - z : z)];
+						if (z < W) S.b[(g + 1 + q) % 2][h] = c.samp[(size_t)jn * W + ((jn % 2) ? W - 1 - z : z)];
 					}
 				}
 			}
 			__syncthreads();
 		}
 	}
-	if (helper) ed_finish(S, c, P, H - 1, nch - 1, h);
+	if (helper) ed_finish<LDS_ROWS, HAS_M1>(S, c, P, H - 1, H * nch - 1, nch - 1, h);
 	if (P.ed_done) {
 		// Completion of the request: the last of its planes to finish copies
 		// the whole output to the host and raises the flag.
