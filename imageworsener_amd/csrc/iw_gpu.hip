@@ -1232,6 +1232,14 @@ __device__ __forceinline__ double lane_f64(double v, int k)
 	".rept " #n "\n\t" ED_STEP ".endr\n\t" \
 	#skip ":\n\t"
 
+// The 64 steps of a full run (len = 64: no lower bit is set).
+#define ED_STEPS64(skip, done) \
+	"s_bitcmp1_b32 %[len], 6\n\t" \
+	"s_cbranch_scc0 " #skip "f\n\t" \
+	".rept 64\n\t" ED_STEP ".endr\n\t" \
+	"s_branch " #done "f\n\t" \
+	#skip ":\n\t"
+
 // The walk steps of a run of `len` ordinary pixels (1..64), one per lane:
 // exec is the run's lanes, each step walks the first active lane -- its
 // error becomes the carried x (v_readfirstlane) -- and then clears that lane
@@ -1263,13 +1271,14 @@ __device__ __forceinline__ void ed_fast_run(const EdFast &f, double m0, unsigned
 		"s_mov_b64 %[save], exec\n\t"
 		"s_and_b64 exec, exec, %[run]\n\t"
 		"s_nop 0\n\t"
-		ED_STEPS(6, 64, 2)
+		ED_STEPS64(2, 9)
 		ED_STEPS(5, 32, 3)
 		ED_STEPS(4, 16, 4)
 		ED_STEPS(3, 8, 5)
 		ED_STEPS(2, 4, 6)
 		ED_STEPS(1, 2, 7)
 		ED_STEPS(0, 1, 8)
+		"9:\n\t"
 		"s_mov_b64 exec, %[save]\n\t"
 		"s_nop 1"
 		: [x] "+{s[40:41]}"(x), [s] "+v"(s), "+{v[102:103]}"(err), "=&{v[100:101]}"(t), [save] "=&s"(save),
@@ -1280,6 +1289,7 @@ __device__ __forceinline__ void ed_fast_run(const EdFast &f, double m0, unsigned
 	(void)t;
 }
 
+#undef ED_STEPS64
 #undef ED_STEPS
 #undef ED_STEP
 
@@ -1306,39 +1316,40 @@ __device__ __forceinline__ void ed_walk_fast(EdLds<LDS_ROWS> &S, const EdCtx &c,
 		EdFast fn = f;
 		if (base + 64 < n) fn = px[base + 64 + lane];
 		const unsigned long long wm = wide[blk] & (nb == 64 ? ~0ull : ((1ull << nb) - 1));
-		int klo = 0, khi = 0;
-		if (wm) {
-			klo = S.klo[g % 3][base + lane];
-			khi = S.khi[g % 3][base + lane];
-		}
 		double s = 0.0, err = 0.0;
-		int a = 0;
-		while (a < nb) {
-			const unsigned long long rest = wm >> a;
-			const int stop = rest ? min(nb, a + __ffsll((unsigned long long)rest) - 1) : nb;
-			if (stop > a) {
-				const int len = stop - a;
-				const unsigned long long run = (len == 64) ? ~0ull : (((1ull << len) - 1) << a);
-				ed_fast_run(f, c.m0, run, len, x, s, err);
-			}
-			if (stop < nb) {
-				const double sl = fmin(fmax(f.b + (f.e + x * c.m0), 0.0), 1.0);
-				const double sk = lane_f64(sl, stop);
-				const int klw = __builtin_amdgcn_readlane(klo, stop), khw = __builtin_amdgcn_readlane(khi, stop);
-				int lv = klw;
-				for (int j0 = klw; j0 < khw; j0 += 64) {
-					const int j = j0 + lane;
-					const int cnt = __popcll(__ballot(j < khw && c.t.T[j] <= sk));
-					lv += cnt;
-					if (cnt < 64) break;
+		if (wm == 0) {
+			// The common block: one run over all its pixels.
+			ed_fast_run(f, c.m0, nb == 64 ? ~0ull : ((1ull << nb) - 1), nb, x, s, err);
+		} else {
+			const int klo = S.klo[g % 3][base + lane], khi = S.khi[g % 3][base + lane];
+			int a = 0;
+			while (a < nb) {
+				const unsigned long long rest = wm >> a;
+				const int stop = rest ? min(nb, a + __ffsll((unsigned long long)rest) - 1) : nb;
+				if (stop > a) {
+					const int len = stop - a;
+					const unsigned long long run = (len == 64) ? ~0ull : (((1ull << len) - 1) << a);
+					ed_fast_run(f, c.m0, run, len, x, s, err);
 				}
-				// The reference's decision at level lv (as ed_error).
-				const double u = c.t.U[lv], m = c.t.M[lv], l0 = c.t.lin[lv], l1 = c.t.lin[min(lv + 1, c.Pq)];
-				x = (sk < u) ? 0.0 : sk - ((sk >= m) ? l1 : l0);
-				s = (lane == stop) ? sl : s;
-				err = (lane == stop) ? x : err;
+				if (stop < nb) {
+					const double sl = fmin(fmax(f.b + (f.e + x * c.m0), 0.0), 1.0);
+					const double sk = lane_f64(sl, stop);
+					const int klw = __builtin_amdgcn_readlane(klo, stop), khw = __builtin_amdgcn_readlane(khi, stop);
+					int lv = klw;
+					for (int k0 = klw; k0 < khw; k0 += 64) {
+						const int k = k0 + lane;
+						const int cnt = __popcll(__ballot(k < khw && c.t.T[k] <= sk));
+						lv += cnt;
+						if (cnt < 64) break;
+					}
+					// The reference's decision at level lv (as ed_error).
+					const double u = c.t.U[lv], m = c.t.M[lv], l0 = c.t.lin[lv], l1 = c.t.lin[min(lv + 1, c.Pq)];
+					x = (sk < u) ? 0.0 : sk - ((sk >= m) ? l1 : l0);
+					s = (lane == stop) ? sl : s;
+					err = (lane == stop) ? x : err;
+				}
+				a = stop + 1;
 			}
-			a = stop + 1;
 		}
 		// Each lane's s for the finish and its error for the rows below
 		// (stage A reads ERR two chunks ahead of the walk).
