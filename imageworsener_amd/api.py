@@ -525,9 +525,9 @@ class _PlanLayout:
 
 
 def code_error(plain, fast):
-    """Difference of two 8-bit results of the same job, in output codes (LSBs)
-    and in linear light (sRGB codes decoded): max, mean, share of samples that
-    differ.  Colour samples of pixels whose alpha is 0 in either result are
+    """Difference of two 8- or 16-bit results of the same job, in output codes
+    (LSBs of the output depth) and in linear light (sRGB codes decoded): max,
+    mean, share of samples that differ.  Colour samples of pixels whose alpha is 0 in either result are
     left out: the output optimiser blackens fully transparent pixels
     (src/imagew-opt.c:1147-1200), so a one-level alpha change to 0 clears
     the colour, and such colour carries no visible value."""
@@ -537,8 +537,14 @@ def code_error(plain, fast):
             fast["imgtype"], fast["bpr"], fast["height"], fast.get("bit_depth")):
         return {"comparable": False}  # the output optimiser chose different formats
     nch = NCH[plain["imgtype"]]
-    a = plain["pixels"].astype(np.int64).reshape(-1, nch)
-    b = fast["pixels"].astype(np.int64).reshape(-1, nch)
+    depth = plain.get("bit_depth", 8)
+    if depth == 16:  # big-endian samples
+        a = np.frombuffer(plain["pixels"].tobytes(), dtype=">u2").astype(np.int64).reshape(-1, nch)
+        b = np.frombuffer(fast["pixels"].tobytes(), dtype=">u2").astype(np.int64).reshape(-1, nch)
+    else:
+        a = plain["pixels"].astype(np.int64).reshape(-1, nch)
+        b = fast["pixels"].astype(np.int64).reshape(-1, nch)
+    maxc = 65535.0 if depth == 16 else 255.0
     d = np.abs(a - b)
     has_alpha = plain["imgtype"] in (IMGTYPE["RGBA"], IMGTYPE["GRAYA"])
     masked = 0
@@ -547,7 +553,7 @@ def code_error(plain, fast):
         clear = (a[:, -1] == 0) | (b[:, -1] == 0)
         masked = int(clear.sum())
         d[clear, :-1] = 0
-    lin = lambda c: np.where(c / 255.0 <= 0.04045, c / 255.0 / 12.92, ((c / 255.0 + 0.055) / 1.055) ** 2.4)
+    lin = lambda c: np.where(c / maxc <= 0.04045, c / maxc / 12.92, ((c / maxc + 0.055) / 1.055) ** 2.4)
     dl = np.abs(lin(a.astype(np.float64)) - lin(b.astype(np.float64)))
     if has_alpha:
         dl[clear, :-1] = 0.0

@@ -1690,13 +1690,13 @@ bool same_dense(const iwx::DenseWeights &a, const iwx::DenseWeights &b)
 }
 
 // Tiles of the chosen-precision form, or none where it does not apply: it
-// covers outputs of up to 8 bits whose channels share one weight list per
-// dimension, with band and tile spans that fit its LDS; the widest tile
+// covers integer outputs (8 or 16 bits) whose channels share one weight list
+// per dimension, with band and tile spans that fit its LDS; the widest tile
 // (32 or 16 output columns) that fits is used for the whole image.
 void plan_fast(const iwamd_plan &p, Staged &S)
 {
 	S.fast.clear();
-	if (p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT || p.out_bit_depth > 8) return;
+	if (p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT || p.out_bit_depth > 16) return;
 	for (int c = 1; c < p.inter_nch; c++) {
 		if (!same_dense(S.dv[c], S.dv[0]) || !same_dense(S.dh[c], S.dh[0])) return;
 		if (S.wv[c].edge_value != S.wv[0].edge_value) return;

@@ -15,8 +15,9 @@
  *
  * 2. Precision mode.  IWAMD_PRECISION_PLAIN (the default) is the FP64 form
  *    whose output bytes equal the reference's.  IWAMD_PRECISION_FAST resamples
- *    on FP16 MFMA (FP32 accumulate) and quantises in FP64; its error against
- *    the plain form is bounded and reported, not zero.
+ *    on FP16 MFMA with split hi/lo operands (FP32 accumulate) and quantises in
+ *    FP64, for 8- and 16-bit integer outputs; its error against the plain form
+ *    is bounded and reported, not zero.
  *
  * 3. Batches and a request loop (iwamd_batch_*, iwamd_server_*).
  */

@@ -65,6 +65,29 @@ def test_fast_sweep(prod, name, job):
     assert mx <= step, (name, mx, mean)
 
 
+def _jobs16():
+    """16-bit outputs: the sweep's 16-bit cases without error diffusion, and
+    RGBA16 -> RGBA16 down and up (premultiplied alpha, sRGB)."""
+    jobs = [(n, j) for n, j in cases.sweep_jobs() if j.get("depth", 8) == 16 and "wide-ed" not in n and not n.endswith("-f")]
+    img = cases.synth.be16(cases.synth.c1(512))
+    jobs.append(("rgba16-down", api.image_job(img, 512, 512, api.IMGTYPE["RGBA"], 16, canvas=(125, 125), depth=16,
+                                              resize=api.resize_both("mitchell"))))
+    jobs.append(("rgba16-up", api.image_job(img, 512, 512, api.IMGTYPE["RGBA"], 16, canvas=(700, 610), depth=16,
+                                            resize=api.resize_both("lanczos3"))))
+    return jobs
+
+
+@pytest.mark.parametrize("name,job", _jobs16(), ids=lambda v: v if isinstance(v, str) else "")
+def test_fast_16bit(prod, name, job):
+    """16-bit outputs: the split operands keep the resampled value within a
+    small fraction of a 16-bit step, so codes differ by at most 1 LSB."""
+    a, b = prod.run(plain(job)), prod.run(fast(job))
+    assert a["bit_depth"] == 16
+    mx, mean = lsb_stats(a, b)
+    say("%s fast vs plain (16-bit): max %d LSB, mean %.5f LSB" % (name, mx, mean))
+    assert mx <= 1 and mean < 0.05, (name, mx, mean)
+
+
 def test_fast_c3_batch(prod):
     jobs = cases.c3_jobs(count=300, max_side=256)
     plain_res = prod.run_batch([plain(j) for j in jobs])
