@@ -107,6 +107,7 @@ _EXT_SIGS = {
     "iwamd_prepare": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
     "iwamd_finish": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
     "iwamd_set_precision": (None, [_c.c_void_p, _c.c_int]),
+    "iwamd_tune_entries": (_c.c_int, [_c.c_void_p, _c.c_int]),
     "iwamd_plan_size": (_c.c_size_t, []),
     "iwamd_device_count": (_c.c_int, []),
     "iwamd_version": (_c.c_char_p, []),
@@ -139,6 +140,12 @@ _EXT_SIGS = {
 
 # Byte offsets into struct iwamd_plan used by executors driven from Python.
 PLAN_OUT_BPR_AND_HEIGHT = None
+
+
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int), ("channels", ctypes.c_int), ("taps_v_log2", ctypes.c_int),
+                ("taps_h_log2", ctypes.c_int), ("width_log2", ctypes.c_int), ("shape", ctypes.c_int),
+                ("n", ctypes.c_int), ("cand", ctypes.c_int * 3), ("ms", ctypes.c_double * 3)]
 
 
 class Library:
@@ -177,6 +184,21 @@ class Library:
             return self._run(ctx, job, executor)
         finally:
             d.iw_destroy_context(ctx)
+
+    def tune_entries(self):
+        """The autotuner's table (iwamd_tune_entries): one dict per tuned
+        class: form ("band" rows per workgroup or "tile" output columns),
+        channels, log2 buckets of taps and width, the chosen shape, and the
+        timed candidates with their resampling ms."""
+        n = self.dll.iwamd_tune_entries(None, 0)
+        buf = (TuneEntry * max(n, 1))()
+        n = self.dll.iwamd_tune_entries(buf, n)
+        out = []
+        for e in buf[:n]:
+            out.append({"form": "band" if e.form == 0 else "tile", "channels": e.channels,
+                        "taps_v_log2": e.taps_v_log2, "taps_h_log2": e.taps_h_log2, "width_log2": e.width_log2,
+                        "shape": e.shape, "candidates": {e.cand[k]: round(e.ms[k], 4) for k in range(e.n)}})
+        return out
 
     def batch(self, jobs):
         """A Batch holding `jobs`, uploaded."""

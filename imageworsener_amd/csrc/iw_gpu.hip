@@ -36,6 +36,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -1646,6 +1647,8 @@ struct Staged {
 	bool in_int = true;
 	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
 	int band_rows = 0;         // rows per band of the fused kernel; 0: two-pass kernels
+	int fast_ti = 0;           // output columns per chosen-precision tile
+	bool shape_forced = false; // band height or tile width set by an option (not tuned)
 	size_t in_bytes = 0, out_bytes = 0;
 	size_t o_in = 0, o_inter = 0, o_out = 0, o_eds = 0, o_ede = 0;
 	std::vector<size_t> o_lut;
@@ -1659,28 +1662,31 @@ size_t band_lds_bytes(int R, int C, int W)
 	return sizeof(double) * (size_t)R * BAND_MAX_K + sizeof(float) * (size_t)C * R * W;
 }
 
-// Largest band height (4, 2 or 1 rows) whose LDS and input-position span fit,
-// or 0.
+// Whether bands of R output rows fit the band kernel: LDS for the rows and
+// input spans of at most BAND_MAX_K positions.
+bool band_fits(const iwamd_plan &p, const std::vector<iwx::DenseWeights> &dv, int R)
+{
+	if (band_lds_bytes(R, p.inter_nch, p.inter_width) > rt().band_lds_max) return false;
+	for (int c = 0; c < p.inter_nch; c++) {
+		const iwx::DenseWeights &d = dv[c];
+		for (int j0 = 0; j0 < p.inter_height; j0 += R) {
+			int lo = 0x7fffffff, hi = -0x7fffffff;
+			for (int r = 0; r < R && j0 + r < p.inter_height; r++) {
+				if (!d.count[j0 + r]) continue;
+				lo = std::min(lo, d.first[j0 + r]);
+				hi = std::max(hi, d.first[j0 + r] + d.count[j0 + r] - 1);
+			}
+			if (hi >= lo && hi - lo + 1 > BAND_MAX_K) return false;
+		}
+	}
+	return true;
+}
+
+// The tallest band that fits (0: none).
 int choose_band_rows(const iwamd_plan &p, const std::vector<iwx::DenseWeights> &dv)
 {
-	const int cands[3] = {4, 2, 1};
-	for (int R : cands) {
-		if (band_lds_bytes(R, p.inter_nch, p.inter_width) > rt().band_lds_max) continue;
-		bool fits = true;
-		for (int c = 0; c < p.inter_nch && fits; c++) {
-			const iwx::DenseWeights &d = dv[c];
-			for (int j0 = 0; j0 < p.inter_height && fits; j0 += R) {
-				int lo = 0x7fffffff, hi = -0x7fffffff;
-				for (int r = 0; r < R && j0 + r < p.inter_height; r++) {
-					if (!d.count[j0 + r]) continue;
-					lo = std::min(lo, d.first[j0 + r]);
-					hi = std::max(hi, d.first[j0 + r] + d.count[j0 + r] - 1);
-				}
-				if (hi >= lo && hi - lo + 1 > BAND_MAX_K) fits = false;
-			}
-		}
-		if (fits) return R;
-	}
+	for (int R : {4, 2, 1})
+		if (band_fits(p, dv, R)) return R;
 	return 0;
 }
 
@@ -1689,11 +1695,41 @@ bool same_dense(const iwx::DenseWeights &a, const iwx::DenseWeights &b)
 	return a.first == b.first && a.count == b.count && a.off == b.off && a.w == b.w;
 }
 
+// Positions [lo, lo + cnt) that rows a..a+n-1 of d reach (cnt 0: none).
+void dense_span(const iwx::DenseWeights &d, int a, int n, int &lo, int &cnt)
+{
+	int l = 0x7fffffff, h = -0x7fffffff;
+	for (int k = a; k < a + n; k++) {
+		if (!d.count[k]) continue;
+		l = std::min(l, d.first[k]);
+		h = std::max(h, d.first[k] + d.count[k] - 1);
+	}
+	if (h < l) {
+		l = 0;
+		h = -1;
+	}
+	lo = l;
+	cnt = h - l + 1;
+}
+
+// Whether tiles of ti output columns span at most FAST_MAX_XT intermediate
+// columns.
+bool fast_tile_fits(const iwamd_plan &p, const iwx::DenseWeights &dh, int ti)
+{
+	for (int i0 = 0; i0 < p.out_width; i0 += ti) {
+		int lo, n;
+		dense_span(dh, i0, std::min(ti, p.out_width - i0), lo, n);
+		if (n > FAST_MAX_XT) return false;
+	}
+	return true;
+}
+
 // Tiles of the chosen-precision form, or none where it does not apply: it
 // covers integer outputs (8 or 16 bits) whose channels share one weight list
-// per dimension, with band and tile spans that fit its LDS; the widest tile
-// (32 or 16 output columns) that fits is used for the whole image.
-void plan_fast(const iwamd_plan &p, Staged &S)
+// per dimension, with band and tile spans that fit its LDS.  want_ti: tile
+// width (32 or 16 output columns) if it fits, else (or 0) the widest that
+// fits, for the whole image.
+void plan_fast(const iwamd_plan &p, Staged &S, int want_ti = 0)
 {
 	S.fast.clear();
 	if (p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT || p.out_bit_depth > 16) return;
@@ -1702,36 +1738,21 @@ void plan_fast(const iwamd_plan &p, Staged &S)
 		if (S.wv[c].edge_value != S.wv[0].edge_value) return;
 	}
 	const iwx::DenseWeights &dv = S.dv[0], &dh = S.dh[0];
-	auto span = [](const iwx::DenseWeights &d, int a, int n, int &lo, int &cnt) {
-		int l = 0x7fffffff, h = -0x7fffffff;
-		for (int k = a; k < a + n; k++) {
-			if (!d.count[k]) continue;
-			l = std::min(l, d.first[k]);
-			h = std::max(h, d.first[k] + d.count[k] - 1);
-		}
-		if (h < l) { l = 0; h = -1; }
-		lo = l;
-		cnt = h - l + 1;
-	};
+	auto span = dense_span;
 	for (int j0 = 0; j0 < p.inter_height; j0 += FAST_ROWS) {
 		int lo, n;
 		span(dv, j0, std::min(FAST_ROWS, p.inter_height - j0), lo, n);
 		if (n > FAST_MAX_KV) return;
 	}
 	int ti = 0;
-	for (int cand : {32, 16}) {
-		bool fits = true;
-		for (int i0 = 0; i0 < p.out_width && fits; i0 += cand) {
-			int lo, n;
-			span(dh, i0, std::min(cand, p.out_width - i0), lo, n);
-			fits = n <= FAST_MAX_XT;
-		}
-		if (fits) {
+	for (int cand : {want_ti, 32, 16}) {
+		if (cand && fast_tile_fits(p, dh, cand)) {
 			ti = cand;
 			break;
 		}
 	}
 	if (!ti) return;
+	S.fast_ti = ti;
 	for (int j0 = 0; j0 < p.inter_height; j0 += FAST_ROWS) {
 		FastItem it;
 		it.job = 0;
@@ -1853,10 +1874,23 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	// two-pass kernels with the intermediate in HBM.
 	const bool force_two = path && !strcmp(path, "twopass");
 	const bool force_band = path && !strcmp(path, "band");
-	if (p.precision == IWAMD_PRECISION_FAST && !S.skip_pixels) plan_fast(p, S);
+	// Options amd:tile (16 or 32) and amd:band_rows (1, 2 or 4) fix the shape
+	// where it fits; otherwise the autotuner may pick it (tune_apply).
+	auto opt_int = [&](const char *name) {
+		const char *v = iw_get_option(ctx, name);
+		return v ? atoi(v) : 0;
+	};
+	const int want_ti = opt_int("amd:tile"), want_rows = opt_int("amd:band_rows");
+	if (p.precision == IWAMD_PRECISION_FAST && !S.skip_pixels)
+		plan_fast(p, S, (want_ti == 16 || want_ti == 32) ? want_ti : 0);
+	if (!S.fast.empty()) S.shape_forced = S.fast_ti == want_ti;
 	if (S.fast.empty()) {
 		S.band_rows = force_two ? 0 : choose_band_rows(p, S.dv);
 		if (!force_band && S.band_rows && p.inter_width > 2048) S.band_rows = 0;
+		if (S.band_rows && (want_rows == 1 || want_rows == 2 || want_rows == 4) && band_fits(p, S.dv, want_rows)) {
+			S.band_rows = want_rows;
+			S.shape_forced = true;
+		}
 	}
 
 	Layout L;
@@ -2071,6 +2105,17 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 // two-pass kernels.
 
 constexpr int kBandR[3] = {1, 2, 4};
+// Bands are also grouped by the LDS they need (<= 16, 32, 64, 128 KiB, more),
+// one persistent launch per group, so a batch's widest rows do not lower the
+// occupancy of its narrow ones.
+constexpr int kLdsClasses = 5;
+
+int lds_class(size_t bytes)
+{
+	int q = 0;
+	while (q < kLdsClasses - 1 && bytes > ((size_t)16384 << q)) q++;
+	return q;
+}
 
 typedef void (*EdKernel)(const DevPlan *, const int2 *);
 const EdKernel kEdKernels[8] = {
@@ -2092,17 +2137,17 @@ int ed_variant(const Staged &S, const iwamd_plan &p, int o)
 
 struct WorkSet {
 	std::vector<DevPlan> plans;       // host copies; the two-pass kernels take them by value
-	std::vector<int2> bands[3];       // (plan, first row), by band height kBandR
-	size_t band_lds[3] = {0, 0, 0};   // largest LDS of a band of each height
+	std::vector<int2> bands[3][kLdsClasses];  // (plan, first row), by band height kBandR and LDS class
+	size_t band_lds[3][kLdsClasses] = {};      // largest LDS of a band in each list
 	std::vector<int2> planes[8];      // (plan, output channel), by ed_variant
 	std::vector<int> twopass;         // plans resampled by vpass + hpass
 	std::vector<FastItem> fast;       // chosen-precision tiles
-	size_t o_plans = 0, o_bands[3] = {0, 0, 0}, o_planes[8] = {0}, o_counters = 0, o_fast = 0, o_done = 0;
+	size_t o_plans = 0, o_bands[3][kLdsClasses] = {}, o_planes[8] = {0}, o_counters = 0, o_fast = 0, o_done = 0;
 	size_t bytes = 0;
 	const FastItem *d_fast = nullptr;
 	int *d_done = nullptr;  // one error-diffusion completion counter per plan
 	const DevPlan *d_plans = nullptr;
-	const int2 *d_bands[3] = {nullptr, nullptr, nullptr};
+	const int2 *d_bands[3][kLdsClasses] = {};
 	const int2 *d_planes[8] = {nullptr};
 	int *d_counters = nullptr;
 };
@@ -2121,8 +2166,10 @@ int ws_add(WorkSet &W, const Staged &S, const iwamd_plan &p)
 		}
 	} else if (S.band_rows) {
 		const int ri = S.band_rows == 4 ? 2 : (S.band_rows == 2 ? 1 : 0);
-		for (int j0 = 0; j0 < p.inter_height; j0 += S.band_rows) W.bands[ri].push_back(make_int2(jx, j0));
-		W.band_lds[ri] = std::max(W.band_lds[ri], band_lds_bytes(S.band_rows, p.inter_nch, p.inter_width));
+		const size_t lds = band_lds_bytes(S.band_rows, p.inter_nch, p.inter_width);
+		const int q = lds_class(lds);
+		for (int j0 = 0; j0 < p.inter_height; j0 += S.band_rows) W.bands[ri][q].push_back(make_int2(jx, j0));
+		W.band_lds[ri][q] = std::max(W.band_lds[ri][q], lds);
 	} else {
 		W.twopass.push_back(jx);
 	}
@@ -2137,9 +2184,10 @@ void ws_layout(WorkSet &W)
 {
 	Layout L;
 	W.o_plans = L.add<DevPlan>(W.plans.size());
-	for (int r = 0; r < 3; r++) W.o_bands[r] = L.add<int2>(W.bands[r].size());
+	for (int r = 0; r < 3; r++)
+		for (int q = 0; q < kLdsClasses; q++) W.o_bands[r][q] = L.add<int2>(W.bands[r][q].size());
 	for (int v = 0; v < 8; v++) W.o_planes[v] = L.add<int2>(W.planes[v].size());
-	W.o_counters = L.add<int>(4);
+	W.o_counters = L.add<int>(3 * kLdsClasses + 1);
 	W.o_fast = L.add<FastItem>(W.fast.size());
 	W.o_done = L.add<int>(W.plans.size());
 	W.bytes = (L.off + 255) & ~(size_t)255;
@@ -2156,8 +2204,10 @@ bool ws_upload(WorkSet &W, char *base, hipStream_t st)
 	W.d_plans = (const DevPlan *)(base + W.o_plans);
 	up(W.o_plans, W.plans.data(), W.plans.size() * sizeof(DevPlan));
 	for (int r = 0; r < 3; r++) {
-		W.d_bands[r] = (const int2 *)(base + W.o_bands[r]);
-		up(W.o_bands[r], W.bands[r].data(), W.bands[r].size() * sizeof(int2));
+		for (int q = 0; q < kLdsClasses; q++) {
+			W.d_bands[r][q] = (const int2 *)(base + W.o_bands[r][q]);
+			up(W.o_bands[r][q], W.bands[r][q].data(), W.bands[r][q].size() * sizeof(int2));
+		}
 	}
 	for (int v = 0; v < 8; v++) {
 		W.d_planes[v] = (const int2 *)(base + W.o_planes[v]);
@@ -2190,7 +2240,7 @@ int device_cu_count()
 void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 {
 	if (persistent) {
-		hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 4);
+		hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 3 * kLdsClasses + 1);
 		const int np = (int)W.plans.size();
 		hipLaunchKernelGGL(reset_counters_kernel, dim3((np + 255) / 256), dim3(256), 0, st, W.d_done, np);
 	}
@@ -2198,24 +2248,26 @@ void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 		const int n = (int)W.fast.size();
 		const int grid = persistent ? std::min(n, device_cu_count() * 2) : n;
 		hipLaunchKernelGGL(fast_kernel, dim3(grid), dim3(256), 0, st, W.d_plans, W.d_fast, n,
-			persistent ? W.d_counters + 3 : nullptr);
+			persistent ? W.d_counters + 3 * kLdsClasses : nullptr);
 	}
-	for (int r = 0; r < 3; r++) {
-		const int n = (int)W.bands[r].size();
-		if (!n) continue;
-		const size_t lds = W.band_lds[r];
-		int grid = n;
-		if (persistent) {
-			const int per_cu = std::max(1, std::min(8, (int)(rt().band_lds_max / std::max<size_t>(lds, 1))));
-			grid = std::min(n, device_cu_count() * per_cu);
+	for (int r = 0; r < 3; r++)
+		for (int q = 0; q < kLdsClasses; q++) {
+			const int n = (int)W.bands[r][q].size();
+			if (!n) continue;
+			const size_t lds = W.band_lds[r][q];
+			const int2 *items = W.d_bands[r][q];
+			int grid = n;
+			if (persistent) {
+				const int per_cu = std::max(1, std::min(8, (int)(rt().band_lds_max / std::max<size_t>(lds, 1))));
+				grid = std::min(n, device_cu_count() * per_cu);
+			}
+			int *ctr = persistent ? W.d_counters + r * kLdsClasses + q : nullptr;
+			switch (kBandR[r]) {
+			case 1: hipLaunchKernelGGL(band_kernel<1>, dim3(grid), dim3(256), lds, st, W.d_plans, items, n, ctr); break;
+			case 2: hipLaunchKernelGGL(band_kernel<2>, dim3(grid), dim3(256), lds, st, W.d_plans, items, n, ctr); break;
+			default: hipLaunchKernelGGL(band_kernel<4>, dim3(grid), dim3(256), lds, st, W.d_plans, items, n, ctr); break;
+			}
 		}
-		int *ctr = persistent ? W.d_counters + r : nullptr;
-		switch (kBandR[r]) {
-		case 1: hipLaunchKernelGGL(band_kernel<1>, dim3(grid), dim3(256), lds, st, W.d_plans, W.d_bands[r], n, ctr); break;
-		case 2: hipLaunchKernelGGL(band_kernel<2>, dim3(grid), dim3(256), lds, st, W.d_plans, W.d_bands[r], n, ctr); break;
-		default: hipLaunchKernelGGL(band_kernel<4>, dim3(grid), dim3(256), lds, st, W.d_plans, W.d_bands[r], n, ctr); break;
-		}
-	}
 	for (int jx : W.twopass) {
 		const DevPlan &D = W.plans[jx];
 		hipLaunchKernelGGL(vpass_kernel, dim3((D.W + 255) / 256, D.Hp), dim3(256), 0, st, D);
@@ -2231,12 +2283,203 @@ void ws_enqueue_errdiff(const WorkSet &W, hipStream_t st)
 	}
 }
 
+// ---- autotuner --------------------------------------------------------------
+//
+// The band height of the band kernel (4, 2 or 1 output rows per workgroup)
+// and the tile width of the chosen-precision form (32 or 16 output columns)
+// change how the work is cut, not the bytes: a band row's accumulator gets
+// its list's terms in list order plus signed zeros for the band's other
+// positions, which leave it unchanged, and a tile's 16-output groups and the
+// 16-column chunks they sum over do not depend on the tile width.  So the
+// shape for a class of jobs -- form, channels, vertical and horizontal taps
+// per output (filter radius x scale factor) and output width, each as
+// ceil(log2) -- is chosen by timing: the first job of a class that may be
+// tuned runs its resampling once per candidate shape on its own device copy,
+// its work repeated to fill the device (a warm run, then the best of three;
+// time per copy: a batch's throughput, not one job's latency), the fastest
+// shape is kept for the class, and later jobs of the class take it.  Batches tune; single jobs
+// and the request loop only look the class up (amd:tune=1 lets a single job
+// tune; IWAMD_TUNE=0 in the environment turns tuning off).
+
+struct TuneKey {
+	int form, nch, tv, th, wb;
+	bool operator<(const TuneKey &o) const
+	{
+		return std::tie(form, nch, tv, th, wb) < std::tie(o.form, o.nch, o.tv, o.th, o.wb);
+	}
+};
+
+struct TuneEntry {
+	int shape = 0, n = 0;
+	int cand[3] = {0, 0, 0};
+	double ms[3] = {0, 0, 0};
+};
+
+struct Tuner {
+	std::mutex mu;
+	std::map<TuneKey, TuneEntry> table;
+};
+
+Tuner &tuner()
+{
+	static Tuner t;
+	return t;
+}
+
+int log2_bucket(double v)
+{
+	int b = 0;
+	while (b < 30 && (double)(1 << b) < v) b++;
+	return b;
+}
+
+// 0: band kernel, 1: chosen-precision tiles, -1: nothing to tune.
+int tune_form(const Staged &S)
+{
+	if (S.skip_pixels) return -1;
+	if (!S.fast.empty()) return 1;
+	return S.band_rows ? 0 : -1;
+}
+
+TuneKey tune_key(const iwamd_plan &p, const Staged &S, int form)
+{
+	auto mean_taps = [](const iwx::DenseWeights &d, int n) {
+		double t = 0.0;
+		for (int k = 0; k < n; k++) t += d.count[k];
+		return n ? t / n : 0.0;
+	};
+	TuneKey k;
+	k.form = form;
+	k.nch = p.inter_nch;
+	k.tv = log2_bucket(mean_taps(S.dv[0], p.inter_height));
+	k.th = log2_bucket(mean_taps(S.dh[0], p.out_width));
+	k.wb = log2_bucket(p.out_width / 64.0);
+	return k;
+}
+
+int tune_cands(const iwamd_plan &p, const Staged &S, int form, int *cand)
+{
+	int n = 0;
+	if (form == 0) {
+		for (int R : {4, 2, 1})
+			if (band_fits(p, S.dv, R)) cand[n++] = R;
+	} else {
+		for (int ti : {32, 16})
+			if (fast_tile_fits(p, S.dh[0], ti)) cand[n++] = ti;
+	}
+	return n;
+}
+
+void tune_set(const iwamd_plan &p, Staged &S, int form, int shape)
+{
+	if (form == 0) S.band_rows = shape;
+	else plan_fast(p, S, shape);
+}
+
+// One job's resampling cost with each candidate shape, on its own device
+// copy, as a batch would see it: the job's work items are repeated until the
+// list holds about four workgroups per CU (the copies write the same bytes to
+// the same output), and the time is divided by the number of copies.
+bool tune_measure(const iwamd_plan &p, Staged &S, int form, const int *cand, int n, double *ms, hipStream_t st)
+{
+	const size_t out_al = (S.out_bytes + 255) & ~(size_t)255;
+	void *blk = dev_alloc(S.bytes + out_al);
+	if (!blk) return false;
+	char *base = (char *)blk;
+	bool ok = upload_job(S, p, base, st);
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+	for (int k = 0; k < n && ok; k++) {
+		tune_set(p, S, form, cand[k]);
+		const int items = form == 0 ? (p.inter_height + S.band_rows - 1) / S.band_rows : (int)S.fast.size();
+		const int copies = std::max(1, std::min(256, 4 * device_cu_count() / std::max(1, items)));
+		WorkSet W;
+		DevPlan D;
+		fill_dev_plan(S, p, base, D);
+		D.out = (uint8_t *)(base + S.bytes);
+		for (int c = 0; c < copies; c++) {
+			ws_add(W, S, p);
+			W.plans.back() = D;
+		}
+		ws_layout(W);
+		void *wb = dev_alloc(W.bytes);
+		if (!wb) {
+			ok = false;
+			break;
+		}
+		ok = ws_upload(W, (char *)wb, st);
+		double best = 1e30;
+		for (int r = 0; r < 4 && ok; r++) {
+			ok = hipEventRecord(e0, st) == hipSuccess;
+			ws_enqueue_resample(W, st, true);
+			ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+			float t = 0.0f;
+			if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess && r > 0) best = std::min(best, (double)t);
+		}
+		ms[k] = best / copies;
+		ok = ok && hipStreamSynchronize(st) == hipSuccess;
+		dev_release(wb, W.bytes);
+	}
+	if (e0) hipEventDestroy(e0);
+	if (e1) hipEventDestroy(e1);
+	(void)hipStreamSynchronize(st);
+	dev_release(blk, S.bytes + out_al);
+	return ok;
+}
+
+bool tune_enabled()
+{
+	const char *v = getenv("IWAMD_TUNE");
+	return !(v && v[0] == '0');
+}
+
+// Gives the plan its class's tuned shape; a class without one is measured on
+// this job first when may_tune.  A shape that does not fit this job (or a
+// failed measurement) keeps the default choice.
+void tune_apply(const iwamd_plan &p, Staged &S, hipStream_t st, bool may_tune)
+{
+	const int form = tune_form(S);
+	if (form < 0 || S.shape_forced) return;
+	int cand[3];
+	const int n = tune_cands(p, S, form, cand);
+	if (n < 2) return;
+	const int orig = form == 0 ? S.band_rows : S.fast_ti;
+	const TuneKey key = tune_key(p, S, form);
+	Tuner &T = tuner();
+	int shape = 0;
+	{
+		std::lock_guard<std::mutex> lock(T.mu);
+		auto it = T.table.find(key);
+		if (it != T.table.end()) shape = it->second.shape;
+	}
+	if (!shape && may_tune && tune_enabled()) {
+		TuneEntry e;
+		if (tune_measure(p, S, form, cand, n, e.ms, st)) {
+			int best = 0;
+			for (int k = 1; k < n; k++)
+				if (e.ms[k] < e.ms[best]) best = k;
+			e.shape = cand[best];
+			e.n = n;
+			for (int k = 0; k < n; k++) e.cand[k] = cand[k];
+			std::lock_guard<std::mutex> lock(T.mu);
+			shape = T.table.emplace(key, e).first->second.shape;
+		}
+	}
+	bool fits = false;
+	for (int k = 0; k < n; k++) fits |= shape && cand[k] == shape;
+	tune_set(p, S, form, fits ? shape : orig);
+}
+
 bool job_upload(iwamd_job *J)
 {
 	iw_context *ctx = J->ctx;
 	const iwamd_plan &p = J->plan;
 	Staged S;
 	if (!stage_job(ctx, p, S, iw_get_option(ctx, "amd:resample"))) return false;
+	{
+		const char *t = iw_get_option(ctx, "amd:tune");
+		tune_apply(p, S, J->stream, t && atoi(t) == 1);
+	}
 	J->skip_pixels = S.skip_pixels;
 	J->n_ed = S.n_ed;
 	for (int e = 0; e < 4; e++) J->ed_list[e] = S.ed_list[e];
@@ -2388,6 +2631,32 @@ __global__ void __launch_bounds__(256) copy_probe_kernel(const uint4 *__restrict
 		dst[k] = src[k];
 }
 
+// The autotuner's table: one entry per tuned class, in key order.
+extern "C" int iwamd_tune_entries(struct iwamd_tune_entry *out, int max)
+{
+	Tuner &T = tuner();
+	std::lock_guard<std::mutex> lock(T.mu);
+	int n = 0;
+	for (const auto &kv : T.table) {
+		if (out && n < max) {
+			iwamd_tune_entry &e = out[n];
+			e.form = kv.first.form;
+			e.channels = kv.first.nch;
+			e.taps_v_log2 = kv.first.tv;
+			e.taps_h_log2 = kv.first.th;
+			e.width_log2 = kv.first.wb + 6;
+			e.shape = kv.second.shape;
+			e.n = kv.second.n;
+			for (int k = 0; k < 3; k++) {
+				e.cand[k] = kv.second.cand[k];
+				e.ms[k] = kv.second.ms[k];
+			}
+		}
+		n++;
+	}
+	return n;
+}
+
 // ed_fast_form and ed_fast_err on the host, for tests: the error of each
 // sample s[i] in [sL, sH] under the fast walk's data for that window.
 // Returns 1, 0 when the window needs the table search, -1 for a
@@ -2503,6 +2772,8 @@ extern "C" int iwamd_batch_upload(struct iwamd_batch *B)
 {
 	if (B->uploaded) return 1;
 	if (B->jobs.empty()) return 0;
+	// Shapes by class (the first job of an untuned class is measured).
+	for (auto *J : B->jobs) tune_apply(J->plan, J->S, B->stream, true);
 	// Layout: [jobs' data][output arena][work lists].
 	size_t off = 0;
 	for (auto *J : B->jobs) {
@@ -2707,6 +2978,7 @@ void server_launch(iwamd_server::Flight *F)
 	for (auto *R : F->reqs) {
 		R->out_off = arena;
 		arena += (R->S.out_bytes + 255) & ~(size_t)255;
+		tune_apply(R->plan, R->S, F->st, false);
 		R->widx = ws_add(F->W, R->S, R->plan);
 	}
 	F->out_total = arena;

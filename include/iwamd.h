@@ -147,6 +147,26 @@ IW_EXPORT(int) iwamd_quant_thresholds(int cstype, double gamma, int P, double *T
  * largest |error| a sample of level k can leave.  Arrays hold P+1 entries;
  * returns P, or -1 for a floating-point or nearest-colour channel. */
 IW_EXPORT(int) iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, double *M, double *eb);
+/* The autotuner's table (no counterpart in the reference).  Band height of
+ * the band kernel and tile width of the chosen-precision form change only
+ * how the work is cut, not the bytes, so each class of jobs -- form,
+ * channels, ceil(log2) of the mean vertical and horizontal taps per output
+ * (filter radius x scale factor) and of the output width -- gets the shape
+ * that ran fastest on its first tuned job.  Batches tune classes they meet;
+ * single jobs (unless the option amd:tune is 1) and the request loop only
+ * look them up; IWAMD_TUNE=0 in the environment turns tuning off; the
+ * options amd:band_rows (1, 2, 4) and amd:tile (16, 32) fix a job's shape.
+ * iwamd_tune_entries() copies up to max entries and returns the count. */
+struct iwamd_tune_entry {
+	int form;          /* 0: band kernel (shape = output rows per band); 1: chosen-precision tiles (shape = output columns) */
+	int channels, taps_v_log2, taps_h_log2, width_log2;
+	int shape;         /* the chosen candidate */
+	int n;             /* candidates timed */
+	int cand[3];
+	double ms[3];      /* resampling time per copy of the tuned job, each candidate (copies fill the device) */
+};
+IW_EXPORT(int) iwamd_tune_entries(struct iwamd_tune_entry *out, int max);
+
 /* The fast error-diffusion walk's decision for samples s[0..n) of a window
  * [sL, sH]: err[i] as the walk computes it.  Returns 1, 0 when the window
  * needs the table search, -1 for a nearest-colour plan.  No counterpart in
