@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "output Mpix/s on 1 GPU, chosen vs FP64 plain at <=1 LSB err; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+MFMA_F16_PEAK_TFLOPS = 2500.0
 # Per-kernel slots of iwamd_job_kernel_ms: the vertical pass (or the whole
 # fused resample: band kernel or the FP16-MFMA kernel), the horizontal pass
 # (0 when fused), error diffusion.
@@ -84,7 +85,20 @@ def measure(lib, job, steps, warmup):
     lat.sort()
     return {"ms_per_step": wall * 1e3 / steps, "kernels_ms": dict(zip(KERNELS, kern_avg)),
             "p50_ms": lat[len(lat) // 2], "p99_ms": lat[min(len(lat) - 1, int(0.99 * len(lat)))],
-            "out_pixels": dj.out_pixels, "alg_bytes": dj.alg_bytes, "result": res}
+            "out_pixels": dj.out_pixels, "alg_bytes": dj.alg_bytes, "flops": dj.flops, "result": res}
+
+
+def mfma_use(flops, ms):
+    """Useful resampling FLOP rate of the chosen form's band contraction
+    against MI355X's dense FP16 matrix peak.  Useful = a multiply and an add
+    per listed tap of both passes; the split hi/lo operands execute three
+    FP16 products per useful one, so the issued rate is three times this."""
+    if not flops or not ms:
+        return None
+    t = flops / (ms * 1e-3) / 1e12
+    return {"useful_tflops": round(t, 3), "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(t / MFMA_F16_PEAK_TFLOPS, 6), "split_products": 3,
+            "issued_frac": round(3 * t / MFMA_F16_PEAK_TFLOPS, 6), "useful_flops_per_launch": flops}
 
 
 def lsb_error(plain, fast):
@@ -132,6 +146,7 @@ def run_batch_timed(lib, jobs, steps, warmup):
     b = lib.batch(jobs)
     try:
         out_px, alg_bytes, n = b.sizes()
+        flops = b.flops()
         for _ in range(warmup):
             b.run()
         b.sync()
@@ -148,7 +163,7 @@ def run_batch_timed(lib, jobs, steps, warmup):
     finally:
         b.close()
     return {"ms": wall * 1e3 / steps, "kernel_ms": sum(kern) / len(kern), "lat": sorted(lat), "out_px": out_px,
-            "alg_bytes": alg_bytes, "n": n, "res": res}
+            "alg_bytes": alg_bytes, "n": n, "flops": flops, "res": res}
 
 
 def bench_c3(a):
@@ -212,6 +227,7 @@ def bench_c3(a):
                          "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_launch": m["alg_bytes"]},
             "cpu_baseline": base,
+            "mfma": mfma_use(forms["fast"]["flops"], forms["fast"]["kernel_ms"]) if "fast" in forms else None,
             "latency_ms": {"p50": round(m["lat"][len(m["lat"]) // 2], 4), "p99": round(m["lat"][-1], 4)},
             "fast_vs_plain": err,
             "forms": {f: {"mpix_s": round(v["out_px"] / 1e6 / (v["ms"] / 1e3), 3), "ms_per_step": round(v["ms"], 4),
@@ -362,6 +378,8 @@ def main():
                    "global_batch": 1, "precision": chosen},
         "roofline": roof,
         "resample": resample,
+        "mfma": mfma_use(forms["fast"]["flops"], forms["fast"]["kernels_ms"].get("resample"))
+                if "fast" in forms else None,
         "cpu_baseline": base,
         "latency_ms": {"p50": round(m["p50_ms"], 4), "p99": round(m["p99_ms"], 4)},
         "fast_vs_plain": err,

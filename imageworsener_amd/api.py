@@ -119,6 +119,8 @@ _EXT_SIGS = {
     "iwamd_job_destroy": (None, [_c.c_void_p]),
     "iwamd_job_kernel_ms": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_double), _c.c_int]),
     "iwamd_job_sizes": (None, [_c.c_void_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "iwamd_job_flops": (_c.c_double, [_c.c_void_p]),
+    "iwamd_batch_flops": (_c.c_double, [_c.c_void_p]),
     "iwamd_batch_create": (_c.c_void_p, []),
     "iwamd_batch_add": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
     "iwamd_batch_upload": (_c.c_int, [_c.c_void_p]),
@@ -380,6 +382,11 @@ class Batch:
         self.lib.dll.iwamd_batch_sizes(self.h, ctypes.byref(px), ctypes.byref(nb), ctypes.byref(n))
         return px.value, nb.value, n.value
 
+    def flops(self):
+        """Useful resampling FLOPs of the uploaded batch (multiply and add
+        per listed tap, both passes, all channels)."""
+        return self.lib.dll.iwamd_batch_flops(self.h)
+
     def finish(self):
         ok = self.lib.dll.iwamd_batch_finish(self.h)
         return [self.lib._collect(ctx, ok and i not in self.failed) for i, ctx in enumerate(self.ctxs)]
@@ -450,6 +457,7 @@ class DeviceJob:
         px, nb = ctypes.c_double(), ctypes.c_double()
         lib.dll.iwamd_job_sizes(handle, ctypes.byref(px), ctypes.byref(nb))
         self.out_pixels, self.alg_bytes = px.value, nb.value
+        self.flops = lib.dll.iwamd_job_flops(handle)
 
     def run(self):
         if not self.lib.dll.iwamd_job_run(self.h):

@@ -50,6 +50,8 @@ namespace {
 // and with one outstanding every LDS wait in its reach becomes a full drain.
 typedef __attribute__((address_space(1))) unsigned char g_u8;
 typedef const __attribute__((address_space(1))) double g_cf64;
+typedef const __attribute__((address_space(1))) unsigned short g_cu16;
+typedef const __attribute__((address_space(1))) int g_cint;
 
 // ---- device-side plan ----------------------------------------------------
 
@@ -79,6 +81,8 @@ enum { Q_THRESH = 0, Q_NC = 1 };
 struct DevOChan {
 	int family, subtype, qmode, P, is_alpha, ed_index;
 	const double *T, *U, *lin, *code, *nc;
+	const unsigned short *ix;          // QuantTable::ix (null: none)
+	int ix_log2;
 	const double *M, *ebpre, *ebsuf;  // error-diffusion decision data (EdTables)
 	double eb_max;
 	int ed_ordered;                    // EdTables::ordered
@@ -272,13 +276,27 @@ __constant__ float c_dither_pattern[2][64] = {
 	 10.5f/64, 28.5f/64, 36.5f/64, 44.5f/64, 42.5f/64, 34.5f/64, 22.5f/64, 4.5f/64,
 	 2.5f/64, 8.5f/64, 16.5f/64, 26.5f/64, 24.5f/64, 14.5f/64, 6.5f/64, 0.5f/64}};
 
-// floor(e(s)): number of thresholds <= s.
-__device__ __forceinline__ int level_floor(const DevOChan &q, double s)
+// Number of entries of the ascending array a[0..n) that are <= s.  With the
+// bin index ix (QuantTable::ix, 2^L bins of [0, 1]) a sample in [0, 1] needs
+// only the entries of its own bin searched: the ix[b] before them lie below
+// the bin's lower edge b/2^L <= s (s*2^L is exact), those from ix[b+1] on at
+// or above its upper edge, which is > s.  Others (no index, s < 0, NaN) search
+// the whole array.
+__device__ __forceinline__ int count_le(const double *a_, int n, const unsigned short *ix_, int L, double s)
 {
-	int lo = 0, n = q.P;
+	const g_cf64 *a = (const g_cf64 *)a_;
+	int lo = 0;
+	if (ix_ && s >= 0.0) {
+		const g_cu16 *ix = (const g_cu16 *)ix_;
+		const int NB = 1 << L;
+		const double f = s * (double)NB;
+		const int b = f >= (double)(NB - 1) ? NB - 1 : (int)f;
+		lo = ix[b];
+		n = ix[b + 1] - lo;
+	}
 	while (n > 0) {
 		const int half = n >> 1;
-		if (q.T[lo + half] <= s) {
+		if (a[lo + half] <= s) {
 			lo += half + 1;
 			n -= half + 1;
 		} else {
@@ -288,20 +306,16 @@ __device__ __forceinline__ int level_floor(const DevOChan &q, double s)
 	return lo;
 }
 
-// Index of the first nearest-colour midpoint above s (0..255).
-__device__ __forceinline__ int nc_search(const double *nc, double s)
+// floor(e(s)): number of thresholds <= s.
+__device__ __forceinline__ int level_floor(const DevOChan &q, double s)
 {
-	int lo = 0, n = 255;
-	while (n > 0) {
-		const int half = n >> 1;
-		if (nc[lo + half] <= s) {
-			lo += half + 1;
-			n -= half + 1;
-		} else {
-			n = half;
-		}
-	}
-	return lo;
+	return count_le(q.T, q.P, q.qmode == Q_THRESH ? q.ix : nullptr, q.ix_log2, s);
+}
+
+// Index of the first nearest-colour midpoint above s (0..255).
+__device__ __forceinline__ int nc_search(const DevOChan &q, double s)
+{
+	return count_le(q.nc, 255, q.ix, q.ix_log2, s);
 }
 
 constexpr uint64_t kMontMinv = 0xfff0bf22ffffffffull;
@@ -327,37 +341,25 @@ __device__ __forceinline__ uint32_t prng_draw(const DevOChan &q, int i, int j)
 	return (uint32_t)((t - 1) / iwx::kPrngA);
 }
 
-__device__ __forceinline__ void store_sample(const DevPlan &P, int i, int j, int o, double s_full)
-{
-	uint8_t *row = P.out + (size_t)j * P.out_bpr;
-	if (P.out_depth == 8) {
-		row[P.out_nch * i + o] = (uint8_t)(0.5 + s_full);
-	} else {
-		const unsigned int v = (unsigned short)(0.5 + s_full);
-		uint8_t *q = row + (size_t)(P.out_nch * i + o) * 2;
-		q[0] = (uint8_t)(v >> 8);
-		q[1] = (uint8_t)(v & 0xff);
-	}
-}
-
 // put_sample_convert_from_linear for the non-error-diffusion families.
 __device__ __forceinline__ double quantize(const DevOChan &q, double s, int i, int j)
 {
-	if (q.qmode == Q_NC) return (double)nc_search(q.nc, s);
+	if (q.qmode == Q_NC) return (double)nc_search(q, s);
+	const g_cf64 *U = (const g_cf64 *)q.U, *lin = (const g_cf64 *)q.lin, *code = (const g_cf64 *)q.code;
 	const int k = level_floor(q, s);
-	if (k >= q.P || s < q.U[k]) return q.code[k];
-	const double d_floor = s - q.lin[k];
-	const double d_ceil = q.lin[k + 1] - s;
+	if (k >= q.P || s < U[k]) return code[k];
+	const double d_floor = s - lin[k];
+	const double d_ceil = lin[k + 1] - s;
 	switch (q.family) {
 	case IW_DITHERFAMILY_NONE:
-		return (d_ceil <= d_floor) ? q.code[k + 1] : q.code[k];
+		return (d_ceil <= d_floor) ? code[k + 1] : code[k];
 	case IW_DITHERFAMILY_ORDERED: {
 		const double thr = (double)c_dither_pattern[q.subtype ? 1 : 0][(i % 8) + 8 * (j % 8)];
-		return (d_floor / (d_floor + d_ceil) >= thr) ? q.code[k + 1] : q.code[k];
+		return (d_floor / (d_floor + d_ceil) >= thr) ? code[k + 1] : code[k];
 	}
 	case IW_DITHERFAMILY_RANDOM: {
 		const double thr = ((double)prng_draw(q, i, j)) / (double)0xffffffff;
-		return (d_floor / (d_floor + d_ceil) >= thr) ? q.code[k + 1] : q.code[k];
+		return (d_floor / (d_floor + d_ceil) >= thr) ? code[k + 1] : code[k];
 	}
 	}
 	return 0.0;
@@ -383,52 +385,120 @@ __device__ __forceinline__ double hgather(const DevPlan &P, int c, int j, int x)
 	return acc;
 }
 
+// The plan fields the per-pixel work reads, copied out of the plan once per
+// work item so that a loop over pixels keeps them in registers instead of
+// loading them again for every pixel.  Slot idx is the idx-th channel in
+// processing order (alpha first).
+struct EmitCtx {
+	int nch, intclamp, checker, check_size, ox, oy, bk_has_trans, late_bk, out_float, out_depth, out_nch;
+	int out_w, out_h;
+	double bk1a, bk2a;
+	size_t out_bpr;
+	uint8_t *out;
+	double *ed_samples;
+	int c[IWAMD_MAX_CH], o[IWAMD_MAX_CH], unassoc[IWAMD_MAX_CH], is_alpha[IWAMD_MAX_CH];
+	DevOChan q[IWAMD_MAX_CH];
+};
+
+__device__ __forceinline__ void emit_ctx(const DevPlan &P, EmitCtx &E)
+{
+	E.nch = P.inter_nch;
+	E.intclamp = P.intclamp;
+	E.checker = P.checker;
+	E.check_size = P.check_size;
+	E.ox = P.ox;
+	E.oy = P.oy;
+	E.bk_has_trans = P.bk_has_trans;
+	E.late_bk = P.apply_bkgd && P.late;
+	E.out_float = P.out_float;
+	E.out_depth = P.out_depth;
+	E.out_nch = P.out_nch;
+	E.out_w = P.out_w;
+	E.out_h = P.out_h;
+	E.bk1a = P.bk1a;
+	E.bk2a = P.bk2a;
+	E.out_bpr = P.out_bpr;
+	E.out = P.out;
+	E.ed_samples = P.ed_samples;
+#pragma unroll
+	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
+		const int c = idx < P.inter_nch ? P.order[idx] : 0;
+		const DevIChan &ic = P.ic[c];
+		E.c[idx] = c;
+		E.o[idx] = idx < P.inter_nch ? ic.output_channel : -1;
+		E.unassoc[idx] = ic.need_unassoc;
+		E.is_alpha[idx] = c == P.inter_alpha;
+		E.q[idx] = P.oc[E.o[idx] >= 0 ? E.o[idx] : 0];
+	}
+}
+
+__device__ __forceinline__ void emit_store(const EmitCtx &E, int i, int j, int o, double s_full)
+{
+	g_u8 *row = (g_u8 *)E.out + (size_t)j * E.out_bpr;
+	if (E.out_depth == 8) {
+		row[E.out_nch * i + o] = (uint8_t)(0.5 + s_full);
+	} else {
+		const unsigned int v = (unsigned short)(0.5 + s_full);
+		g_u8 *q = row + (size_t)(E.out_nch * i + o) * 2;
+		q[0] = (uint8_t)(v >> 8);
+		q[1] = (uint8_t)(v & 0xff);
+	}
+}
+
 // One output pixel: every channel's horizontally resampled value (alpha
 // first, rounded to FP32 like the reference's final_alpha32), unpremultiply,
 // late background, then store, quantise, or hand the sample to error
 // diffusion (src/imagew-main.c:1036-1123).  gather(c) returns the resampled
 // value of intermediate channel c at this pixel.
 template <class Gather>
-__device__ __forceinline__ void emit_pixel(const DevPlan &P, int i, int j, Gather gather)
+__device__ __forceinline__ void emit_px(const EmitCtx &E, int i, int j, Gather gather)
 {
 	float alpha_f = 0.0f;
 	int alt = 0;
-	if (P.checker)
-		alt = (((P.ox + i) / P.check_size) % 2) != (((P.oy + j) / P.check_size) % 2);
-	const double bkalpha = P.bk_has_trans ? (alt ? P.bk2a : P.bk1a) : 0.0;
-	for (int idx = 0; idx < P.inter_nch; idx++) {
-		const int c = P.order[idx];
-		const DevIChan &ic = P.ic[c];
-		double v = gather(c);
-		if (P.intclamp) v = clamp01(v);
-		const bool is_alpha = (c == P.inter_alpha);
+	if (E.checker)
+		alt = (((E.ox + i) / E.check_size) % 2) != (((E.oy + j) / E.check_size) % 2);
+	const double bkalpha = E.bk_has_trans ? (alt ? E.bk2a : E.bk1a) : 0.0;
+#pragma unroll
+	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
+		if (idx >= E.nch) break;
+		double v = gather(E.c[idx]);
+		if (E.intclamp) v = clamp01(v);
+		const bool is_alpha = E.is_alpha[idx];
 		if (is_alpha) alpha_f = (float)v;
-		const int o = ic.output_channel;
+		const int o = E.o[idx];
 		if (o < 0) continue;
-		const DevOChan &q = P.oc[o];
-		if (ic.need_unassoc) {
+		const DevOChan &q = E.q[idx];
+		if (E.unassoc[idx]) {
 			const double a = (double)alpha_f;
 			if (a != 0.0) v /= a;
-			if (P.apply_bkgd && P.late) {
+			if (E.late_bk) {
 				const double bk = alt ? q.bk2 : q.bk1;
-				if (P.bk_has_trans) v = v * a + bk * bkalpha * (1.0 - a);
+				if (E.bk_has_trans) v = v * a + bk * bkalpha * (1.0 - a);
 				else v = v * a + bk * (1.0 - a);
 			}
-		} else if (is_alpha && P.bk_has_trans) {
+		} else if (is_alpha && E.bk_has_trans) {
 			v = v + bkalpha * (1.0 - v);
 		}
-		if (P.out_float) {
+		if (E.out_float) {
 			const float f = (float)v;
-			memcpy(P.out + (size_t)j * P.out_bpr + (size_t)(P.out_nch * i + o) * 4, &f, 4);
+			memcpy(E.out + (size_t)j * E.out_bpr + (size_t)(E.out_nch * i + o) * 4, &f, 4);
 			continue;
 		}
 		v = clamp01(v);
 		if (q.family == IW_DITHERFAMILY_ERRDIFF) {
-			P.ed_samples[((size_t)q.ed_index * P.out_h + j) * P.out_w + i] = v;
+			((__attribute__((address_space(1))) double *)E.ed_samples)[((size_t)q.ed_index * E.out_h + j) * E.out_w + i] = v;
 			continue;
 		}
-		store_sample(P, i, j, o, quantize(q, v, i, j));
+		emit_store(E, i, j, o, quantize(q, v, i, j));
 	}
+}
+
+template <class Gather>
+__device__ __forceinline__ void emit_pixel(const DevPlan &P, int i, int j, Gather gather)
+{
+	EmitCtx E;
+	emit_ctx(P, E);
+	emit_px(E, i, j, gather);
 }
 
 __global__ void __launch_bounds__(256) hpass_kernel(DevPlan P)
@@ -525,10 +595,12 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 	__syncthreads();
 	// Horizontal pass and per-pixel work for the band's rows.
 	const int n_px = P.out_w * rows;
+	EmitCtx E;
+	emit_ctx(P, E);
 	for (int t = tid; t < n_px; t += blockDim.x) {
 		const int r = t / P.out_w, i = t % P.out_w;
 		const int j = j0 + r;
-		emit_pixel(P, i, j, [&](int c) {
+		emit_px(E, i, j, [&](int c) {
 			const DevWeights &h = P.ic[c].h;
 			const float *row = inter + ((size_t)c * R + r) * W;
 			if (h.mode == iwx::WeightList::COPY) {
@@ -607,10 +679,13 @@ struct FastItem {
 // fp16((v - hi)*2048), so a product is hi*hi + (hi*lo + lo*hi)/2048 to about
 // 22 bits -- three MFMAs, the two cross terms in their own FP32 sum.  (A
 // single FP16 product carries 11 bits, which the division by a small
-// resampled alpha would magnify past one output level.)
-struct FastLds {
-	_Float16 wv[2][FAST_ROWS][FAST_MAX_KV];                 // hi, lo
-	_Float16 inter[2][IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_XT];  // hi, lo
+// resampled alpha would magnify past one output level.)  Intermediate rows
+// carry 16 zero columns past the tile's last 16-column group, so the
+// horizontal pass reads four-column groups without bounds checks.
+#define FAST_XT_STRIDE (FAST_MAX_XT + 16)
+struct __attribute__((aligned(16))) FastLds {
+	_Float16 wv[2][FAST_ROWS][FAST_MAX_KV];                    // hi, lo
+	_Float16 inter[2][IWAMD_MAX_CH][FAST_ROWS][FAST_XT_STRIDE];  // hi, lo
 	float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
 };
 
@@ -623,6 +698,167 @@ __device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo)
 __device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c)
 {
 	return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// Direct colour-in for integer inputs of 8 or 16 bits whose intermediate
+// channels each read one stored sample (no grey conversion, no virtual
+// channel): the logical-to-physical map is affine, byte offset
+// base + x*dx + y*dy for logical (x, y), and a pixel's alpha is looked up once
+// for all its channels.  The arithmetic is colour_in's, in the same order.
+// Every load is unconditional -- slots past the plan's channels repeat
+// channel 0 and an unneeded alpha reads channel 0 too -- so a lane issues all
+// of a row's loads before it waits for any.
+struct FastCin {
+	long long base, dx, dy;
+	const double *lut[IWAMD_MAX_CH];
+	double bkgd[IWAMD_MAX_CH], edge[IWAMD_MAX_CH];
+	int chb[IWAMD_MAX_CH];   // byte of each channel's sample within a pixel
+	int mode[IWAMD_MAX_CH];  // 0: as read, 1: times alpha, 2: early background
+	int et[IWAMD_MAX_CH];    // rows outside the image read edge[c]
+	int ab;                  // byte of the alpha sample
+	const double *alut;      // its table (channel 0's when alpha is not needed)
+};
+
+__device__ __forceinline__ int fast_cin_depth(const DevPlan &P, FastCin &F)
+{
+	if (P.in_float || (P.bit_depth != 8 && P.bit_depth != 16)) return 0;
+	const int bps = P.bit_depth / 8, pb = P.nch_phys * bps;
+	bool need_a = false;
+	for (int c = 0; c < IWAMD_MAX_CH; c++) {
+		const DevIChan &ic = P.ic[c < P.inter_nch ? c : 0];
+		if (ic.cvt_gray || ic.input_channel >= P.nch_phys || !ic.lut[0]) return 0;
+		F.chb[c] = ic.input_channel * bps;
+		F.lut[c] = ic.lut[0];
+		F.mode[c] = ic.need_unassoc ? 1 : (ic.early_bkgd ? 2 : 0);
+		F.bkgd[c] = ic.bkgd;
+		F.edge[c] = ic.v.edge;
+		F.et[c] = ic.v.edge_transparent;
+		need_a |= F.mode[c] != 0;
+	}
+	F.ab = F.chb[0];
+	F.alut = F.lut[0];
+	if (need_a) {
+		if (P.alpha_in_index >= P.nch_phys || !P.alpha_lut) return 0;
+		F.ab = P.alpha_in_index * bps;
+		F.alut = P.alpha_lut;
+	}
+	int rx, ry;
+	long long o[3];
+	const int px[3] = {0, 1, 0}, py[3] = {0, 0, 1};
+	for (int k = 0; k < 3; k++) {
+		logical_to_physical(P, px[k], py[k], rx, ry);
+		o[k] = (long long)ry * (long long)P.in_bpr + (long long)rx * pb;
+	}
+	F.base = o[0];
+	F.dx = o[1] - o[0];
+	F.dy = o[2] - o[0];
+	return P.bit_depth;
+}
+
+template <int DEPTH>
+__device__ __forceinline__ unsigned int fast_raw(const g_u8 *in, long long off)
+{
+	if (DEPTH == 8) return in[off];
+	return ((unsigned int)in[off] << 8) | in[off + 1];
+}
+
+// Intermediate channels 0..C-1 of logical input (xc, p), xc in [0, W): the
+// band's sample, as band_sample(P, P.ic[c], xc, p, P.ih) would give it
+// (channels from C on are left unspecified).
+template <int DEPTH>
+__device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, long long cb, int p,
+	float v[IWAMD_MAX_CH])
+{
+	const int C = P.inter_nch;
+	if (DEPTH == 0) {
+		// cb holds the column here.
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++)
+			v[c] = c < C ? (float)band_sample(P, P.ic[c], (int)cb, p, P.ih) : 0.0f;
+		return;
+	}
+	const int pin = p < 0 ? 0 : (p >= P.ih ? P.ih - 1 : p);
+	const long long off = cb + (long long)pin * F.dy;
+	const g_u8 *in = (const g_u8 *)P.in;
+	unsigned int raw[IWAMD_MAX_CH];
+#pragma unroll
+	for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = fast_raw<DEPTH>(in, off + F.chb[c]);
+	const double a = ((const g_cf64 *)F.alut)[fast_raw<DEPTH>(in, off + F.ab)];
+#pragma unroll
+	for (int c = 0; c < IWAMD_MAX_CH; c++) {
+		const double x = ((const g_cf64 *)F.lut[c])[raw[c]];
+		const double xm = x * a;
+		const double xb = a * x + (1.0 - a) * F.bkgd[c];
+		double y = F.mode[c] == 1 ? xm : (F.mode[c] == 2 ? xb : x);
+		if (p != pin && F.et[c]) y = F.edge[c];
+		v[c] = (float)y;
+	}
+}
+
+// Vertical pass of a tile: 16-column groups of the intermediate, one per wave
+// in turn; each lane fetches its own B operands (rows kk + 4*lq .. +3 of its
+// column), so no sample is read twice within a group.
+template <int DEPTH>
+__device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &w, FastLds &S, const FastCin &F)
+{
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int C = P.inter_nch;
+	const int lr = lane & 15, lq = lane >> 4;
+	const int kv_pad = (w.kv + 15) & ~15;
+	const float inv = 1.0f / 2048.0f;
+	for (int ct = wave; ct * 16 < w.nx; ct += 4) {
+		const int col = ct * 16 + lr;
+		const int x = w.x_lo + col;
+		const bool outside = x < 0 || x >= P.W;
+		const bool live = col < w.nx;
+		const int xc = x < 0 ? 0 : (x >= P.W ? P.W - 1 : x);
+		const long long cb = DEPTH ? F.base + (long long)xc * F.dx : (long long)xc;
+		float4_t acc[IWAMD_MAX_CH], acl[IWAMD_MAX_CH];
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) {
+			acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+			acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+		}
+		for (int kk = 0; kk < kv_pad; kk += 16) {
+			const half4_t ah = *(const half4_t *)&S.wv[0][lr][kk + 4 * lq];
+			const half4_t al = *(const half4_t *)&S.wv[1][lr][kk + 4 * lq];
+			float v[4][IWAMD_MAX_CH];
+#pragma unroll
+			for (int t = 0; t < 4; t++) fast_fetch<DEPTH>(P, F, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+			half4_t bh[IWAMD_MAX_CH], bl[IWAMD_MAX_CH];
+#pragma unroll
+			for (int t = 0; t < 4; t++) {
+				const bool in = live && kk + 4 * lq + t < w.kv;
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++) {
+					_Float16 h, l;
+					split16(in ? v[t][c] : 0.0f, h, l);
+					bh[c][t] = h;
+					bl[c][t] = l;
+				}
+			}
+#pragma unroll
+			for (int c = 0; c < IWAMD_MAX_CH; c++) {
+				if (c >= C) continue;
+				acc[c] = mfma16(ah, bh[c], acc[c]);
+				acl[c] = mfma16(ah, bl[c], acl[c]);
+				acl[c] = mfma16(al, bh[c], acl[c]);
+			}
+		}
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) {
+			if (c >= C) continue;
+#pragma unroll
+			for (int t = 0; t < 4; t++) {
+				float v = acc[c][t] + acl[c][t] * inv;
+				// A virtual column outside a TRANSPARENT edge is the edge value.
+				if (outside && P.ic[c].h.edge_transparent) v = (float)P.ic[c].h.edge;
+				if (P.intclamp) v = fminf(fmaxf(v, 0.0f), 1.0f);
+				if (!live) v = 0.0f;
+				split16(v, S.inter[0][c][4 * lq + t][col], S.inter[1][c][4 * lq + t][col]);
+			}
+		}
+	}
 }
 
 // Operand layout of v_mfma_f32_16x16x16f16 for lane l: A[l%16][4*(l/16)+t],
@@ -638,84 +874,56 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 	const float inv = 1.0f / 2048.0f;
 
 	// Vertical weights of the band, split.
-	for (int t = tid; t < FAST_ROWS * kv_pad; t += blockDim.x) {
-		const int r = t / kv_pad, k = t % kv_pad;
-		float v = 0.0f;
-		const int j = w.j0 + r;
-		if (j < P.Hp && k < w.kv) {
-			const int kk = w.p_lo + k - dv.dfirst[j];
-			if (kk >= 0 && kk < dv.dcount[j]) v = (float)dv.dw[dv.doff[j] + kk];
+	{
+		const g_cint *vfirst = (const g_cint *)dv.dfirst, *vcount = (const g_cint *)dv.dcount;
+		const g_cint *voff = (const g_cint *)dv.doff;
+		const g_cf64 *vw = (const g_cf64 *)dv.dw;
+		for (int t = tid; t < FAST_ROWS * kv_pad; t += blockDim.x) {
+			const int r = t / kv_pad, k = t % kv_pad;
+			float v = 0.0f;
+			const int j = w.j0 + r;
+			if (j < P.Hp && k < w.kv) {
+				const int kk = w.p_lo + k - vfirst[j];
+				if (kk >= 0 && kk < vcount[j]) v = (float)vw[voff[j] + kk];
+			}
+			split16(v, S.wv[0][r][k], S.wv[1][r][k]);
 		}
-		split16(v, S.wv[0][r][k], S.wv[1][r][k]);
+	}
+	// Zero columns past the last 16-column group (read by the horizontal pass).
+	const int nx16 = (w.nx + 15) & ~15;
+	for (int t = tid; t < 2 * C * FAST_ROWS * 16; t += blockDim.x) {
+		const int col = t & 15, r = (t >> 4) & 15, hc = t >> 8;
+		S.inter[hc & 1][hc >> 1][r][nx16 + col] = (_Float16)0.0f;
 	}
 	__syncthreads();
 
-	// Vertical pass: 16-column tiles of the intermediate, one per wave in turn.
-	for (int ct = wave; ct * 16 < w.nx; ct += 4) {
-		const int col = ct * 16 + lr;
-		const int x = w.x_lo + col;
-		const bool outside = x < 0 || x >= P.W;
-		const int xc = x < 0 ? 0 : (x >= P.W ? P.W - 1 : x);
-		float4_t acc[IWAMD_MAX_CH], acl[IWAMD_MAX_CH];
-#pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++) {
-			acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-			acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-		}
-		for (int kk = 0; kk < kv_pad; kk += 16) {
-			half4_t ah, al;
-#pragma unroll
-			for (int t = 0; t < 4; t++) {
-				ah[t] = S.wv[0][lr][kk + 4 * lq + t];
-				al[t] = S.wv[1][lr][kk + 4 * lq + t];
-			}
-			half4_t bh[IWAMD_MAX_CH], bl[IWAMD_MAX_CH];
-#pragma unroll
-			for (int t = 0; t < 4; t++) {
-				const int k = kk + 4 * lq + t;
-#pragma unroll
-				for (int c = 0; c < IWAMD_MAX_CH; c++) {
-					float v = 0.0f;
-					if (c < C && k < w.kv && col < w.nx) v = (float)band_sample(P, P.ic[c], xc, w.p_lo + k, P.ih);
-					_Float16 h, l;
-					split16(v, h, l);
-					bh[c][t] = h;
-					bl[c][t] = l;
-				}
-			}
-#pragma unroll
-			for (int c = 0; c < IWAMD_MAX_CH; c++) {
-				if (c >= C) continue;
-				acc[c] = mfma16(ah, bh[c], acc[c]);
-				acl[c] = mfma16(ah, bl[c], acl[c]);
-				acl[c] = mfma16(al, bh[c], acl[c]);
-			}
-		}
-		if (col < w.nx) {
-#pragma unroll
-			for (int c = 0; c < IWAMD_MAX_CH; c++) {
-				if (c >= C) continue;
-#pragma unroll
-				for (int t = 0; t < 4; t++) {
-					float v = acc[c][t] + acl[c][t] * inv;
-					// A virtual column outside a TRANSPARENT edge is the edge value.
-					if (outside && P.ic[c].h.edge_transparent) v = (float)P.ic[c].h.edge;
-					if (P.intclamp) v = fminf(fmaxf(v, 0.0f), 1.0f);
-					split16(v, S.inter[0][c][4 * lq + t][col], S.inter[1][c][4 * lq + t][col]);
-				}
-			}
-		}
+	FastCin F;
+	switch (fast_cin_depth(P, F)) {
+	case 8: fast_vertical<8>(P, w, S, F); break;
+	case 16: fast_vertical<16>(P, w, S, F); break;
+	default: fast_vertical<0>(P, w, S, F); break;
 	}
 	__syncthreads();
 
-	// Horizontal pass: 16-column output groups, one per wave.
+	// Horizontal pass: 16-column output groups, one per wave.  K runs over
+	// intermediate positions from the group's first listed one rounded down to
+	// a multiple of 4 (x_lo is one, so the groups are four-aligned in LDS);
+	// positions a lane's list does not hold weigh 0.
+	const g_cint *hfirst = (const g_cint *)dh.dfirst, *hcount = (const g_cint *)dh.dcount;
+	const g_cint *hoff = (const g_cint *)dh.doff;
+	const g_cf64 *hw = (const g_cf64 *)dh.dw;
 	for (int g = wave; g * 16 < w.ni; g += 4) {
 		const int i = w.i0 + g * 16 + lr;
 		const bool live = (g * 16 + lr) < w.ni;
-		int f = 0x7fffffff, e = -0x7fffffff;
-		if (live && dh.dcount[i] > 0) {
-			f = dh.dfirst[i];
-			e = f + dh.dcount[i] - 1;
+		int f = 0x7fffffff, e = -0x7fffffff, lf = 0, ln = 0, lo = 0;
+		if (live) {
+			ln = hcount[i];
+			lf = hfirst[i];
+			lo = hoff[i];
+			if (ln > 0) {
+				f = lf;
+				e = lf + ln - 1;
+			}
 		}
 		for (int m = 1; m < 16; m <<= 1) {
 			f = min(f, __shfl_xor(f, m));
@@ -728,16 +936,12 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 			acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
 		}
 		if (e >= f) {
-			for (int x0 = f - w.x_lo; x0 <= e - w.x_lo; x0 += 16) {
+			for (int x0 = (f & ~3) - w.x_lo; x0 <= e - w.x_lo; x0 += 16) {
 				half4_t bh, bl;
 #pragma unroll
 				for (int t = 0; t < 4; t++) {
-					const int xk = x0 + 4 * lq + t;
-					float v = 0.0f;
-					if (live) {
-						const int kk = w.x_lo + xk - dh.dfirst[i];
-						if (kk >= 0 && kk < dh.dcount[i]) v = (float)dh.dw[dh.doff[i] + kk];
-					}
+					const int kk = w.x_lo + x0 + 4 * lq + t - lf;
+					const float v = (kk >= 0 && kk < ln) ? (float)hw[lo + kk] : 0.0f;
 					_Float16 h, l;
 					split16(v, h, l);
 					bh[t] = h;
@@ -746,14 +950,8 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 #pragma unroll
 				for (int c = 0; c < IWAMD_MAX_CH; c++) {
 					if (c >= C) continue;
-					half4_t ah, al;
-#pragma unroll
-					for (int t = 0; t < 4; t++) {
-						const int xk = x0 + 4 * lq + t;
-						const bool in = xk >= 0 && xk < w.nx;
-						ah[t] = in ? S.inter[0][c][lr][xk] : (_Float16)0.0f;
-						al[t] = in ? S.inter[1][c][lr][xk] : (_Float16)0.0f;
-					}
+					const half4_t ah = *(const half4_t *)&S.inter[0][c][lr][x0 + 4 * lq];
+					const half4_t al = *(const half4_t *)&S.inter[1][c][lr][x0 + 4 * lq];
 					acc[c] = mfma16(ah, bh, acc[c]);
 					acl[c] = mfma16(ah, bl, acl[c]);
 					acl[c] = mfma16(al, bh, acl[c]);
@@ -770,9 +968,11 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 
 	// Per-pixel work in FP64, as in the plain form.
 	const int rows = min(FAST_ROWS, P.Hp - w.j0);
+	EmitCtx E;
+	emit_ctx(P, E);
 	for (int t = tid; t < rows * w.ni; t += blockDim.x) {
 		const int r = t / w.ni, ii = t % w.ni;
-		emit_pixel(P, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
+		emit_px(E, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
 	}
 }
 
@@ -1026,7 +1226,7 @@ struct EdCtx {
 	int out_nch, out_depth;
 };
 
-// store_sample() through the context's global view of the output.
+// emit_store() through the context's global view of the output.
 __device__ __forceinline__ void ed_store(const EdCtx &c, int i, int j, double s_full)
 {
 	g_u8 *row = c.out + (size_t)j * c.out_bpr;
@@ -1617,7 +1817,7 @@ struct iwamd_job {
 	hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 	int n_ed = 0;
 	int ed_list[4] = {-1, -1, -1, -1};
-	double out_pixels = 0.0, alg_bytes = 0.0;
+	double out_pixels = 0.0, alg_bytes = 0.0, flops = 0.0;
 	int band_rows = 0;  // rows per band of the fused kernel; 0: two-pass kernels
 	bool ran = false;
 	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
@@ -1629,7 +1829,7 @@ namespace {
 // Host side of one job's device data: the tables, where each lives in the
 // job's part of a device block, and the resampling path.
 struct WOff { size_t start, src, w, copy, dfirst, dcount, doff, dw; };
-struct QOff { size_t T, U, lin, code, nc, prow, pcol, M, pre, suf; };
+struct QOff { size_t T, U, lin, code, nc, ix, prow, pcol, M, pre, suf; };
 
 struct Staged {
 	std::vector<iwx::WeightList> wv, wh;
@@ -1712,13 +1912,26 @@ void dense_span(const iwx::DenseWeights &d, int a, int n, int &lo, int &cnt)
 	cnt = h - l + 1;
 }
 
+// Intermediate columns of a tile: the span of its lists, started at a
+// multiple of 4 (the horizontal pass sums over four-aligned groups of
+// positions, the same for every tile width).
+void fast_tile_span(const iwx::DenseWeights &dh, int i0, int ni, int &lo, int &cnt)
+{
+	dense_span(dh, i0, ni, lo, cnt);
+	if (cnt > 0) {
+		const int hi = lo + cnt - 1;
+		lo &= ~3;
+		cnt = hi - lo + 1;
+	}
+}
+
 // Whether tiles of ti output columns span at most FAST_MAX_XT intermediate
 // columns.
 bool fast_tile_fits(const iwamd_plan &p, const iwx::DenseWeights &dh, int ti)
 {
 	for (int i0 = 0; i0 < p.out_width; i0 += ti) {
 		int lo, n;
-		dense_span(dh, i0, std::min(ti, p.out_width - i0), lo, n);
+		fast_tile_span(dh, i0, std::min(ti, p.out_width - i0), lo, n);
 		if (n > FAST_MAX_XT) return false;
 	}
 	return true;
@@ -1733,6 +1946,8 @@ void plan_fast(const iwamd_plan &p, Staged &S, int want_ti = 0)
 {
 	S.fast.clear();
 	if (p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT || p.out_bit_depth > 16) return;
+	// Floating-point inputs can hold values past FP16's range.
+	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT) return;
 	for (int c = 1; c < p.inter_nch; c++) {
 		if (!same_dense(S.dv[c], S.dv[0]) || !same_dense(S.dh[c], S.dh[0])) return;
 		if (S.wv[c].edge_value != S.wv[0].edge_value) return;
@@ -1761,7 +1976,7 @@ void plan_fast(const iwamd_plan &p, Staged &S, int want_ti = 0)
 		for (int i0 = 0; i0 < p.out_width; i0 += ti) {
 			it.i0 = i0;
 			it.ni = std::min(ti, p.out_width - i0);
-			span(dh, i0, it.ni, it.x_lo, it.nx);
+			fast_tile_span(dh, i0, it.ni, it.x_lo, it.nx);
 			S.fast.push_back(it);
 		}
 	}
@@ -1931,6 +2146,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 		q.lin = L.add<double>(S.qt[o].lin.size());
 		q.code = L.add<double>(S.qt[o].code.size());
 		q.nc = L.add<double>(S.qt[o].nc.size());
+		q.ix = L.add<uint16_t>(S.qt[o].ix.size());
 		q.prow = L.add<uint64_t>(S.pt[o].row_q.size());
 		q.pcol = L.add<uint64_t>(S.pt[o].col_mont.size());
 		q.M = L.add<double>(S.et[o].M.size());
@@ -1976,6 +2192,7 @@ bool upload_job(const Staged &S, const iwamd_plan &p, char *base, hipStream_t st
 		up(q.lin, S.qt[o].lin.data(), S.qt[o].lin.size() * 8);
 		up(q.code, S.qt[o].code.data(), S.qt[o].code.size() * 8);
 		up(q.nc, S.qt[o].nc.data(), S.qt[o].nc.size() * 8);
+		up(q.ix, S.qt[o].ix.data(), S.qt[o].ix.size() * 2);
 		up(q.prow, S.pt[o].row_q.data(), S.pt[o].row_q.size() * 8);
 		up(q.pcol, S.pt[o].col_mont.data(), S.pt[o].col_mont.size() * 8);
 		up(q.M, S.et[o].M.data(), S.et[o].M.size() * 8);
@@ -2075,6 +2292,8 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 			d.lin = (const double *)(base + q.lin);
 			d.code = (const double *)(base + q.code);
 			d.nc = (const double *)(base + q.nc);
+			d.ix = S.qt[o].ix.empty() ? nullptr : (const unsigned short *)(base + q.ix);
+			d.ix_log2 = S.qt[o].ix_log2;
 			d.prng_row = (const uint64_t *)(base + q.prow);
 			d.prng_col = (const uint64_t *)(base + q.pcol);
 			d.M = (const double *)(base + q.M);
@@ -2281,6 +2500,22 @@ void ws_enqueue_errdiff(const WorkSet &W, hipStream_t st)
 		const int n = (int)W.planes[v].size();
 		if (n) hipLaunchKernelGGL(kEdKernels[v], dim3(n), dim3(ED_THREADS), 0, st, W.d_plans, W.d_planes[v]);
 	}
+}
+
+// Useful floating-point operations of a plan's resampling: a multiply and an
+// add per listed tap, vertically for every intermediate sample and
+// horizontally for every output sample, per channel.
+double resample_flops(const iwamd_plan &p, const Staged &S)
+{
+	if (S.skip_pixels || S.dv.empty() || S.dh.empty()) return 0.0;
+	double f = 0.0;
+	for (int c = 0; c < p.inter_nch; c++) {
+		double tv = 0.0, th = 0.0;
+		for (int j = 0; j < p.inter_height; j++) tv += S.dv[c].count[j];
+		for (int i = 0; i < p.out_width; i++) th += S.dh[c].count[i];
+		f += 2.0 * (tv * p.inter_width + th * p.out_height);
+	}
+	return f;
 }
 
 // ---- autotuner --------------------------------------------------------------
@@ -2511,6 +2746,7 @@ bool job_upload(iwamd_job *J)
 	}
 	J->out_pixels = (double)p.out_width * p.out_height;
 	J->alg_bytes = (double)S.in_bytes + (double)S.out_bytes;
+	J->flops = resample_flops(p, S);
 	return true;
 }
 
@@ -2591,6 +2827,11 @@ extern "C" void iwamd_job_sizes(struct iwamd_job *J, double *out_pixels, double 
 {
 	*out_pixels = J->out_pixels;
 	*alg_bytes = J->alg_bytes;
+}
+
+extern "C" double iwamd_job_flops(struct iwamd_job *J)
+{
+	return J->flops;
 }
 
 extern "C" int iwamd_job_finish(struct iwamd_job *J)
@@ -2735,7 +2976,7 @@ struct iwamd_batch {
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	bool uploaded = false, ran = false, failed = false;
-	double out_pixels = 0.0, alg_bytes = 0.0;
+	double out_pixels = 0.0, alg_bytes = 0.0, flops = 0.0;
 };
 
 extern "C" struct iwamd_batch *iwamd_batch_create(void)
@@ -2788,6 +3029,7 @@ extern "C" int iwamd_batch_upload(struct iwamd_batch *B)
 		ws_add(B->W, J->S, J->plan);
 		B->out_pixels += (double)J->plan.out_width * J->plan.out_height;
 		B->alg_bytes += (double)J->S.in_bytes + (double)J->S.out_bytes;
+		B->flops += resample_flops(J->plan, J->S);
 	}
 	B->out_total = arena;
 	ws_layout(B->W);
@@ -2881,6 +3123,11 @@ extern "C" int iwamd_batch_kernel_ms(struct iwamd_batch *B, double *ms, int n)
 	hipEventElapsedTime(&t, B->ev[0], B->ev[1]);
 	ms[0] = t;
 	return 1;
+}
+
+extern "C" double iwamd_batch_flops(struct iwamd_batch *B)
+{
+	return B->flops;
 }
 
 extern "C" void iwamd_batch_sizes(struct iwamd_batch *B, double *out_pixels, double *alg_bytes, int *n_jobs)

@@ -312,6 +312,22 @@ QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc)
 			prev = cur;
 		}
 	}
+	// Bins: about eight per searched entry, 2^11..2^16 of them.
+	const std::vector<double> &a = want_nc ? q.nc : q.T;
+	const int n = want_nc ? (int)q.nc.size() : P;
+	if (n > 65535) return q;  // counts would not fit: no index, full search
+	int L = 11;
+	while (L < 16 && (1 << L) < 8 * n) L++;
+	const int NB = 1 << L;
+	q.ix_log2 = L;
+	q.ix.resize(NB + 1);
+	int m = 0;
+	for (int b = 0; b < NB; b++) {
+		const double edge = (double)b / (double)NB;
+		while (m < n && a[m] < edge) m++;
+		q.ix[b] = (uint16_t)m;
+	}
+	q.ix[NB] = (uint16_t)n;
 	return q;
 }
 

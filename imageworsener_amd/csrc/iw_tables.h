@@ -62,6 +62,13 @@ struct QuantTable {
 	int P = 0;                   // number of levels - 1
 	std::vector<double> T, U, lin, code;
 	std::vector<double> nc;      // nearest-colour midpoints (255), when used
+	// Bin index over the array the quantiser searches (nc when used, else T),
+	// 2^ix_log2 bins of [0, 1]: ix[b] = #{m : a[m] < b / 2^ix_log2} for
+	// b < 2^ix_log2, and the array's length last.  A sample s in bin b counts
+	// ix[b] entries below it for certain and needs only a[ix[b] .. ix[b+1])
+	// compared.
+	std::vector<uint16_t> ix;
+	int ix_log2 = 0;
 };
 
 QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc);

@@ -196,6 +196,10 @@ IW_EXPORT(int) iwamd_job_kernel_ms(struct iwamd_job *job, double *ms, int n);
 /* Output samples (pixels) and algorithmic HBM bytes (input + output image)
  * of one run. */
 IW_EXPORT(void) iwamd_job_sizes(struct iwamd_job *job, double *out_pixels, double *alg_bytes);
+/* Useful floating-point operations of a run's resampling: a multiply and an
+ * add per listed tap of both passes, per channel (the chosen form's split
+ * operands execute three products for each). */
+IW_EXPORT(double) iwamd_job_flops(struct iwamd_job *job);
 
 /* Batches.  iwamd_batch_add() runs the host decisions of a context
  * (consuming its single use, like iw_process_image) and stages its tables;
@@ -220,6 +224,8 @@ IW_EXPORT(void) iwamd_batch_destroy(struct iwamd_batch *batch);
 /* Device time of the last run (HIP events around the graph), ms in ms[0]. */
 IW_EXPORT(int) iwamd_batch_kernel_ms(struct iwamd_batch *batch, double *ms, int n);
 IW_EXPORT(void) iwamd_batch_sizes(struct iwamd_batch *batch, double *out_pixels, double *alg_bytes, int *n_jobs);
+/* As iwamd_job_flops, summed over the batch (after iwamd_batch_upload). */
+IW_EXPORT(double) iwamd_batch_flops(struct iwamd_batch *batch);
 
 /* Request loop with continuous batching.  iwamd_server_create() starts a
  * worker thread on the current device with `streams` streams and an HBM
