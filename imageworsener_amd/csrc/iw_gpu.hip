@@ -665,6 +665,10 @@ __global__ void reset_counters_kernel(int *c, int n)
 #define FAST_MAX_KV 256   // input rows of a band
 #define FAST_MAX_XT 160   // intermediate columns of a tile
 #define FAST_MAX_TI 32    // output columns of a tile
+// Tiles whose band and span fit these run in a kernel with a third of the
+// LDS, four workgroups per CU instead of two.
+#define FAST_S_KV 96
+#define FAST_S_XT 80
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -681,12 +685,18 @@ struct FastItem {
 // single FP16 product carries 11 bits, which the division by a small
 // resampled alpha would magnify past one output level.)  Intermediate rows
 // carry 16 zero columns past the tile's last 16-column group, so the
-// horizontal pass reads four-column groups without bounds checks.
-#define FAST_XT_STRIDE (FAST_MAX_XT + 16)
+// horizontal pass reads four-column groups without bounds checks.  The
+// band weights (read by the vertical pass) and the horizontal sums (written
+// by the horizontal pass, read by the per-pixel work) are never live at once
+// and share their space.
+template <int KV, int XT>
 struct __attribute__((aligned(16))) FastLds {
-	_Float16 wv[2][FAST_ROWS][FAST_MAX_KV];                    // hi, lo
-	_Float16 inter[2][IWAMD_MAX_CH][FAST_ROWS][FAST_XT_STRIDE];  // hi, lo
-	float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
+	static constexpr int kKV = KV, kXT = XT;
+	union {
+		_Float16 wv[2][FAST_ROWS][KV];                                  // hi, lo
+		float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
+	};
+	_Float16 inter[2][IWAMD_MAX_CH][FAST_ROWS][XT + 16];  // hi, lo
 };
 
 __device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo)
@@ -798,8 +808,8 @@ __device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, l
 // Vertical pass of a tile: 16-column groups of the intermediate, one per wave
 // in turn; each lane fetches its own B operands (rows kk + 4*lq .. +3 of its
 // column), so no sample is read twice within a group.
-template <int DEPTH>
-__device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &w, FastLds &S, const FastCin &F)
+template <int DEPTH, class Lds>
+__device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &w, Lds &S, const FastCin &F)
 {
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int C = P.inter_nch;
@@ -863,7 +873,8 @@ __device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &
 
 // Operand layout of v_mfma_f32_16x16x16f16 for lane l: A[l%16][4*(l/16)+t],
 // B[4*(l/16)+t][l%16], D[4*(l/16)+t][l%16], t = 0..3.
-__device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, FastLds &S)
+template <class Lds>
+__device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, Lds &S)
 {
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int C = P.inter_nch;
@@ -899,9 +910,9 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 
 	FastCin F;
 	switch (fast_cin_depth(P, F)) {
-	case 8: fast_vertical<8>(P, w, S, F); break;
-	case 16: fast_vertical<16>(P, w, S, F); break;
-	default: fast_vertical<0>(P, w, S, F); break;
+	case 8: fast_vertical<8, Lds>(P, w, S, F); break;
+	case 16: fast_vertical<16, Lds>(P, w, S, F); break;
+	default: fast_vertical<0, Lds>(P, w, S, F); break;
 	}
 	__syncthreads();
 
@@ -976,10 +987,11 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, F
 	}
 }
 
-__global__ void __launch_bounds__(256) fast_kernel(const DevPlan *__restrict__ plans,
+template <int KV, int XT, int WAVES_PER_SIMD>
+__global__ void __launch_bounds__(256, WAVES_PER_SIMD) fast_kernel(const DevPlan *__restrict__ plans,
 	const FastItem *__restrict__ items, int n_items, int *counter)
 {
-	__shared__ FastLds S;
+	__shared__ FastLds<KV, XT> S;
 	__shared__ int next_item;
 	int it = blockIdx.x;
 	while (it < n_items) {
@@ -2360,10 +2372,10 @@ struct WorkSet {
 	size_t band_lds[3][kLdsClasses] = {};      // largest LDS of a band in each list
 	std::vector<int2> planes[8];      // (plan, output channel), by ed_variant
 	std::vector<int> twopass;         // plans resampled by vpass + hpass
-	std::vector<FastItem> fast;       // chosen-precision tiles
-	size_t o_plans = 0, o_bands[3][kLdsClasses] = {}, o_planes[8] = {0}, o_counters = 0, o_fast = 0, o_done = 0;
+	std::vector<FastItem> fast[2];    // chosen-precision tiles: small (FAST_S_*), large
+	size_t o_plans = 0, o_bands[3][kLdsClasses] = {}, o_planes[8] = {0}, o_counters = 0, o_fast[2] = {0, 0}, o_done = 0;
 	size_t bytes = 0;
-	const FastItem *d_fast = nullptr;
+	const FastItem *d_fast[2] = {nullptr, nullptr};
 	int *d_done = nullptr;  // one error-diffusion completion counter per plan
 	const DevPlan *d_plans = nullptr;
 	const int2 *d_bands[3][kLdsClasses] = {};
@@ -2381,7 +2393,7 @@ int ws_add(WorkSet &W, const Staged &S, const iwamd_plan &p)
 	if (!S.fast.empty()) {
 		for (FastItem it : S.fast) {
 			it.job = jx;
-			W.fast.push_back(it);
+			W.fast[it.kv <= FAST_S_KV && it.nx <= FAST_S_XT ? 0 : 1].push_back(it);
 		}
 	} else if (S.band_rows) {
 		const int ri = S.band_rows == 4 ? 2 : (S.band_rows == 2 ? 1 : 0);
@@ -2406,8 +2418,8 @@ void ws_layout(WorkSet &W)
 	for (int r = 0; r < 3; r++)
 		for (int q = 0; q < kLdsClasses; q++) W.o_bands[r][q] = L.add<int2>(W.bands[r][q].size());
 	for (int v = 0; v < 8; v++) W.o_planes[v] = L.add<int2>(W.planes[v].size());
-	W.o_counters = L.add<int>(3 * kLdsClasses + 1);
-	W.o_fast = L.add<FastItem>(W.fast.size());
+	W.o_counters = L.add<int>(3 * kLdsClasses + 2);
+	for (int k = 0; k < 2; k++) W.o_fast[k] = L.add<FastItem>(W.fast[k].size());
 	W.o_done = L.add<int>(W.plans.size());
 	W.bytes = (L.off + 255) & ~(size_t)255;
 }
@@ -2433,8 +2445,10 @@ bool ws_upload(WorkSet &W, char *base, hipStream_t st)
 		up(W.o_planes[v], W.planes[v].data(), W.planes[v].size() * sizeof(int2));
 	}
 	W.d_counters = (int *)(base + W.o_counters);
-	W.d_fast = (const FastItem *)(base + W.o_fast);
-	up(W.o_fast, W.fast.data(), W.fast.size() * sizeof(FastItem));
+	for (int k = 0; k < 2; k++) {
+		W.d_fast[k] = (const FastItem *)(base + W.o_fast[k]);
+		up(W.o_fast[k], W.fast[k].data(), W.fast[k].size() * sizeof(FastItem));
+	}
 	W.d_done = (int *)(base + W.o_done);
 	return ok;
 }
@@ -2459,15 +2473,21 @@ int device_cu_count()
 void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 {
 	if (persistent) {
-		hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 3 * kLdsClasses + 1);
+		hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 3 * kLdsClasses + 2);
 		const int np = (int)W.plans.size();
 		hipLaunchKernelGGL(reset_counters_kernel, dim3((np + 255) / 256), dim3(256), 0, st, W.d_done, np);
 	}
-	if (!W.fast.empty()) {
-		const int n = (int)W.fast.size();
+	if (!W.fast[0].empty()) {
+		const int n = (int)W.fast[0].size();
+		const int grid = persistent ? std::min(n, device_cu_count() * 4) : n;
+		hipLaunchKernelGGL((fast_kernel<FAST_S_KV, FAST_S_XT, 4>), dim3(grid), dim3(256), 0, st, W.d_plans,
+			W.d_fast[0], n, persistent ? W.d_counters + 3 * kLdsClasses : nullptr);
+	}
+	if (!W.fast[1].empty()) {
+		const int n = (int)W.fast[1].size();
 		const int grid = persistent ? std::min(n, device_cu_count() * 2) : n;
-		hipLaunchKernelGGL(fast_kernel, dim3(grid), dim3(256), 0, st, W.d_plans, W.d_fast, n,
-			persistent ? W.d_counters + 3 * kLdsClasses : nullptr);
+		hipLaunchKernelGGL((fast_kernel<FAST_MAX_KV, FAST_MAX_XT, 2>), dim3(grid), dim3(256), 0, st, W.d_plans,
+			W.d_fast[1], n, persistent ? W.d_counters + 3 * kLdsClasses + 1 : nullptr);
 	}
 	for (int r = 0; r < 3; r++)
 		for (int q = 0; q < kLdsClasses; q++) {
