@@ -83,6 +83,8 @@ struct DevOChan {
 	const double *T, *U, *lin, *code, *nc;
 	const unsigned short *ix;          // QuantTable::ix (null: none)
 	int ix_log2;
+	const double *D;                   // QuantTable::D (null: not folded)
+	int code_is_level;
 	const double *M, *ebpre, *ebsuf;  // error-diffusion decision data (EdTables)
 	double eb_max;
 	int ed_ordered;                    // EdTables::ordered
@@ -282,15 +284,20 @@ __constant__ float c_dither_pattern[2][64] = {
 // the bin's lower edge b/2^L <= s (s*2^L is exact), those from ix[b+1] on at
 // or above its upper edge, which is > s.  Others (no index, s < 0, NaN) search
 // the whole array.
+__device__ __forceinline__ int qbin(int L, double s)
+{
+	const int NB = 1 << L;
+	const double f = s * (double)NB;
+	return f >= (double)(NB - 1) ? NB - 1 : (int)f;
+}
+
 __device__ __forceinline__ int count_le(const double *a_, int n, const unsigned short *ix_, int L, double s)
 {
 	const g_cf64 *a = (const g_cf64 *)a_;
 	int lo = 0;
 	if (ix_ && s >= 0.0) {
 		const g_cu16 *ix = (const g_cu16 *)ix_;
-		const int NB = 1 << L;
-		const double f = s * (double)NB;
-		const int b = f >= (double)(NB - 1) ? NB - 1 : (int)f;
+		const int b = qbin(L, s);
 		lo = ix[b];
 		n = ix[b + 1] - lo;
 	}
@@ -309,7 +316,7 @@ __device__ __forceinline__ int count_le(const double *a_, int n, const unsigned 
 // floor(e(s)): number of thresholds <= s.
 __device__ __forceinline__ int level_floor(const DevOChan &q, double s)
 {
-	return count_le(q.T, q.P, q.qmode == Q_THRESH ? q.ix : nullptr, q.ix_log2, s);
+	return count_le(q.T, q.P, (q.qmode == Q_THRESH && !q.D) ? q.ix : nullptr, q.ix_log2, s);
 }
 
 // Index of the first nearest-colour midpoint above s (0..255).
@@ -346,6 +353,11 @@ __device__ __forceinline__ double quantize(const DevOChan &q, double s, int i, i
 {
 	if (q.qmode == Q_NC) return (double)nc_search(q, s);
 	const g_cf64 *U = (const g_cf64 *)q.U, *lin = (const g_cf64 *)q.lin, *code = (const g_cf64 *)q.code;
+	if (q.D) {
+		// No dither, folded (QuantTable::D).
+		const int k = count_le(q.D, q.P, q.ix, q.ix_log2, s);
+		return q.code_is_level ? (double)k : code[k];
+	}
 	const int k = level_floor(q, s);
 	if (k >= q.P || s < U[k]) return code[k];
 	const double d_floor = s - lin[k];
@@ -398,6 +410,14 @@ struct EmitCtx {
 	double *ed_samples;
 	int c[IWAMD_MAX_CH], o[IWAMD_MAX_CH], unassoc[IWAMD_MAX_CH], is_alpha[IWAMD_MAX_CH];
 	DevOChan q[IWAMD_MAX_CH];
+	// simple: every output channel is quantised without dither through one
+	// binned search (nearest-colour midpoints or QuantTable::D) or handed to
+	// error diffusion; sa/sn/six/sL: the searched array, its length and bin
+	// index; code: count (slevel) or scode[count].
+	int simple;
+	int ed[IWAMD_MAX_CH], sn[IWAMD_MAX_CH], sL[IWAMD_MAX_CH], slevel[IWAMD_MAX_CH];
+	const double *sa[IWAMD_MAX_CH], *scode[IWAMD_MAX_CH];
+	const unsigned short *six[IWAMD_MAX_CH];
 };
 
 __device__ __forceinline__ void emit_ctx(const DevPlan &P, EmitCtx &E)
@@ -429,6 +449,22 @@ __device__ __forceinline__ void emit_ctx(const DevPlan &P, EmitCtx &E)
 		E.unassoc[idx] = ic.need_unassoc;
 		E.is_alpha[idx] = c == P.inter_alpha;
 		E.q[idx] = P.oc[E.o[idx] >= 0 ? E.o[idx] : 0];
+	}
+	E.simple = !P.out_float;
+#pragma unroll
+	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
+		const DevOChan &q = E.q[idx];
+		const bool nc = q.qmode == Q_NC;
+		E.ed[idx] = q.family == IW_DITHERFAMILY_ERRDIFF;
+		E.sa[idx] = nc ? q.nc : q.D;
+		E.sn[idx] = nc ? 255 : q.P;
+		E.six[idx] = q.ix;
+		E.sL[idx] = q.ix_log2;
+		E.scode[idx] = q.code;
+		E.slevel[idx] = nc || q.code_is_level;
+		if (idx < E.nch && E.o[idx] >= 0 && !E.ed[idx] &&
+			!(q.family == IW_DITHERFAMILY_NONE && q.ix && E.sa[idx] && E.sn[idx] > 0))
+			E.simple = 0;
 	}
 }
 
@@ -490,6 +526,100 @@ __device__ __forceinline__ void emit_px(const EmitCtx &E, int i, int j, Gather g
 			continue;
 		}
 		emit_store(E, i, j, o, quantize(q, v, i, j));
+	}
+}
+
+// emit_px for N pixels at once under EmitCtx::simple.  The table searches of
+// every channel and pixel go in stages -- all bin-index loads, then all
+// entry loads, then the codes -- so their latencies overlap instead of
+// adding up.  A bin holds at most two entries for 8-bit tables; a larger one
+// (or a sample outside [0, 1]) is searched by count_le.  gather(n, c):
+// channel c of pixel n; pixels with live[n] false are computed, not stored.
+template <int N, class Gather>
+__device__ __forceinline__ void emit_px_simple(const EmitCtx &E, const int (&pi)[N], const int (&pj)[N],
+	const bool (&live)[N], Gather gather)
+{
+	double v[IWAMD_MAX_CH][N];
+	float alpha_f[N];
+	int alt[N];
+	double bkalpha[N];
+#pragma unroll
+	for (int n = 0; n < N; n++) {
+		alpha_f[n] = 0.0f;
+		alt[n] = 0;
+		if (E.checker)
+			alt[n] = (((E.ox + pi[n]) / E.check_size) % 2) != (((E.oy + pj[n]) / E.check_size) % 2);
+		bkalpha[n] = E.bk_has_trans ? (alt[n] ? E.bk2a : E.bk1a) : 0.0;
+	}
+#pragma unroll
+	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
+		if (idx >= E.nch) break;
+		const DevOChan &q = E.q[idx];
+#pragma unroll
+		for (int n = 0; n < N; n++) {
+			double x = gather(n, E.c[idx]);
+			if (E.intclamp) x = clamp01(x);
+			const bool is_alpha = E.is_alpha[idx];
+			if (is_alpha) alpha_f[n] = (float)x;
+			if (E.unassoc[idx]) {
+				const double a = (double)alpha_f[n];
+				if (a != 0.0) x /= a;
+				if (E.late_bk) {
+					const double bk = alt[n] ? q.bk2 : q.bk1;
+					if (E.bk_has_trans) x = x * a + bk * bkalpha[n] * (1.0 - a);
+					else x = x * a + bk * (1.0 - a);
+				}
+			} else if (is_alpha && E.bk_has_trans) {
+				x = x + bkalpha[n] * (1.0 - x);
+			}
+			v[idx][n] = clamp01(x);
+		}
+	}
+	int lo[IWAMD_MAX_CH][N], cnt[IWAMD_MAX_CH][N];
+#pragma unroll
+	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
+		if (idx >= E.nch) break;
+		if (E.o[idx] < 0 || E.ed[idx]) continue;
+		const g_cu16 *ix = (const g_cu16 *)E.six[idx];
+#pragma unroll
+		for (int n = 0; n < N; n++) {
+			const int b = qbin(E.sL[idx], v[idx][n]);
+			lo[idx][n] = ix[b];
+			cnt[idx][n] = ix[b + 1];
+		}
+	}
+	double a0[IWAMD_MAX_CH][N], a1[IWAMD_MAX_CH][N];
+#pragma unroll
+	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
+		if (idx >= E.nch) break;
+		if (E.o[idx] < 0 || E.ed[idx]) continue;
+		const g_cf64 *a = (const g_cf64 *)E.sa[idx];
+		const int last = E.sn[idx] - 1;
+#pragma unroll
+		for (int n = 0; n < N; n++) {
+			a0[idx][n] = a[min(lo[idx][n], last)];
+			a1[idx][n] = a[min(lo[idx][n] + 1, last)];
+		}
+	}
+#pragma unroll
+	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
+		if (idx >= E.nch) break;
+		const int o = E.o[idx];
+		if (o < 0) continue;
+#pragma unroll
+		for (int n = 0; n < N; n++) {
+			if (!live[n]) continue;
+			const double x = v[idx][n];
+			if (E.ed[idx]) {
+				((__attribute__((address_space(1))) double *)E.ed_samples)[((size_t)E.q[idx].ed_index * E.out_h + pj[n]) *
+					E.out_w + pi[n]] = x;
+				continue;
+			}
+			const int l = lo[idx][n], m = cnt[idx][n] - l;
+			int k = l + (m > 0 && a0[idx][n] <= x) + (m > 1 && a1[idx][n] <= x);
+			if (m > 2 || !(x >= 0.0)) k = count_le(E.sa[idx], E.sn[idx], E.six[idx], E.sL[idx], x);
+			emit_store(E, pi[n], pj[n], o, E.slevel[idx] ? (double)k : ((const g_cf64 *)E.scode[idx])[k]);
+		}
 	}
 }
 
@@ -981,9 +1111,21 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, L
 	const int rows = min(FAST_ROWS, P.Hp - w.j0);
 	EmitCtx E;
 	emit_ctx(P, E);
-	for (int t = tid; t < rows * w.ni; t += blockDim.x) {
-		const int r = t / w.ni, ii = t % w.ni;
-		emit_px(E, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
+	const int npx = rows * w.ni;
+	if (E.simple) {
+		// Two pixels per thread at once.
+		for (int t = tid; t < npx; t += 2 * blockDim.x) {
+			const int t1 = t + (int)blockDim.x < npx ? t + (int)blockDim.x : t;
+			const int r[2] = {t / w.ni, t1 / w.ni}, ii[2] = {t % w.ni, t1 % w.ni};
+			const int pi[2] = {w.i0 + ii[0], w.i0 + ii[1]}, pj[2] = {w.j0 + r[0], w.j0 + r[1]};
+			const bool live[2] = {true, t1 != t};
+			emit_px_simple<2>(E, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
+		}
+	} else {
+		for (int t = tid; t < npx; t += blockDim.x) {
+			const int r = t / w.ni, ii = t % w.ni;
+			emit_px(E, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
+		}
 	}
 }
 
@@ -1841,7 +1983,7 @@ namespace {
 // Host side of one job's device data: the tables, where each lives in the
 // job's part of a device block, and the resampling path.
 struct WOff { size_t start, src, w, copy, dfirst, dcount, doff, dw; };
-struct QOff { size_t T, U, lin, code, nc, ix, prow, pcol, M, pre, suf; };
+struct QOff { size_t T, U, lin, code, nc, ix, D, prow, pcol, M, pre, suf; };
 
 struct Staged {
 	std::vector<iwx::WeightList> wv, wh;
@@ -2064,7 +2206,8 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	S.pt.resize(p.out_nch);
 	if (p.out_sampletype != IW_SAMPLETYPE_FLOATINGPOINT) {
 		for (int o = 0; o < p.out_nch; o++) {
-			S.qt[o] = build_quant_table(p.och[o], p.och[o].use_nc_table != 0);
+			S.qt[o] = build_quant_table(p.och[o], p.och[o].use_nc_table != 0,
+				p.och[o].ditherfamily == IW_DITHERFAMILY_NONE);
 			if (p.och[o].ditherfamily == IW_DITHERFAMILY_RANDOM) {
 				S.pt[o] = build_prng_tables(p.och[o].random_seed, p.out_width, p.out_height);
 				if (!S.pt[o].ok) {
@@ -2159,6 +2302,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 		q.code = L.add<double>(S.qt[o].code.size());
 		q.nc = L.add<double>(S.qt[o].nc.size());
 		q.ix = L.add<uint16_t>(S.qt[o].ix.size());
+		q.D = L.add<double>(S.qt[o].D.size());
 		q.prow = L.add<uint64_t>(S.pt[o].row_q.size());
 		q.pcol = L.add<uint64_t>(S.pt[o].col_mont.size());
 		q.M = L.add<double>(S.et[o].M.size());
@@ -2205,6 +2349,7 @@ bool upload_job(const Staged &S, const iwamd_plan &p, char *base, hipStream_t st
 		up(q.code, S.qt[o].code.data(), S.qt[o].code.size() * 8);
 		up(q.nc, S.qt[o].nc.data(), S.qt[o].nc.size() * 8);
 		up(q.ix, S.qt[o].ix.data(), S.qt[o].ix.size() * 2);
+		up(q.D, S.qt[o].D.data(), S.qt[o].D.size() * 8);
 		up(q.prow, S.pt[o].row_q.data(), S.pt[o].row_q.size() * 8);
 		up(q.pcol, S.pt[o].col_mont.data(), S.pt[o].col_mont.size() * 8);
 		up(q.M, S.et[o].M.data(), S.et[o].M.size() * 8);
@@ -2305,6 +2450,8 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 			d.code = (const double *)(base + q.code);
 			d.nc = (const double *)(base + q.nc);
 			d.ix = S.qt[o].ix.empty() ? nullptr : (const unsigned short *)(base + q.ix);
+			d.D = S.qt[o].D.empty() ? nullptr : (const double *)(base + q.D);
+			d.code_is_level = S.qt[o].code_is_level ? 1 : 0;
 			d.ix_log2 = S.qt[o].ix_log2;
 			d.prng_row = (const uint64_t *)(base + q.prow);
 			d.prng_col = (const uint64_t *)(base + q.pcol);
@@ -2934,6 +3081,25 @@ extern "C" int iwamd_ed_fast_check(const struct iwamd_ochan *och, double sL, dou
 		    f))
 		return 0;
 	for (int i = 0; i < n; i++) err[i] = ed_fast_err(f, s[i]);
+	return 1;
+}
+
+// The no-dither quantiser as the kernels run it, on the host, for tests: the
+// code of each sample from the folded boundaries QuantTable::D through the
+// bin index (qbin, then the bin's entries), as count_le does on the device.
+extern "C" int iwamd_quant_fold_check(const struct iwamd_ochan *och, int n, const double *s, double *code)
+{
+	if (och->use_nc_table) return -1;
+	const iwx::QuantTable q = iwx::build_quant_table(*och, false, true);
+	if (q.ix.empty()) return 0;
+	const int NB = 1 << q.ix_log2;
+	for (int i = 0; i < n; i++) {
+		const double f = s[i] * (double)NB;
+		const int b = f >= (double)(NB - 1) ? NB - 1 : (int)f;
+		const int lo = q.ix[b], hi = q.ix[b + 1];
+		const int k = (int)(std::upper_bound(q.D.begin() + lo, q.D.begin() + hi, s[i]) - q.D.begin());
+		code[i] = q.code[k];
+	}
 	return 1;
 }
 

@@ -281,7 +281,21 @@ const std::pair<std::vector<double>, std::vector<double>> &thresholds(const stru
 	return g_thresh_cache.emplace(key, std::make_pair(std::move(T), std::move(U))).first->second;
 }
 
-QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc)
+double round_up_point(double lo, double hi)
+{
+	auto ceil_wins = [&](double s) { return (hi - s) <= (s - lo); };
+	if (ceil_wins(0.0)) return 0.0;
+	if (!ceil_wins(1.0)) return 2.0;
+	uint64_t a = 0, b = dbits(1.0);  // !ceil_wins(a), ceil_wins(b)
+	while (b - a > 1) {
+		const uint64_t mid = a + (b - a) / 2;
+		if (ceil_wins(bitsd(mid))) b = mid;
+		else a = mid;
+	}
+	return bitsd(b);
+}
+
+QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc, bool fold_none)
 {
 	QuantTable q;
 	const double M = oc.maxcolorcode_dbl;
@@ -312,8 +326,20 @@ QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc)
 			prev = cur;
 		}
 	}
+	q.code_is_level = true;
+	for (int k = 0; k <= P; k++)
+		if (q.code[k] != (double)k) q.code_is_level = false;
+	if (fold_none && !want_nc) {
+		q.D.resize(P);
+		for (int k = 0; k < P; k++) {
+			const double m = round_up_point(q.lin[k], q.lin[k + 1]);
+			double d = std::max(q.U[k], m);
+			if (k > 0) d = std::max(d, q.T[k - 1]);
+			q.D[k] = std::min(q.T[k], d);
+		}
+	}
 	// Bins: about eight per searched entry, 2^11..2^16 of them.
-	const std::vector<double> &a = want_nc ? q.nc : q.T;
+	const std::vector<double> &a = want_nc ? q.nc : (fold_none ? q.D : q.T);
 	const int n = want_nc ? (int)q.nc.size() : P;
 	if (n > 65535) return q;  // counts would not fit: no index, full search
 	int L = 11;
@@ -367,19 +393,7 @@ EdTables build_ed_tables(const QuantTable &q)
 	t.eb.assign(P + 1, 0.0);
 	for (int k = 0; k < P; k++) {
 		const double lo = q.lin[k], hi = q.lin[k + 1];
-		auto ceil_wins = [&](double s) { return (hi - s) <= (s - lo); };
-		double m = 2.0;
-		if (ceil_wins(0.0)) {
-			m = 0.0;
-		} else if (ceil_wins(1.0)) {
-			uint64_t a = 0, b = dbits(1.0);  // !ceil_wins(a), ceil_wins(b)
-			while (b - a > 1) {
-				const uint64_t mid = a + (b - a) / 2;
-				if (ceil_wins(bitsd(mid))) b = mid;
-				else a = mid;
-			}
-			m = bitsd(b);
-		}
+		const double m = round_up_point(lo, hi);
 		t.M[k] = m;
 		// Non-exact samples of level k: [U[k], T[k]).
 		const double u = q.U[k], tk = q.T[k];

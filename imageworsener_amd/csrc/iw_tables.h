@@ -62,6 +62,14 @@ struct QuantTable {
 	int P = 0;                   // number of levels - 1
 	std::vector<double> T, U, lin, code;
 	std::vector<double> nc;      // nearest-colour midpoints (255), when used
+	// The no-dither quantiser in one search (src/imagew-main.c:669-736):
+	// level k = #{T <= s}; an exact sample (s < U[k]) keeps code[k], another
+	// rounds up iff (lin[k+1] - s) <= (s - lin[k]), which holds exactly for
+	// s >= M[k] (round_up_point).  So the code is code[#{k : D[k] <= s}] with
+	// D[k] = min(T[k], max(T[k-1], U[k], M[k])) -- each D[k] lies in
+	// [T[k-1], T[k]], so D is ascending and counts only its own level's step.
+	std::vector<double> D;       // when folded (P entries)
+	bool code_is_level = false;  // code[k] == k for every k
 	// Bin index over the array the quantiser searches (nc when used, else T),
 	// 2^ix_log2 bins of [0, 1]: ix[b] = #{m : a[m] < b / 2^ix_log2} for
 	// b < 2^ix_log2, and the array's length last.  A sample s in bin b counts
@@ -71,7 +79,9 @@ struct QuantTable {
 	int ix_log2 = 0;
 };
 
-QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc);
+// want_nc: nearest-colour midpoints.  fold_none (no dither, no nc): the
+// searched array is QuantTable::D below instead of T.
+QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc, bool fold_none = false);
 
 // Decision data for error diffusion over a QuantTable (src/imagew-main.c:
 // 730-748).  A non-exact sample s at level k is rounded up iff
@@ -89,6 +99,10 @@ struct EdTables {
 };
 
 EdTables build_ed_tables(const QuantTable &q);
+
+// The smallest s in [0, 1] with (hi - s) <= (s - lo) in FP64 (2: none; the
+// predicate is monotone in s).
+double round_up_point(double lo, double hi);
 
 // (T, U) for a transfer function and level count; cached per process.
 const std::pair<std::vector<double>, std::vector<double>> &thresholds(const struct iw_csdescr &cs, int P);

@@ -174,6 +174,11 @@ IW_EXPORT(int) iwamd_tune_entries(struct iwamd_tune_entry *out, int max);
  * src/imagew-main.c:712-748. */
 IW_EXPORT(int) iwamd_ed_fast_check(const struct iwamd_ochan *och, double sL, double sH, int n, const double *s,
 	double *err);
+/* The no-dither quantiser for samples s[0..n) in [0, 1] as the kernels run
+ * it (folded boundaries, bin index): code[i].  Returns 1, -1 for a
+ * nearest-colour plan.  No counterpart in the reference: it checks the
+ * device's restatement of src/imagew-main.c:669-736 with no dither. */
+IW_EXPORT(int) iwamd_quant_fold_check(const struct iwamd_ochan *och, int n, const double *s, double *code);
 
 /* Device-resident jobs.  iwamd_job_create() runs the host decisions on ctx
  * (consuming its single use), builds the device tables and uploads the input

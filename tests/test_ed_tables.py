@@ -159,3 +159,47 @@ def test_ed_fast_form(prod, name, cstype, gamma, maxcc, cc):
                 want, _ = _ref_error(s, _level(s, T), U, lin)
                 assert got == want, (name, k, sL, sH, s, got, want)
     assert accepted > windows // 2
+
+
+def _ref_code(s, P, T, U, lin, code):
+    """The reference's no-dither output code for linear sample s
+    (src/imagew-main.c:669-736): the floor level, kept when exact, else the
+    nearer of the two candidates in linear light (ties up)."""
+    k = _level(s, T)
+    if k >= P or s < U[k]:
+        return code[k]
+    return code[k + 1] if (lin[k + 1] - s) <= (s - lin[k]) else code[k]
+
+
+@pytest.mark.parametrize("name,cstype,gamma,maxcc,cc", CASES, ids=[c[0] for c in CASES])
+def test_quant_fold(prod, name, cstype, gamma, maxcc, cc):
+    """The kernels' no-dither quantiser -- one search over the folded
+    boundaries D[k] = min(T[k], max(T[k-1], U[k], M[k])) through the bin
+    index -- gives the reference's code at every threshold, exactness bound
+    and round-up point and the doubles beside them, and at random samples."""
+    P, T, U, lin, M, eb = _tables(prod, cstype, gamma, maxcc, cc)
+    code = [float(k) if not cc else float(int(0.5000000001 + k * (maxcc / P))) for k in range(P + 1)]
+    lay = api._PlanLayout.get()
+    och = lay.OChan()
+    och.ditherfamily, och.dithersubtype = api.DITHER["none"]
+    och.color_count = cc
+    och.maxcolorcode_int = maxcc
+    och.maxcolorcode_dbl = float(maxcc)
+    och.out_cs = api.CsDescr(cstype, 0, gamma)
+    fn = prod.dll.iwamd_quant_fold_check
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    fn.restype = ctypes.c_int
+    rng = random.Random(13)
+    levels = range(P + 1) if P < 1000 else list(range(64)) + rng.sample(range(64, P - 64), 300) + list(range(P - 64, P + 1))
+    pts = [0.0, 1.0, _prev(1.0)] + [rng.random() for _ in range(4000)]
+    for k in levels:
+        for th in ([T[k]] if k < P else []) + [U[k], M[k]]:
+            if 0.0 <= th <= 1.0:
+                pts += [v for v in (_prev(th), th, _next(th)) if 0.0 <= v <= 1.0]
+    n = len(pts)
+    s_arr = (ctypes.c_double * n)(*pts)
+    c_arr = (ctypes.c_double * n)()
+    assert fn(ctypes.byref(och), n, s_arr, c_arr) == 1
+    bad = [(s, c_arr[i], _ref_code(s, P, T, U, lin, code)) for i, s in enumerate(pts)
+           if c_arr[i] != _ref_code(s, P, T, U, lin, code)]
+    assert not bad, bad[:5]
