@@ -410,14 +410,6 @@ struct EmitCtx {
 	double *ed_samples;
 	int c[IWAMD_MAX_CH], o[IWAMD_MAX_CH], unassoc[IWAMD_MAX_CH], is_alpha[IWAMD_MAX_CH];
 	DevOChan q[IWAMD_MAX_CH];
-	// simple: every output channel is quantised without dither through one
-	// binned search (nearest-colour midpoints or QuantTable::D) or handed to
-	// error diffusion; sa/sn/six/sL: the searched array, its length and bin
-	// index; code: count (slevel) or scode[count].
-	int simple;
-	int ed[IWAMD_MAX_CH], sn[IWAMD_MAX_CH], sL[IWAMD_MAX_CH], slevel[IWAMD_MAX_CH];
-	const double *sa[IWAMD_MAX_CH], *scode[IWAMD_MAX_CH];
-	const unsigned short *six[IWAMD_MAX_CH];
 };
 
 __device__ __forceinline__ void emit_ctx(const DevPlan &P, EmitCtx &E)
@@ -450,22 +442,56 @@ __device__ __forceinline__ void emit_ctx(const DevPlan &P, EmitCtx &E)
 		E.is_alpha[idx] = c == P.inter_alpha;
 		E.q[idx] = P.oc[E.o[idx] >= 0 ? E.o[idx] : 0];
 	}
-	E.simple = !P.out_float;
+}
+
+// The per-pixel work's plan fields for the common case -- integer output, no
+// checkerboard or late background, every output channel either quantised
+// without dither through one binned search (nearest-colour midpoints or
+// QuantTable::D) or handed to error diffusion -- packed small so that a pixel
+// loop keeps them in scalar registers.  Slot idx is the idx-th channel in
+// processing order.  f: channel (bits 0-1), output channel + 1 (2-4; 0:
+// none), unassociate (5), is alpha (6), error diffusion (7), code = count (8),
+// bin-index log2 (9-13).  n: searched entries, or the plane index for error
+// diffusion.
+struct EmitSimple {
+	int nch, intclamp, out_depth, out_nch, out_w, out_h;
+	size_t out_bpr;
+	uint8_t *out;
+	double *ed_samples;
+	int f[IWAMD_MAX_CH], n[IWAMD_MAX_CH];
+	const double *sa[IWAMD_MAX_CH], *scode[IWAMD_MAX_CH];
+	const unsigned short *six[IWAMD_MAX_CH];
+};
+
+__device__ __forceinline__ bool emit_simple_ctx(const DevPlan &P, EmitSimple &E)
+{
+	if (P.out_float || P.checker || (P.apply_bkgd && P.late) || P.bk_has_trans) return false;
+	E.nch = P.inter_nch;
+	E.intclamp = P.intclamp;
+	E.out_depth = P.out_depth;
+	E.out_nch = P.out_nch;
+	E.out_w = P.out_w;
+	E.out_h = P.out_h;
+	E.out_bpr = P.out_bpr;
+	E.out = P.out;
+	E.ed_samples = P.ed_samples;
+	bool ok = true;
 #pragma unroll
 	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
-		const DevOChan &q = E.q[idx];
-		const bool nc = q.qmode == Q_NC;
-		E.ed[idx] = q.family == IW_DITHERFAMILY_ERRDIFF;
+		const int c = idx < P.inter_nch ? P.order[idx] : 0;
+		const DevIChan &ic = P.ic[c];
+		const int o = idx < P.inter_nch ? ic.output_channel : -1;
+		const DevOChan &q = P.oc[o >= 0 ? o : 0];
+		const bool nc = q.qmode == Q_NC, ed = q.family == IW_DITHERFAMILY_ERRDIFF;
 		E.sa[idx] = nc ? q.nc : q.D;
-		E.sn[idx] = nc ? 255 : q.P;
-		E.six[idx] = q.ix;
-		E.sL[idx] = q.ix_log2;
 		E.scode[idx] = q.code;
-		E.slevel[idx] = nc || q.code_is_level;
-		if (idx < E.nch && E.o[idx] >= 0 && !E.ed[idx] &&
-			!(q.family == IW_DITHERFAMILY_NONE && q.ix && E.sa[idx] && E.sn[idx] > 0))
-			E.simple = 0;
+		E.six[idx] = q.ix;
+		E.n[idx] = ed ? q.ed_index : (nc ? 255 : q.P);
+		E.f[idx] = c | ((o + 1) << 2) | (ic.need_unassoc ? 1 << 5 : 0) | (c == P.inter_alpha ? 1 << 6 : 0) |
+			(ed ? 1 << 7 : 0) | ((nc || q.code_is_level) ? 1 << 8 : 0) | (q.ix_log2 << 9);
+		if (o >= 0 && !ed && !(q.family == IW_DITHERFAMILY_NONE && q.ix && E.sa[idx] && E.n[idx] > 0)) ok = false;
 	}
+	return ok;
 }
 
 __device__ __forceinline__ void emit_store(const EmitCtx &E, int i, int j, int o, double s_full)
@@ -529,96 +555,93 @@ __device__ __forceinline__ void emit_px(const EmitCtx &E, int i, int j, Gather g
 	}
 }
 
-// emit_px for N pixels at once under EmitCtx::simple.  The table searches of
+// emit_px for N pixels at once under EmitSimple.  The table searches of
 // every channel and pixel go in stages -- all bin-index loads, then all
 // entry loads, then the codes -- so their latencies overlap instead of
 // adding up.  A bin holds at most two entries for 8-bit tables; a larger one
 // (or a sample outside [0, 1]) is searched by count_le.  gather(n, c):
 // channel c of pixel n; pixels with live[n] false are computed, not stored.
 template <int N, class Gather>
-__device__ __forceinline__ void emit_px_simple(const EmitCtx &E, const int (&pi)[N], const int (&pj)[N],
+__device__ __forceinline__ void emit_px_simple(const EmitSimple &E, const int (&pi)[N], const int (&pj)[N],
 	const bool (&live)[N], Gather gather)
 {
 	double v[IWAMD_MAX_CH][N];
 	float alpha_f[N];
-	int alt[N];
-	double bkalpha[N];
 #pragma unroll
-	for (int n = 0; n < N; n++) {
-		alpha_f[n] = 0.0f;
-		alt[n] = 0;
-		if (E.checker)
-			alt[n] = (((E.ox + pi[n]) / E.check_size) % 2) != (((E.oy + pj[n]) / E.check_size) % 2);
-		bkalpha[n] = E.bk_has_trans ? (alt[n] ? E.bk2a : E.bk1a) : 0.0;
-	}
+	for (int n = 0; n < N; n++) alpha_f[n] = 0.0f;
 #pragma unroll
 	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
 		if (idx >= E.nch) break;
-		const DevOChan &q = E.q[idx];
+		const int f = E.f[idx];
 #pragma unroll
 		for (int n = 0; n < N; n++) {
-			double x = gather(n, E.c[idx]);
+			double x = gather(n, f & 3);
 			if (E.intclamp) x = clamp01(x);
-			const bool is_alpha = E.is_alpha[idx];
-			if (is_alpha) alpha_f[n] = (float)x;
-			if (E.unassoc[idx]) {
+			if (f & (1 << 6)) alpha_f[n] = (float)x;
+			if (f & (1 << 5)) {
 				const double a = (double)alpha_f[n];
 				if (a != 0.0) x /= a;
-				if (E.late_bk) {
-					const double bk = alt[n] ? q.bk2 : q.bk1;
-					if (E.bk_has_trans) x = x * a + bk * bkalpha[n] * (1.0 - a);
-					else x = x * a + bk * (1.0 - a);
-				}
-			} else if (is_alpha && E.bk_has_trans) {
-				x = x + bkalpha[n] * (1.0 - x);
 			}
 			v[idx][n] = clamp01(x);
 		}
 	}
-	int lo[IWAMD_MAX_CH][N], cnt[IWAMD_MAX_CH][N];
+	int lo[IWAMD_MAX_CH][N], hi[IWAMD_MAX_CH][N];
 #pragma unroll
 	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
 		if (idx >= E.nch) break;
-		if (E.o[idx] < 0 || E.ed[idx]) continue;
+		const int f = E.f[idx];
+		if (!(f >> 2 & 7) || (f & (1 << 7))) continue;
 		const g_cu16 *ix = (const g_cu16 *)E.six[idx];
 #pragma unroll
 		for (int n = 0; n < N; n++) {
-			const int b = qbin(E.sL[idx], v[idx][n]);
+			const int b = qbin(f >> 9 & 31, v[idx][n]);
 			lo[idx][n] = ix[b];
-			cnt[idx][n] = ix[b + 1];
+			hi[idx][n] = ix[b + 1];
 		}
 	}
 	double a0[IWAMD_MAX_CH][N], a1[IWAMD_MAX_CH][N];
 #pragma unroll
 	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
 		if (idx >= E.nch) break;
-		if (E.o[idx] < 0 || E.ed[idx]) continue;
+		const int f = E.f[idx];
+		if (!(f >> 2 & 7) || (f & (1 << 7))) continue;
 		const g_cf64 *a = (const g_cf64 *)E.sa[idx];
-		const int last = E.sn[idx] - 1;
+		const int last = E.n[idx] - 1;
 #pragma unroll
 		for (int n = 0; n < N; n++) {
 			a0[idx][n] = a[min(lo[idx][n], last)];
 			a1[idx][n] = a[min(lo[idx][n] + 1, last)];
 		}
 	}
+	g_u8 *px[N];
+#pragma unroll
+	for (int n = 0; n < N; n++)
+		px[n] = (g_u8 *)E.out + (size_t)pj[n] * E.out_bpr + (size_t)E.out_nch * pi[n] * (E.out_depth == 8 ? 1 : 2);
 #pragma unroll
 	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
 		if (idx >= E.nch) break;
-		const int o = E.o[idx];
+		const int f = E.f[idx], o = (f >> 2 & 7) - 1;
 		if (o < 0) continue;
 #pragma unroll
 		for (int n = 0; n < N; n++) {
 			if (!live[n]) continue;
 			const double x = v[idx][n];
-			if (E.ed[idx]) {
-				((__attribute__((address_space(1))) double *)E.ed_samples)[((size_t)E.q[idx].ed_index * E.out_h + pj[n]) *
+			if (f & (1 << 7)) {
+				((__attribute__((address_space(1))) double *)E.ed_samples)[((size_t)E.n[idx] * E.out_h + pj[n]) *
 					E.out_w + pi[n]] = x;
 				continue;
 			}
-			const int l = lo[idx][n], m = cnt[idx][n] - l;
+			const int l = lo[idx][n], m = hi[idx][n] - l;
 			int k = l + (m > 0 && a0[idx][n] <= x) + (m > 1 && a1[idx][n] <= x);
-			if (m > 2 || !(x >= 0.0)) k = count_le(E.sa[idx], E.sn[idx], E.six[idx], E.sL[idx], x);
-			emit_store(E, pi[n], pj[n], o, E.slevel[idx] ? (double)k : ((const g_cf64 *)E.scode[idx])[k]);
+			if (m > 2 || !(x >= 0.0)) k = count_le(E.sa[idx], E.n[idx], E.six[idx], f >> 9 & 31, x);
+			const double code = (f & (1 << 8)) ? (double)k : ((const g_cf64 *)E.scode[idx])[k];
+			if (E.out_depth == 8) {
+				px[n][o] = (uint8_t)(0.5 + code);
+			} else {
+				const unsigned int u = (unsigned short)(0.5 + code);
+				px[n][2 * o] = (uint8_t)(u >> 8);
+				px[n][2 * o + 1] = (uint8_t)(u & 0xff);
+			}
 		}
 	}
 }
@@ -857,6 +880,7 @@ struct FastCin {
 	int et[IWAMD_MAX_CH];    // rows outside the image read edge[c]
 	int ab;                  // byte of the alpha sample
 	const double *alut;      // its table (channel 0's when alpha is not needed)
+	int lean;                // no channel in mode 2 and none with a transparent edge
 };
 
 __device__ __forceinline__ int fast_cin_depth(const DevPlan &P, FastCin &F)
@@ -875,6 +899,9 @@ __device__ __forceinline__ int fast_cin_depth(const DevPlan &P, FastCin &F)
 		F.et[c] = ic.v.edge_transparent;
 		need_a |= F.mode[c] != 0;
 	}
+	F.lean = 1;
+	for (int c = 0; c < IWAMD_MAX_CH; c++)
+		if (F.mode[c] == 2 || F.et[c]) F.lean = 0;
 	F.ab = F.chb[0];
 	F.alut = F.lut[0];
 	if (need_a) {
@@ -904,8 +931,10 @@ __device__ __forceinline__ unsigned int fast_raw(const g_u8 *in, long long off)
 
 // Intermediate channels 0..C-1 of logical input (xc, p), xc in [0, W): the
 // band's sample, as band_sample(P, P.ic[c], xc, p, P.ih) would give it
-// (channels from C on are left unspecified).
-template <int DEPTH>
+// (channels from C on are left unspecified).  LEAN: no early background and
+// no transparent edge (FastCin::lean), so a channel is its table value times
+// alpha or times 1 (which leaves it unchanged).
+template <int DEPTH, bool LEAN>
 __device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, long long cb, int p,
 	float v[IWAMD_MAX_CH])
 {
@@ -927,6 +956,10 @@ __device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, l
 #pragma unroll
 	for (int c = 0; c < IWAMD_MAX_CH; c++) {
 		const double x = ((const g_cf64 *)F.lut[c])[raw[c]];
+		if (LEAN) {
+			v[c] = (float)(x * (F.mode[c] ? a : 1.0));
+			continue;
+		}
 		const double xm = x * a;
 		const double xb = a * x + (1.0 - a) * F.bkgd[c];
 		double y = F.mode[c] == 1 ? xm : (F.mode[c] == 2 ? xb : x);
@@ -938,7 +971,7 @@ __device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, l
 // Vertical pass of a tile: 16-column groups of the intermediate, one per wave
 // in turn; each lane fetches its own B operands (rows kk + 4*lq .. +3 of its
 // column), so no sample is read twice within a group.
-template <int DEPTH, class Lds>
+template <int DEPTH, bool LEAN, class Lds>
 __device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &w, Lds &S, const FastCin &F)
 {
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -964,7 +997,7 @@ __device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &
 			const half4_t al = *(const half4_t *)&S.wv[1][lr][kk + 4 * lq];
 			float v[4][IWAMD_MAX_CH];
 #pragma unroll
-			for (int t = 0; t < 4; t++) fast_fetch<DEPTH>(P, F, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+			for (int t = 0; t < 4; t++) fast_fetch<DEPTH, LEAN>(P, F, cb, w.p_lo + kk + 4 * lq + t, v[t]);
 			half4_t bh[IWAMD_MAX_CH], bl[IWAMD_MAX_CH];
 #pragma unroll
 			for (int t = 0; t < 4; t++) {
@@ -1040,9 +1073,15 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, L
 
 	FastCin F;
 	switch (fast_cin_depth(P, F)) {
-	case 8: fast_vertical<8, Lds>(P, w, S, F); break;
-	case 16: fast_vertical<16, Lds>(P, w, S, F); break;
-	default: fast_vertical<0, Lds>(P, w, S, F); break;
+	case 8:
+		if (F.lean) fast_vertical<8, true, Lds>(P, w, S, F);
+		else fast_vertical<8, false, Lds>(P, w, S, F);
+		break;
+	case 16:
+		if (F.lean) fast_vertical<16, true, Lds>(P, w, S, F);
+		else fast_vertical<16, false, Lds>(P, w, S, F);
+		break;
+	default: fast_vertical<0, false, Lds>(P, w, S, F); break;
 	}
 	__syncthreads();
 
@@ -1109,19 +1148,20 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, L
 
 	// Per-pixel work in FP64, as in the plain form.
 	const int rows = min(FAST_ROWS, P.Hp - w.j0);
-	EmitCtx E;
-	emit_ctx(P, E);
 	const int npx = rows * w.ni;
-	if (E.simple) {
+	EmitSimple ES;
+	if (emit_simple_ctx(P, ES)) {
 		// Two pixels per thread at once.
 		for (int t = tid; t < npx; t += 2 * blockDim.x) {
 			const int t1 = t + (int)blockDim.x < npx ? t + (int)blockDim.x : t;
 			const int r[2] = {t / w.ni, t1 / w.ni}, ii[2] = {t % w.ni, t1 % w.ni};
 			const int pi[2] = {w.i0 + ii[0], w.i0 + ii[1]}, pj[2] = {w.j0 + r[0], w.j0 + r[1]};
 			const bool live[2] = {true, t1 != t};
-			emit_px_simple<2>(E, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
+			emit_px_simple<2>(ES, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
 		}
 	} else {
+		EmitCtx E;
+		emit_ctx(P, E);
 		for (int t = tid; t < npx; t += blockDim.x) {
 			const int r = t / w.ni, ii = t % w.ni;
 			emit_px(E, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
