@@ -748,27 +748,41 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 	__syncthreads();
 	// Horizontal pass and per-pixel work for the band's rows.
 	const int n_px = P.out_w * rows;
-	EmitCtx E;
-	emit_ctx(P, E);
-	for (int t = tid; t < n_px; t += blockDim.x) {
-		const int r = t / P.out_w, i = t % P.out_w;
-		const int j = j0 + r;
-		emit_px(E, i, j, [&](int c) {
-			const DevWeights &h = P.ic[c].h;
-			const float *row = inter + ((size_t)c * R + r) * W;
-			if (h.mode == iwx::WeightList::COPY) {
-				const int s = h.copy_src[i];
-				return (s < 0) ? 0.0 : (double)row[s];
-			}
-			double acc = 0.0;
-			const int t1 = h.start[i + 1];
-			for (int u = h.start[i]; u < t1; u++) {
-				const int s = h.src[u];
-				const double v = (s < 0) ? h.edge : (double)row[s];
-				acc = acc + v * h.w[u];
-			}
-			return acc;
-		});
+	auto hsum = [&](int c, int r, int i) {
+		const DevWeights &h = P.ic[c].h;
+		const float *row = inter + ((size_t)c * R + r) * W;
+		if (h.mode == iwx::WeightList::COPY) {
+			const int s = h.copy_src[i];
+			return (s < 0) ? 0.0 : (double)row[s];
+		}
+		const g_cint *start = (const g_cint *)h.start, *src = (const g_cint *)h.src;
+		const g_cf64 *hw = (const g_cf64 *)h.w;
+		double acc = 0.0;
+		const int t1 = start[i + 1];
+		for (int u = start[i]; u < t1; u++) {
+			const int s = src[u];
+			const double v = (s < 0) ? h.edge : (double)row[s];
+			acc = acc + v * hw[u];
+		}
+		return acc;
+	};
+	EmitSimple ES;
+	if (emit_simple_ctx(P, ES)) {
+		// Two pixels per thread at once.
+		for (int t = tid; t < n_px; t += 2 * blockDim.x) {
+			const int t1 = t + (int)blockDim.x < n_px ? t + (int)blockDim.x : t;
+			const int r[2] = {t / P.out_w, t1 / P.out_w}, pi[2] = {t % P.out_w, t1 % P.out_w};
+			const int pj[2] = {j0 + r[0], j0 + r[1]};
+			const bool live[2] = {true, t1 != t};
+			emit_px_simple<2>(ES, pi, pj, live, [&](int n, int c) { return hsum(c, r[n], pi[n]); });
+		}
+	} else {
+		EmitCtx E;
+		emit_ctx(P, E);
+		for (int t = tid; t < n_px; t += blockDim.x) {
+			const int r = t / P.out_w, i = t % P.out_w;
+			emit_px(E, i, j0 + r, [&](int c) { return hsum(c, r, i); });
+		}
 	}
 }
 
