@@ -123,6 +123,7 @@ struct DevPlan {
 	int *host_flag;         // the same, set to 1 when host_out holds the output
 	size_t out_bytes;
 	int apply_bkgd, late, checker, check_size, ox, oy, bk_has_trans;
+	int v_shared;  // every channel has the same vertical weights (dense form) in WEIGHTS mode
 	double bk1a, bk2a;
 };
 
@@ -689,6 +690,167 @@ __device__ __forceinline__ double band_sample(const DevPlan &P, const DevIChan &
 	return colour_in(P, ic, x, p);
 }
 
+// Direct colour-in for integer inputs of 8 or 16 bits whose intermediate
+// channels each read one stored sample (no grey conversion, no virtual
+// channel): the logical-to-physical map is affine, byte offset
+// base + x*dx + y*dy for logical (x, y), and a pixel's alpha is looked up once
+// for all its channels.  The arithmetic is colour_in's, in the same order.
+// Every load is unconditional -- slots past the plan's channels repeat
+// channel 0 and an unneeded alpha reads channel 0 too -- so a lane issues all
+// of a row's loads before it waits for any.
+struct FastCin {
+	long long base, dx, dy;
+	const double *lut[IWAMD_MAX_CH];
+	double bkgd[IWAMD_MAX_CH], edge[IWAMD_MAX_CH];
+	int chb[IWAMD_MAX_CH];   // byte of each channel's sample within a pixel
+	int mode[IWAMD_MAX_CH];  // 0: as read, 1: times alpha, 2: early background
+	int et[IWAMD_MAX_CH];    // rows outside the image read edge[c]
+	int ab;                  // byte of the alpha sample
+	const double *alut;      // its table (channel 0's when alpha is not needed)
+	int lean;                // no channel in mode 2 and none with a transparent edge
+};
+
+__device__ __forceinline__ int fast_cin_depth(const DevPlan &P, FastCin &F)
+{
+	if (P.in_float || (P.bit_depth != 8 && P.bit_depth != 16)) return 0;
+	const int bps = P.bit_depth / 8, pb = P.nch_phys * bps;
+	bool need_a = false;
+	for (int c = 0; c < IWAMD_MAX_CH; c++) {
+		const DevIChan &ic = P.ic[c < P.inter_nch ? c : 0];
+		if (ic.cvt_gray || ic.input_channel >= P.nch_phys || !ic.lut[0]) return 0;
+		F.chb[c] = ic.input_channel * bps;
+		F.lut[c] = ic.lut[0];
+		F.mode[c] = ic.need_unassoc ? 1 : (ic.early_bkgd ? 2 : 0);
+		F.bkgd[c] = ic.bkgd;
+		F.edge[c] = ic.v.edge;
+		F.et[c] = ic.v.edge_transparent;
+		need_a |= F.mode[c] != 0;
+	}
+	F.lean = 1;
+	for (int c = 0; c < IWAMD_MAX_CH; c++)
+		if (F.mode[c] == 2 || F.et[c]) F.lean = 0;
+	F.ab = F.chb[0];
+	F.alut = F.lut[0];
+	if (need_a) {
+		if (P.alpha_in_index >= P.nch_phys || !P.alpha_lut) return 0;
+		F.ab = P.alpha_in_index * bps;
+		F.alut = P.alpha_lut;
+	}
+	int rx, ry;
+	long long o[3];
+	const int px[3] = {0, 1, 0}, py[3] = {0, 0, 1};
+	for (int k = 0; k < 3; k++) {
+		logical_to_physical(P, px[k], py[k], rx, ry);
+		o[k] = (long long)ry * (long long)P.in_bpr + (long long)rx * pb;
+	}
+	F.base = o[0];
+	F.dx = o[1] - o[0];
+	F.dy = o[2] - o[0];
+	return P.bit_depth;
+}
+
+template <int DEPTH>
+__device__ __forceinline__ unsigned int fast_raw(const g_u8 *in, long long off)
+{
+	if (DEPTH == 8) return in[off];
+	return ((unsigned int)in[off] << 8) | in[off + 1];
+}
+
+// Intermediate channels 0..C-1 of logical input (xc, p), xc in [0, W): the
+// band's sample, as band_sample(P, P.ic[c], xc, p, P.ih) would give it
+// (channels from C on are left unspecified).  LEAN: no early background and
+// no transparent edge (FastCin::lean), so a channel is its table value times
+// alpha or times 1 (which leaves it unchanged).
+template <int DEPTH, bool LEAN, class T = float>
+__device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, long long cb, int p,
+	T v[IWAMD_MAX_CH])
+{
+	const int C = P.inter_nch;
+	if (DEPTH == 0) {
+		// cb holds the column here.
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++)
+			v[c] = c < C ? (T)band_sample(P, P.ic[c], (int)cb, p, P.ih) : (T)0;
+		return;
+	}
+	const int pin = p < 0 ? 0 : (p >= P.ih ? P.ih - 1 : p);
+	const long long off = cb + (long long)pin * F.dy;
+	const g_u8 *in = (const g_u8 *)P.in;
+	unsigned int raw[IWAMD_MAX_CH];
+#pragma unroll
+	for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = fast_raw<DEPTH>(in, off + F.chb[c]);
+	const double a = ((const g_cf64 *)F.alut)[fast_raw<DEPTH>(in, off + F.ab)];
+#pragma unroll
+	for (int c = 0; c < IWAMD_MAX_CH; c++) {
+		const double x = ((const g_cf64 *)F.lut[c])[raw[c]];
+		if (LEAN) {
+			v[c] = (T)(x * (F.mode[c] ? a : 1.0));
+			continue;
+		}
+		const double xm = x * a;
+		const double xb = a * x + (1.0 - a) * F.bkgd[c];
+		double y = F.mode[c] == 1 ? xm : (F.mode[c] == 2 ? xb : x);
+		if (p != pin && F.et[c]) y = F.edge[c];
+		v[c] = (T)y;
+	}
+}
+
+// The band's vertical pass when every channel shares one weight list
+// (DevPlan::v_shared) and the input takes the direct colour-in: each column's
+// pixels are read once for all channels, four input rows at a time with every
+// load issued before the first is used.  Each accumulator still gets its terms
+// in list order, with the same FP64 operations as colour_in and the generic
+// pass.
+template <int R, int DEPTH, bool LEAN>
+__device__ __forceinline__ void band_vertical_direct(const DevPlan &P, const FastCin &F, int j0, int rows,
+	const double *wd, float *inter)
+{
+	const int tid = threadIdx.x;
+	const int W = P.W, C = P.inter_nch;
+	const DevWeights &dv = P.ic[0].v;
+	int p_lo = 0x7fffffff, p_hi = -0x7fffffff;
+	for (int r = 0; r < rows; r++) {
+		const int n = dv.dcount[j0 + r];
+		if (n == 0) continue;
+		p_lo = min(p_lo, dv.dfirst[j0 + r]);
+		p_hi = max(p_hi, dv.dfirst[j0 + r] + n - 1);
+	}
+	const int K = (p_hi >= p_lo) ? p_hi - p_lo + 1 : 0;
+	for (int x = tid; x < W; x += blockDim.x) {
+		const long long cb = F.base + (long long)x * F.dx;
+		double acc[IWAMD_MAX_CH][R];
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++)
+#pragma unroll
+			for (int r = 0; r < R; r++) acc[c][r] = 0.0;
+		for (int k = 0; k < K; k += 4) {
+			double v[4][IWAMD_MAX_CH];
+#pragma unroll
+			for (int t = 0; t < 4; t++) fast_fetch<DEPTH, LEAN, double>(P, F, cb, p_lo + min(k + t, K - 1), v[t]);
+#pragma unroll
+			for (int t = 0; t < 4; t++) {
+				if (k + t >= K) break;
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++) {
+					if (c >= C) break;
+#pragma unroll
+					for (int r = 0; r < R; r++) acc[c][r] = acc[c][r] + v[t][c] * wd[r * BAND_MAX_K + k + t];
+				}
+			}
+		}
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) {
+			if (c >= C) break;
+#pragma unroll
+			for (int r = 0; r < R; r++) {
+				double a = acc[c][r];
+				if (P.intclamp) a = clamp01(a);
+				inter[((size_t)c * R + r) * W + x] = (float)a;
+			}
+		}
+	}
+}
+
 template <int R>
 __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned char *band_lds)
 {
@@ -699,7 +861,9 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 	const int W = P.W, C = P.inter_nch;
 	const int num_in = P.ih;
 
-	for (int c = 0; c < C; c++) {
+	FastCin F;
+	const int depth = P.v_shared ? fast_cin_depth(P, F) : 0;
+	for (int c = 0; c < (depth ? 1 : C); c++) {
 		const DevIChan &ic = P.ic[c];
 		const DevWeights &dv = ic.v;
 		// The band's input positions.
@@ -722,6 +886,17 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 			wd[t] = w;
 		}
 		__syncthreads();
+		if (depth) {
+			// Channel 0's weights are every channel's.
+			if (depth == 8) {
+				if (F.lean) band_vertical_direct<R, 8, true>(P, F, j0, rows, wd, inter);
+				else band_vertical_direct<R, 8, false>(P, F, j0, rows, wd, inter);
+			} else {
+				if (F.lean) band_vertical_direct<R, 16, true>(P, F, j0, rows, wd, inter);
+				else band_vertical_direct<R, 16, false>(P, F, j0, rows, wd, inter);
+			}
+			continue;
+		}
 		for (int x = tid; x < W; x += blockDim.x) {
 			double acc[R];
 #pragma unroll
@@ -875,111 +1050,6 @@ __device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo)
 __device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c)
 {
 	return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
-
-// Direct colour-in for integer inputs of 8 or 16 bits whose intermediate
-// channels each read one stored sample (no grey conversion, no virtual
-// channel): the logical-to-physical map is affine, byte offset
-// base + x*dx + y*dy for logical (x, y), and a pixel's alpha is looked up once
-// for all its channels.  The arithmetic is colour_in's, in the same order.
-// Every load is unconditional -- slots past the plan's channels repeat
-// channel 0 and an unneeded alpha reads channel 0 too -- so a lane issues all
-// of a row's loads before it waits for any.
-struct FastCin {
-	long long base, dx, dy;
-	const double *lut[IWAMD_MAX_CH];
-	double bkgd[IWAMD_MAX_CH], edge[IWAMD_MAX_CH];
-	int chb[IWAMD_MAX_CH];   // byte of each channel's sample within a pixel
-	int mode[IWAMD_MAX_CH];  // 0: as read, 1: times alpha, 2: early background
-	int et[IWAMD_MAX_CH];    // rows outside the image read edge[c]
-	int ab;                  // byte of the alpha sample
-	const double *alut;      // its table (channel 0's when alpha is not needed)
-	int lean;                // no channel in mode 2 and none with a transparent edge
-};
-
-__device__ __forceinline__ int fast_cin_depth(const DevPlan &P, FastCin &F)
-{
-	if (P.in_float || (P.bit_depth != 8 && P.bit_depth != 16)) return 0;
-	const int bps = P.bit_depth / 8, pb = P.nch_phys * bps;
-	bool need_a = false;
-	for (int c = 0; c < IWAMD_MAX_CH; c++) {
-		const DevIChan &ic = P.ic[c < P.inter_nch ? c : 0];
-		if (ic.cvt_gray || ic.input_channel >= P.nch_phys || !ic.lut[0]) return 0;
-		F.chb[c] = ic.input_channel * bps;
-		F.lut[c] = ic.lut[0];
-		F.mode[c] = ic.need_unassoc ? 1 : (ic.early_bkgd ? 2 : 0);
-		F.bkgd[c] = ic.bkgd;
-		F.edge[c] = ic.v.edge;
-		F.et[c] = ic.v.edge_transparent;
-		need_a |= F.mode[c] != 0;
-	}
-	F.lean = 1;
-	for (int c = 0; c < IWAMD_MAX_CH; c++)
-		if (F.mode[c] == 2 || F.et[c]) F.lean = 0;
-	F.ab = F.chb[0];
-	F.alut = F.lut[0];
-	if (need_a) {
-		if (P.alpha_in_index >= P.nch_phys || !P.alpha_lut) return 0;
-		F.ab = P.alpha_in_index * bps;
-		F.alut = P.alpha_lut;
-	}
-	int rx, ry;
-	long long o[3];
-	const int px[3] = {0, 1, 0}, py[3] = {0, 0, 1};
-	for (int k = 0; k < 3; k++) {
-		logical_to_physical(P, px[k], py[k], rx, ry);
-		o[k] = (long long)ry * (long long)P.in_bpr + (long long)rx * pb;
-	}
-	F.base = o[0];
-	F.dx = o[1] - o[0];
-	F.dy = o[2] - o[0];
-	return P.bit_depth;
-}
-
-template <int DEPTH>
-__device__ __forceinline__ unsigned int fast_raw(const g_u8 *in, long long off)
-{
-	if (DEPTH == 8) return in[off];
-	return ((unsigned int)in[off] << 8) | in[off + 1];
-}
-
-// Intermediate channels 0..C-1 of logical input (xc, p), xc in [0, W): the
-// band's sample, as band_sample(P, P.ic[c], xc, p, P.ih) would give it
-// (channels from C on are left unspecified).  LEAN: no early background and
-// no transparent edge (FastCin::lean), so a channel is its table value times
-// alpha or times 1 (which leaves it unchanged).
-template <int DEPTH, bool LEAN>
-__device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, long long cb, int p,
-	float v[IWAMD_MAX_CH])
-{
-	const int C = P.inter_nch;
-	if (DEPTH == 0) {
-		// cb holds the column here.
-#pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++)
-			v[c] = c < C ? (float)band_sample(P, P.ic[c], (int)cb, p, P.ih) : 0.0f;
-		return;
-	}
-	const int pin = p < 0 ? 0 : (p >= P.ih ? P.ih - 1 : p);
-	const long long off = cb + (long long)pin * F.dy;
-	const g_u8 *in = (const g_u8 *)P.in;
-	unsigned int raw[IWAMD_MAX_CH];
-#pragma unroll
-	for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = fast_raw<DEPTH>(in, off + F.chb[c]);
-	const double a = ((const g_cf64 *)F.alut)[fast_raw<DEPTH>(in, off + F.ab)];
-#pragma unroll
-	for (int c = 0; c < IWAMD_MAX_CH; c++) {
-		const double x = ((const g_cf64 *)F.lut[c])[raw[c]];
-		if (LEAN) {
-			v[c] = (float)(x * (F.mode[c] ? a : 1.0));
-			continue;
-		}
-		const double xm = x * a;
-		const double xb = a * x + (1.0 - a) * F.bkgd[c];
-		double y = F.mode[c] == 1 ? xm : (F.mode[c] == 2 ? xb : x);
-		if (p != pin && F.et[c]) y = F.edge[c];
-		v[c] = (float)y;
-	}
 }
 
 // Vertical pass of a tile: 16-column groups of the intermediate, one per wave
@@ -2443,6 +2513,9 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 	}
 	D.W = p.inter_width;
 	D.Hp = p.inter_height;
+	D.v_shared = (int)S.dv.size() == p.inter_nch && p.inter_nch > 0;
+	for (int c = 0; c < (int)S.dv.size() && D.v_shared; c++)
+		if (!same_dense(S.dv[c], S.dv[0]) || S.wv[c].mode != iwx::WeightList::WEIGHTS) D.v_shared = 0;
 	for (int c = 0; c < p.inter_nch; c++) {
 		const iwamd_ichan &s = p.ich[c];
 		DevIChan &d = D.ic[c];
