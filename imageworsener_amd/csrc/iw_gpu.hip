@@ -124,6 +124,7 @@ struct DevPlan {
 	size_t out_bytes;
 	int apply_bkgd, late, checker, check_size, ox, oy, bk_has_trans;
 	int v_shared;  // every channel has the same vertical weights (dense form) in WEIGHTS mode
+	int h_shared;  // the same for the horizontal weights
 	double bk1a, bk2a;
 };
 
@@ -560,23 +561,37 @@ __device__ __forceinline__ void emit_px(const EmitCtx &E, int i, int j, Gather g
 // every channel and pixel go in stages -- all bin-index loads, then all
 // entry loads, then the codes -- so their latencies overlap instead of
 // adding up.  A bin holds at most two entries for 8-bit tables; a larger one
-// (or a sample outside [0, 1]) is searched by count_le.  gather(n, c):
-// channel c of pixel n; pixels with live[n] false are computed, not stored.
-template <int N, class Gather>
+// (or a sample outside [0, 1]) is searched by count_le.  ALL: gather(n, g)
+// fills g[c] with intermediate channel c of pixel n (c < nch) at once; else
+// gather(n, c) returns one channel.  Pixels with live[n] false are computed,
+// not stored.
+template <int N, bool ALL, class Gather>
 __device__ __forceinline__ void emit_px_simple(const EmitSimple &E, const int (&pi)[N], const int (&pj)[N],
 	const bool (&live)[N], Gather gather)
 {
 	double v[IWAMD_MAX_CH][N];
+	double g[N][IWAMD_MAX_CH];
 	float alpha_f[N];
 #pragma unroll
-	for (int n = 0; n < N; n++) alpha_f[n] = 0.0f;
+	for (int n = 0; n < N; n++) {
+		alpha_f[n] = 0.0f;
+		if constexpr (ALL) gather(n, g[n]);
+	}
 #pragma unroll
 	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
 		if (idx >= E.nch) break;
 		const int f = E.f[idx];
 #pragma unroll
 		for (int n = 0; n < N; n++) {
-			double x = gather(n, f & 3);
+			double x;
+			if constexpr (ALL) {
+				x = g[n][0];
+#pragma unroll
+				for (int c = 1; c < IWAMD_MAX_CH; c++)
+					if ((f & 3) == c) x = g[n][c];
+			} else {
+				x = gather(n, f & 3);
+			}
 			if (E.intclamp) x = clamp01(x);
 			if (f & (1 << 6)) alpha_f[n] = (float)x;
 			if (f & (1 << 5)) {
@@ -943,13 +958,46 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 	};
 	EmitSimple ES;
 	if (emit_simple_ctx(P, ES)) {
+		// Every channel's horizontal sum of one pixel: through the shared dense
+		// list (one weight load per tap for all channels) when the channels
+		// share it, else channel by channel.  A dense position the list does
+		// not hold weighs 0, which leaves a sum's bits unchanged.
+		const DevWeights &h0 = P.ic[0].h;
+		const g_cint *hfirst = (const g_cint *)h0.dfirst, *hcount = (const g_cint *)h0.dcount;
+		const g_cint *hoff = (const g_cint *)h0.doff;
+		const g_cf64 *hdw = (const g_cf64 *)h0.dw;
+		auto hsum_all = [&](int r, int i, double (&g)[IWAMD_MAX_CH]) {
+			if (!P.h_shared) {
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++) g[c] = c < C ? hsum(c, r, i) : 0.0;
+				return;
+			}
+			double acc[IWAMD_MAX_CH] = {0.0, 0.0, 0.0, 0.0};
+			const int f0 = hfirst[i], n = hcount[i], o0 = hoff[i];
+			for (int u = 0; u < n; u++) {
+				const double w = hdw[o0 + u];
+				const int pos = f0 + u;
+				const bool outside = pos < 0 || pos >= W;
+				const int pc = pos < 0 ? 0 : (pos >= W ? W - 1 : pos);
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++) {
+					if (c >= C) break;
+					const DevWeights &h = P.ic[c].h;
+					const double v = (outside && h.edge_transparent) ? h.edge :
+						(double)inter[((size_t)c * R + r) * W + pc];
+					acc[c] = acc[c] + v * w;
+				}
+			}
+#pragma unroll
+			for (int c = 0; c < IWAMD_MAX_CH; c++) g[c] = acc[c];
+		};
 		// Two pixels per thread at once.
 		for (int t = tid; t < n_px; t += 2 * blockDim.x) {
 			const int t1 = t + (int)blockDim.x < n_px ? t + (int)blockDim.x : t;
 			const int r[2] = {t / P.out_w, t1 / P.out_w}, pi[2] = {t % P.out_w, t1 % P.out_w};
 			const int pj[2] = {j0 + r[0], j0 + r[1]};
 			const bool live[2] = {true, t1 != t};
-			emit_px_simple<2>(ES, pi, pj, live, [&](int n, int c) { return hsum(c, r[n], pi[n]); });
+			emit_px_simple<2, true>(ES, pi, pj, live, [&](int n, double (&g)[IWAMD_MAX_CH]) { hsum_all(r[n], pi[n], g); });
 		}
 	} else {
 		EmitCtx E;
@@ -1241,7 +1289,7 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, L
 			const int r[2] = {t / w.ni, t1 / w.ni}, ii[2] = {t % w.ni, t1 % w.ni};
 			const int pi[2] = {w.i0 + ii[0], w.i0 + ii[1]}, pj[2] = {w.j0 + r[0], w.j0 + r[1]};
 			const bool live[2] = {true, t1 != t};
-			emit_px_simple<2>(ES, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
+			emit_px_simple<2, false>(ES, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
 		}
 	} else {
 		EmitCtx E;
@@ -2516,6 +2564,9 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 	D.v_shared = (int)S.dv.size() == p.inter_nch && p.inter_nch > 0;
 	for (int c = 0; c < (int)S.dv.size() && D.v_shared; c++)
 		if (!same_dense(S.dv[c], S.dv[0]) || S.wv[c].mode != iwx::WeightList::WEIGHTS) D.v_shared = 0;
+	D.h_shared = (int)S.dh.size() == p.inter_nch && p.inter_nch > 0;
+	for (int c = 0; c < (int)S.dh.size() && D.h_shared; c++)
+		if (!same_dense(S.dh[c], S.dh[0]) || S.wh[c].mode != iwx::WeightList::WEIGHTS) D.h_shared = 0;
 	for (int c = 0; c < p.inter_nch; c++) {
 		const iwamd_ichan &s = p.ich[c];
 		DevIChan &d = D.ic[c];
