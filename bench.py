@@ -106,11 +106,11 @@ def lsb_error(plain, fast):
     return api.code_error(plain, fast)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, summary="c1_plain_hbm_counters.txt"):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated against a plain copy
     kernel); None when absent."""
-    path = os.path.join(ROOT, "profiles", "c1_plain_hbm_counters.txt")
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         with open(path) as f:
             for line in f:
@@ -223,9 +223,12 @@ def bench_c3(a):
             "config": {"workload": "C3: 4096 RGBA8 images, sides 64..1024, x0.25..2 per axis, "
                                    "triangle/catrom/lanczos2/lanczos3, 8-bit, no dither; one batch",
                        "global_batch": m["n"], "precision": chosen},
-            "roofline": {"kernel": "%s (persistent, hipGraph)" % ("fast_kernel" if chosen == "fast" else "band_kernel"),
+            "roofline": {"kernel": "%s (persistent, hipGraph step)" % (
+                             "fast_kernel, both LDS classes" if chosen == "fast" else "band_kernel"),
                          "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_launch": m["alg_bytes"]},
+                         "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic("c3_fast", "c3_fast_hbm_counters.txt") if chosen == "fast" else None,
+                         "alg_bytes_per_launch": m["alg_bytes"]},
             "cpu_baseline": base,
             "mfma": mfma_use(forms["fast"]["flops"], forms["fast"]["kernel_ms"]) if "fast" in forms else None,
             "latency_ms": {"p50": round(m["lat"][len(m["lat"]) // 2], 4), "p99": round(m["lat"][-1], 4)},
