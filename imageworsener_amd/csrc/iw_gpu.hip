@@ -1058,7 +1058,7 @@ __global__ void reset_counters_kernel(int *c, int n)
 // Tiles whose band and span fit these run in a kernel with a third of the
 // LDS, four workgroups per CU instead of two.
 #define FAST_S_KV 96
-#define FAST_S_XT 80
+#define FAST_S_XT 72
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -1075,17 +1075,13 @@ struct FastItem {
 // single FP16 product carries 11 bits, which the division by a small
 // resampled alpha would magnify past one output level.)  Intermediate rows
 // carry 16 zero columns past the tile's last 16-column group, so the
-// horizontal pass reads four-column groups without bounds checks.  The
-// band weights (read by the vertical pass) and the horizontal sums (written
-// by the horizontal pass, read by the per-pixel work) are never live at once
-// and share their space.
+// horizontal pass reads four-column groups without bounds checks.  The band
+// weights stay for all of a band's tiles.
 template <int KV, int XT>
 struct __attribute__((aligned(16))) FastLds {
 	static constexpr int kKV = KV, kXT = XT;
-	union {
-		_Float16 wv[2][FAST_ROWS][KV];                                  // hi, lo
-		float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
-	};
+	_Float16 wv[2][FAST_ROWS][KV];                        // hi, lo
+	float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
 	_Float16 inter[2][IWAMD_MAX_CH][FAST_ROWS][XT + 16];  // hi, lo
 };
 
@@ -1168,19 +1164,27 @@ __device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &
 
 // Operand layout of v_mfma_f32_16x16x16f16 for lane l: A[l%16][4*(l/16)+t],
 // B[4*(l/16)+t][l%16], D[4*(l/16)+t][l%16], t = 0..3.
+//
+// A work item is a band: the tiles tiles[first .. first+count) of one plan,
+// which share their 16 output rows and their band of input rows.  The band's
+// weights go to LDS once; then each tile runs the vertical pass, the
+// horizontal pass and the per-pixel work.
 template <class Lds>
-__device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, Lds &S)
+__device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__restrict__ tiles, int first, int count,
+	Lds &S)
 {
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int C = P.inter_nch;
 	const int lr = lane & 15, lq = lane >> 4;
-	const int kv_pad = (w.kv + 15) & ~15;
+	const FastItem w0 = tiles[first];
+	const int kv_pad = (w0.kv + 15) & ~15;
 	const DevWeights &dv = P.ic[0].v;  // every channel shares the lists
 	const DevWeights &dh = P.ic[0].h;
 	const float inv = 1.0f / 2048.0f;
 
 	// Vertical weights of the band, split.
 	{
+		const FastItem &w = w0;
 		const g_cint *vfirst = (const g_cint *)dv.dfirst, *vcount = (const g_cint *)dv.dcount;
 		const g_cint *voff = (const g_cint *)dv.doff;
 		const g_cf64 *vw = (const g_cf64 *)dv.dw;
@@ -1195,122 +1199,129 @@ __device__ __forceinline__ void fast_body(const DevPlan &P, const FastItem &w, L
 			split16(v, S.wv[0][r][k], S.wv[1][r][k]);
 		}
 	}
-	// Zero columns past the last 16-column group (read by the horizontal pass).
-	const int nx16 = (w.nx + 15) & ~15;
-	for (int t = tid; t < 2 * C * FAST_ROWS * 16; t += blockDim.x) {
-		const int col = t & 15, r = (t >> 4) & 15, hc = t >> 8;
-		S.inter[hc & 1][hc >> 1][r][nx16 + col] = (_Float16)0.0f;
-	}
-	__syncthreads();
-
 	FastCin F;
-	switch (fast_cin_depth(P, F)) {
-	case 8:
-		if (F.lean) fast_vertical<8, true, Lds>(P, w, S, F);
-		else fast_vertical<8, false, Lds>(P, w, S, F);
-		break;
-	case 16:
-		if (F.lean) fast_vertical<16, true, Lds>(P, w, S, F);
-		else fast_vertical<16, false, Lds>(P, w, S, F);
-		break;
-	default: fast_vertical<0, false, Lds>(P, w, S, F); break;
-	}
-	__syncthreads();
-
-	// Horizontal pass: 16-column output groups, one per wave.  K runs over
-	// intermediate positions from the group's first listed one rounded down to
-	// a multiple of 4 (x_lo is one, so the groups are four-aligned in LDS);
-	// positions a lane's list does not hold weigh 0.
-	const g_cint *hfirst = (const g_cint *)dh.dfirst, *hcount = (const g_cint *)dh.dcount;
-	const g_cint *hoff = (const g_cint *)dh.doff;
-	const g_cf64 *hw = (const g_cf64 *)dh.dw;
-	for (int g = wave; g * 16 < w.ni; g += 4) {
-		const int i = w.i0 + g * 16 + lr;
-		const bool live = (g * 16 + lr) < w.ni;
-		int f = 0x7fffffff, e = -0x7fffffff, lf = 0, ln = 0, lo = 0;
-		if (live) {
-			ln = hcount[i];
-			lf = hfirst[i];
-			lo = hoff[i];
-			if (ln > 0) {
-				f = lf;
-				e = lf + ln - 1;
-			}
-		}
-		for (int m = 1; m < 16; m <<= 1) {
-			f = min(f, __shfl_xor(f, m));
-			e = max(e, __shfl_xor(e, m));
-		}
-		float4_t acc[IWAMD_MAX_CH], acl[IWAMD_MAX_CH];
-#pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++) {
-			acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-			acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-		}
-		if (e >= f) {
-			for (int x0 = (f & ~3) - w.x_lo; x0 <= e - w.x_lo; x0 += 16) {
-				half4_t bh, bl;
-#pragma unroll
-				for (int t = 0; t < 4; t++) {
-					const int kk = w.x_lo + x0 + 4 * lq + t - lf;
-					const float v = (kk >= 0 && kk < ln) ? (float)hw[lo + kk] : 0.0f;
-					_Float16 h, l;
-					split16(v, h, l);
-					bh[t] = h;
-					bl[t] = l;
-				}
-#pragma unroll
-				for (int c = 0; c < IWAMD_MAX_CH; c++) {
-					if (c >= C) continue;
-					const half4_t ah = *(const half4_t *)&S.inter[0][c][lr][x0 + 4 * lq];
-					const half4_t al = *(const half4_t *)&S.inter[1][c][lr][x0 + 4 * lq];
-					acc[c] = mfma16(ah, bh, acc[c]);
-					acl[c] = mfma16(ah, bl, acl[c]);
-					acl[c] = mfma16(al, bh, acl[c]);
-				}
-			}
-		}
-#pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++)
-			if (c < C)
-#pragma unroll
-				for (int t = 0; t < 4; t++) S.hout[c][4 * lq + t][g * 16 + lr] = acc[c][t] + acl[c][t] * inv;
-	}
-	__syncthreads();
-
-	// Per-pixel work in FP64, as in the plain form.
-	const int rows = min(FAST_ROWS, P.Hp - w.j0);
-	const int npx = rows * w.ni;
+	const int depth = fast_cin_depth(P, F);
 	EmitSimple ES;
-	if (emit_simple_ctx(P, ES)) {
-		// Two pixels per thread at once.
-		for (int t = tid; t < npx; t += 2 * blockDim.x) {
-			const int t1 = t + (int)blockDim.x < npx ? t + (int)blockDim.x : t;
-			const int r[2] = {t / w.ni, t1 / w.ni}, ii[2] = {t % w.ni, t1 % w.ni};
-			const int pi[2] = {w.i0 + ii[0], w.i0 + ii[1]}, pj[2] = {w.j0 + r[0], w.j0 + r[1]};
-			const bool live[2] = {true, t1 != t};
-			emit_px_simple<2, false>(ES, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
+	const bool simple = emit_simple_ctx(P, ES);
+	const int rows = min(FAST_ROWS, P.Hp - w0.j0);
+	for (int tt = 0; tt < count; tt++) {
+		const FastItem w = tiles[first + tt];
+		// The band's weights are in place, and the previous tile's horizontal
+		// pass and per-pixel work are done with the intermediate and the sums.
+		__syncthreads();
+		// Zero columns past the last 16-column group (read by the horizontal pass).
+		const int nx16 = (w.nx + 15) & ~15;
+		for (int t = tid; t < 2 * C * FAST_ROWS * 16; t += blockDim.x) {
+			const int col = t & 15, r = (t >> 4) & 15, hc = t >> 8;
+			S.inter[hc & 1][hc >> 1][r][nx16 + col] = (_Float16)0.0f;
 		}
-	} else {
-		EmitCtx E;
-		emit_ctx(P, E);
-		for (int t = tid; t < npx; t += blockDim.x) {
-			const int r = t / w.ni, ii = t % w.ni;
-			emit_px(E, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
+
+		switch (depth) {
+		case 8:
+			if (F.lean) fast_vertical<8, true, Lds>(P, w, S, F);
+			else fast_vertical<8, false, Lds>(P, w, S, F);
+			break;
+		case 16:
+			if (F.lean) fast_vertical<16, true, Lds>(P, w, S, F);
+			else fast_vertical<16, false, Lds>(P, w, S, F);
+			break;
+		default: fast_vertical<0, false, Lds>(P, w, S, F); break;
+		}
+		__syncthreads();
+
+		// Horizontal pass: 16-column output groups, one per wave.  K runs over
+		// intermediate positions from the group's first listed one rounded down to
+		// a multiple of 4 (x_lo is one, so the groups are four-aligned in LDS);
+		// positions a lane's list does not hold weigh 0.
+		const g_cint *hfirst = (const g_cint *)dh.dfirst, *hcount = (const g_cint *)dh.dcount;
+		const g_cint *hoff = (const g_cint *)dh.doff;
+		const g_cf64 *hw = (const g_cf64 *)dh.dw;
+		for (int g = wave; g * 16 < w.ni; g += 4) {
+			const int i = w.i0 + g * 16 + lr;
+			const bool live = (g * 16 + lr) < w.ni;
+			int f = 0x7fffffff, e = -0x7fffffff, lf = 0, ln = 0, lo = 0;
+			if (live) {
+				ln = hcount[i];
+				lf = hfirst[i];
+				lo = hoff[i];
+				if (ln > 0) {
+					f = lf;
+					e = lf + ln - 1;
+				}
+			}
+			for (int m = 1; m < 16; m <<= 1) {
+				f = min(f, __shfl_xor(f, m));
+				e = max(e, __shfl_xor(e, m));
+			}
+			float4_t acc[IWAMD_MAX_CH], acl[IWAMD_MAX_CH];
+#pragma unroll
+			for (int c = 0; c < IWAMD_MAX_CH; c++) {
+				acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+				acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+			}
+			if (e >= f) {
+				for (int x0 = (f & ~3) - w.x_lo; x0 <= e - w.x_lo; x0 += 16) {
+					half4_t bh, bl;
+#pragma unroll
+					for (int t = 0; t < 4; t++) {
+						const int kk = w.x_lo + x0 + 4 * lq + t - lf;
+						const float v = (kk >= 0 && kk < ln) ? (float)hw[lo + kk] : 0.0f;
+						_Float16 h, l;
+						split16(v, h, l);
+						bh[t] = h;
+						bl[t] = l;
+					}
+#pragma unroll
+					for (int c = 0; c < IWAMD_MAX_CH; c++) {
+						if (c >= C) continue;
+						const half4_t ah = *(const half4_t *)&S.inter[0][c][lr][x0 + 4 * lq];
+						const half4_t al = *(const half4_t *)&S.inter[1][c][lr][x0 + 4 * lq];
+						acc[c] = mfma16(ah, bh, acc[c]);
+						acl[c] = mfma16(ah, bl, acl[c]);
+						acl[c] = mfma16(al, bh, acl[c]);
+					}
+				}
+			}
+#pragma unroll
+			for (int c = 0; c < IWAMD_MAX_CH; c++)
+				if (c < C)
+#pragma unroll
+					for (int t = 0; t < 4; t++) S.hout[c][4 * lq + t][g * 16 + lr] = acc[c][t] + acl[c][t] * inv;
+		}
+		__syncthreads();
+
+		// Per-pixel work in FP64, as in the plain form.
+		const int npx = rows * w.ni;
+		if (simple) {
+			// Two pixels per thread at once.
+			for (int t = tid; t < npx; t += 2 * blockDim.x) {
+				const int t1 = t + (int)blockDim.x < npx ? t + (int)blockDim.x : t;
+				const int r[2] = {t / w.ni, t1 / w.ni}, ii[2] = {t % w.ni, t1 % w.ni};
+				const int pi[2] = {w.i0 + ii[0], w.i0 + ii[1]}, pj[2] = {w.j0 + r[0], w.j0 + r[1]};
+				const bool live[2] = {true, t1 != t};
+				emit_px_simple<2, false>(ES, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
+			}
+		} else {
+			EmitCtx E;
+			emit_ctx(P, E);
+			for (int t = tid; t < npx; t += blockDim.x) {
+				const int r = t / w.ni, ii = t % w.ni;
+				emit_px(E, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
+			}
 		}
 	}
 }
 
 template <int KV, int XT, int WAVES_PER_SIMD>
 __global__ void __launch_bounds__(256, WAVES_PER_SIMD) fast_kernel(const DevPlan *__restrict__ plans,
-	const FastItem *__restrict__ items, int n_items, int *counter)
+	const FastItem *__restrict__ tiles, const int2 *__restrict__ items, int n_items, int *counter)
 {
 	__shared__ FastLds<KV, XT> S;
 	__shared__ int next_item;
 	int it = blockIdx.x;
 	while (it < n_items) {
-		const FastItem w = items[it];
-		fast_body(plans[w.job], w, S);
+		const int2 band = items[it];  // first tile, tile count
+		fast_band(plans[tiles[band.x].job], tiles, band.x, band.y, S);
 		if (!counter) break;
 		__syncthreads();
 		if (threadIdx.x == 0) next_item = (int)gridDim.x + atomicAdd(counter, 1);
@@ -2697,10 +2708,16 @@ struct WorkSet {
 	size_t band_lds[3][kLdsClasses] = {};      // largest LDS of a band in each list
 	std::vector<int2> planes[8];      // (plan, output channel), by ed_variant
 	std::vector<int> twopass;         // plans resampled by vpass + hpass
-	std::vector<FastItem> fast[2];    // chosen-precision tiles: small (FAST_S_*), large
+	// Chosen-precision tiles, and work items over them: runs of one band's
+	// tiles (first tile, count) by LDS class, small (FAST_S_*) and large.
+	std::vector<FastItem> fast_tiles;
+	std::vector<int4> fast_runs;      // (first, count, class, -) per band, before splitting
+	std::vector<int2> fast[2];
 	size_t o_plans = 0, o_bands[3][kLdsClasses] = {}, o_planes[8] = {0}, o_counters = 0, o_fast[2] = {0, 0}, o_done = 0;
 	size_t bytes = 0;
-	const FastItem *d_fast[2] = {nullptr, nullptr};
+	const FastItem *d_fast_tiles = nullptr;
+	const int2 *d_fast[2] = {nullptr, nullptr};
+	size_t o_fast_tiles = 0;
 	int *d_done = nullptr;  // one error-diffusion completion counter per plan
 	const DevPlan *d_plans = nullptr;
 	const int2 *d_bands[3][kLdsClasses] = {};
@@ -2716,9 +2733,18 @@ int ws_add(WorkSet &W, const Staged &S, const iwamd_plan &p)
 	W.plans.push_back(DevPlan());
 	if (S.skip_pixels) return jx;
 	if (!S.fast.empty()) {
-		for (FastItem it : S.fast) {
-			it.job = jx;
-			W.fast[it.kv <= FAST_S_KV && it.nx <= FAST_S_XT ? 0 : 1].push_back(it);
+		for (size_t k = 0; k < S.fast.size();) {
+			size_t e = k;
+			int maxnx = 0;
+			while (e < S.fast.size() && S.fast[e].j0 == S.fast[k].j0) maxnx = std::max(maxnx, S.fast[e++].nx);
+			const int first = (int)W.fast_tiles.size();
+			for (size_t u = k; u < e; u++) {
+				FastItem it = S.fast[u];
+				it.job = jx;
+				W.fast_tiles.push_back(it);
+			}
+			W.fast_runs.push_back(make_int4(first, (int)(e - k), S.fast[k].kv <= FAST_S_KV && maxnx <= FAST_S_XT ? 0 : 1, 0));
+			k = e;
 		}
 	} else if (S.band_rows) {
 		const int ri = S.band_rows == 4 ? 2 : (S.band_rows == 2 ? 1 : 0);
@@ -2736,15 +2762,25 @@ int ws_add(WorkSet &W, const Staged &S, const iwamd_plan &p)
 	return jx;
 }
 
+int device_cu_count();
+
 void ws_layout(WorkSet &W)
 {
+	// Work items of the chosen form: a band's tiles share its weights in LDS,
+	// in runs of up to eight tiles, fewer when the work set holds too few
+	// tiles to give every workgroup slot several items.
+	const int per = std::max(1, std::min(8, (int)(W.fast_tiles.size() / (8 * (size_t)device_cu_count()))));
+	for (int k = 0; k < 2; k++) W.fast[k].clear();
+	for (const int4 &r : W.fast_runs)
+		for (int a = 0; a < r.y; a += per) W.fast[r.z].push_back(make_int2(r.x + a, std::min(per, r.y - a)));
 	Layout L;
 	W.o_plans = L.add<DevPlan>(W.plans.size());
 	for (int r = 0; r < 3; r++)
 		for (int q = 0; q < kLdsClasses; q++) W.o_bands[r][q] = L.add<int2>(W.bands[r][q].size());
 	for (int v = 0; v < 8; v++) W.o_planes[v] = L.add<int2>(W.planes[v].size());
 	W.o_counters = L.add<int>(3 * kLdsClasses + 2);
-	for (int k = 0; k < 2; k++) W.o_fast[k] = L.add<FastItem>(W.fast[k].size());
+	W.o_fast_tiles = L.add<FastItem>(W.fast_tiles.size());
+	for (int k = 0; k < 2; k++) W.o_fast[k] = L.add<int2>(W.fast[k].size());
 	W.o_done = L.add<int>(W.plans.size());
 	W.bytes = (L.off + 255) & ~(size_t)255;
 }
@@ -2770,9 +2806,11 @@ bool ws_upload(WorkSet &W, char *base, hipStream_t st)
 		up(W.o_planes[v], W.planes[v].data(), W.planes[v].size() * sizeof(int2));
 	}
 	W.d_counters = (int *)(base + W.o_counters);
+	W.d_fast_tiles = (const FastItem *)(base + W.o_fast_tiles);
+	up(W.o_fast_tiles, W.fast_tiles.data(), W.fast_tiles.size() * sizeof(FastItem));
 	for (int k = 0; k < 2; k++) {
-		W.d_fast[k] = (const FastItem *)(base + W.o_fast[k]);
-		up(W.o_fast[k], W.fast[k].data(), W.fast[k].size() * sizeof(FastItem));
+		W.d_fast[k] = (const int2 *)(base + W.o_fast[k]);
+		up(W.o_fast[k], W.fast[k].data(), W.fast[k].size() * sizeof(int2));
 	}
 	W.d_done = (int *)(base + W.o_done);
 	return ok;
@@ -2806,13 +2844,13 @@ void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 		const int n = (int)W.fast[0].size();
 		const int grid = persistent ? std::min(n, device_cu_count() * 4) : n;
 		hipLaunchKernelGGL((fast_kernel<FAST_S_KV, FAST_S_XT, 4>), dim3(grid), dim3(256), 0, st, W.d_plans,
-			W.d_fast[0], n, persistent ? W.d_counters + 3 * kLdsClasses : nullptr);
+			W.d_fast_tiles, W.d_fast[0], n, persistent ? W.d_counters + 3 * kLdsClasses : nullptr);
 	}
 	if (!W.fast[1].empty()) {
 		const int n = (int)W.fast[1].size();
 		const int grid = persistent ? std::min(n, device_cu_count() * 2) : n;
 		hipLaunchKernelGGL((fast_kernel<FAST_MAX_KV, FAST_MAX_XT, 2>), dim3(grid), dim3(256), 0, st, W.d_plans,
-			W.d_fast[1], n, persistent ? W.d_counters + 3 * kLdsClasses + 1 : nullptr);
+			W.d_fast_tiles, W.d_fast[1], n, persistent ? W.d_counters + 3 * kLdsClasses + 1 : nullptr);
 	}
 	for (int r = 0; r < 3; r++)
 		for (int q = 0; q < kLdsClasses; q++) {
