@@ -1199,13 +1199,13 @@ __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__re
 			split16(v, S.wv[0][r][k], S.wv[1][r][k]);
 		}
 	}
-	FastCin F;
-	const int depth = fast_cin_depth(P, F);
-	EmitSimple ES;
-	const bool simple = emit_simple_ctx(P, ES);
 	const int rows = min(FAST_ROWS, P.Hp - w0.j0);
 	for (int tt = 0; tt < count; tt++) {
 		const FastItem w = tiles[first + tt];
+		FastCin F;
+		const int depth = fast_cin_depth(P, F);
+		EmitSimple ES;
+		const bool simple = emit_simple_ctx(P, ES);
 		// The band's weights are in place, and the previous tile's horizontal
 		// pass and per-pixel work are done with the intermediate and the sums.
 		__syncthreads();
