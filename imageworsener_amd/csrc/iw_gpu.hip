@@ -761,13 +761,16 @@ __device__ __forceinline__ int fast_cin_depth(const DevPlan &P, FastCin &F)
 	F.base = o[0];
 	F.dx = o[1] - o[0];
 	F.dy = o[2] - o[0];
+	// Four 8-bit samples per pixel, every pixel 4-byte aligned: one 32-bit
+	// load per pixel ("depth" 32).
+	if (P.bit_depth == 8 && pb == 4 && ((F.base | F.dx | F.dy) & 3) == 0 && ((uintptr_t)P.in & 3) == 0) return 32;
 	return P.bit_depth;
 }
 
 template <int DEPTH>
 __device__ __forceinline__ unsigned int fast_raw(const g_u8 *in, long long off)
 {
-	if (DEPTH == 8) return in[off];
+	if (DEPTH != 16) return in[off];
 	return ((unsigned int)in[off] << 8) | in[off + 1];
 }
 
@@ -791,10 +794,18 @@ __device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, l
 	const int pin = p < 0 ? 0 : (p >= P.ih ? P.ih - 1 : p);
 	const long long off = cb + (long long)pin * F.dy;
 	const g_u8 *in = (const g_u8 *)P.in;
-	unsigned int raw[IWAMD_MAX_CH];
+	unsigned int raw[IWAMD_MAX_CH], araw;
+	if (DEPTH == 32) {
+		const unsigned int word = *(const __attribute__((address_space(1))) unsigned int *)(in + off);
 #pragma unroll
-	for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = fast_raw<DEPTH>(in, off + F.chb[c]);
-	const double a = ((const g_cf64 *)F.alut)[fast_raw<DEPTH>(in, off + F.ab)];
+		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = (word >> (8 * F.chb[c])) & 0xffu;
+		araw = (word >> (8 * F.ab)) & 0xffu;
+	} else {
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = fast_raw<DEPTH>(in, off + F.chb[c]);
+		araw = fast_raw<DEPTH>(in, off + F.ab);
+	}
+	const double a = ((const g_cf64 *)F.alut)[araw];
 #pragma unroll
 	for (int c = 0; c < IWAMD_MAX_CH; c++) {
 		const double x = ((const g_cf64 *)F.lut[c])[raw[c]];
@@ -903,7 +914,10 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 		__syncthreads();
 		if (depth) {
 			// Channel 0's weights are every channel's.
-			if (depth == 8) {
+			if (depth == 32) {
+				if (F.lean) band_vertical_direct<R, 32, true>(P, F, j0, rows, wd, inter);
+				else band_vertical_direct<R, 32, false>(P, F, j0, rows, wd, inter);
+			} else if (depth == 8) {
 				if (F.lean) band_vertical_direct<R, 8, true>(P, F, j0, rows, wd, inter);
 				else band_vertical_direct<R, 8, false>(P, F, j0, rows, wd, inter);
 			} else {
@@ -1217,6 +1231,10 @@ __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__re
 		}
 
 		switch (depth) {
+		case 32:
+			if (F.lean) fast_vertical<32, true, Lds>(P, w, S, F);
+			else fast_vertical<32, false, Lds>(P, w, S, F);
+			break;
 		case 8:
 			if (F.lean) fast_vertical<8, true, Lds>(P, w, S, F);
 			else fast_vertical<8, false, Lds>(P, w, S, F);
