@@ -1311,13 +1311,11 @@ __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__re
 		// Per-pixel work in FP64, as in the plain form.
 		const int npx = rows * w.ni;
 		if (simple) {
-			// Two pixels per thread at once.
-			for (int t = tid; t < npx; t += 2 * blockDim.x) {
-				const int t1 = t + (int)blockDim.x < npx ? t + (int)blockDim.x : t;
-				const int r[2] = {t / w.ni, t1 / w.ni}, ii[2] = {t % w.ni, t1 % w.ni};
-				const int pi[2] = {w.i0 + ii[0], w.i0 + ii[1]}, pj[2] = {w.j0 + r[0], w.j0 + r[1]};
-				const bool live[2] = {true, t1 != t};
-				emit_px_simple<2, false>(ES, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
+			for (int t = tid; t < npx; t += blockDim.x) {
+				const int r[1] = {t / w.ni}, ii[1] = {t % w.ni};
+				const int pi[1] = {w.i0 + ii[0]}, pj[1] = {w.j0 + r[0]};
+				const bool live[1] = {true};
+				emit_px_simple<1, false>(ES, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
 			}
 		} else {
 			EmitCtx E;
