@@ -105,3 +105,15 @@ def test_fast_error_diffusion_mean(prod):
     mx, mean = lsb_stats(prod.run(plain(job)), prod.run(fast(job)))
     say("C1/8 FS fast vs plain: max %d LSB, mean %.4f LSB" % (mx, mean))
     assert mean < 0.5
+
+
+def test_fast_alone_equals_mixed(prod):
+    """A tile's sums depend only on its own plan: a job's bytes alone (work
+    items of single tiles) and inside a batch large enough that work items
+    are runs of up to eight tiles of a band are the same."""
+    jobs = [fast(j) for j in cases.c3_jobs(count=400, max_side=512)]
+    mixed = prod.run_batch(jobs)
+    for k in range(0, len(jobs), 67):
+        alone = prod.run(jobs[k])
+        cases.assert_same(alone, mixed[k], "fast alone/mixed %d" % k)
+    say("fast form: %d jobs alone equal to the batch of %d" % (len(range(0, len(jobs), 67)), len(jobs)))
