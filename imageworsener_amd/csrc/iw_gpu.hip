@@ -1005,13 +1005,11 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 #pragma unroll
 			for (int c = 0; c < IWAMD_MAX_CH; c++) g[c] = acc[c];
 		};
-		// Two pixels per thread at once.
-		for (int t = tid; t < n_px; t += 2 * blockDim.x) {
-			const int t1 = t + (int)blockDim.x < n_px ? t + (int)blockDim.x : t;
-			const int r[2] = {t / P.out_w, t1 / P.out_w}, pi[2] = {t % P.out_w, t1 % P.out_w};
-			const int pj[2] = {j0 + r[0], j0 + r[1]};
-			const bool live[2] = {true, t1 != t};
-			emit_px_simple<2, true>(ES, pi, pj, live, [&](int n, double (&g)[IWAMD_MAX_CH]) { hsum_all(r[n], pi[n], g); });
+		for (int t = tid; t < n_px; t += blockDim.x) {
+			const int r[1] = {t / P.out_w}, pi[1] = {t % P.out_w};
+			const int pj[1] = {j0 + r[0]};
+			const bool live[1] = {true};
+			emit_px_simple<1, true>(ES, pi, pj, live, [&](int n, double (&g)[IWAMD_MAX_CH]) { hsum_all(r[n], pi[n], g); });
 		}
 	} else {
 		EmitCtx E;
