@@ -2206,6 +2206,8 @@ struct Staged {
 	std::vector<WOff> ov, oh;
 	std::vector<QOff> oq;
 	size_t bytes = 0;          // size of the job's part of a block
+	size_t o_tables = 0;       // start of the host-built tables (through the end of the part)
+	std::vector<uint8_t> pack;  // their host image, sent in one copy (lives until the stream syncs)
 };
 
 size_t band_lds_bytes(int R, int C, int W)
@@ -2468,6 +2470,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	S.o_out = out_elsewhere ? 0 : L.add<uint8_t>(S.out_bytes);
 	S.o_eds = L.add<double>((size_t)S.n_ed * p.out_height * p.out_width);
 	S.o_ede = L.add<double>((size_t)S.n_ed * 4 * p.out_width);
+	S.o_tables = L.add<uint8_t>(0);
 	S.o_lut.resize(S.luts.size());
 	for (size_t i = 0; i < S.luts.size(); i++) S.o_lut[i] = L.add<double>(S.luts[i].size());
 	S.ov.resize(p.inter_nch);
@@ -2510,15 +2513,19 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	return true;
 }
 
-// Queue the host-to-device copies of a staged job into its part of a block.
-// The host tables must live until the stream has been synchronised.
-bool upload_job(const Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
+// Queue the host-to-device copies of a staged job into its part of a block:
+// the input image, then every host-built table in one copy of the tables
+// region from a host image of it (S.pack).  S must live until the stream has
+// been synchronised.
+bool upload_job(Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
 {
 	bool ok = true;
+	if (S.in_bytes) ok &= hipMemcpyAsync(base + S.o_in, p.in_pixels, S.in_bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+	const size_t t0 = S.o_tables;
+	S.pack.assign(S.bytes - t0, 0);
 	auto up = [&](size_t off, const void *src, size_t bytes) {
-		if (bytes) ok &= hipMemcpyAsync(base + off, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+		if (bytes) memcpy(S.pack.data() + (off - t0), src, bytes);
 	};
-	up(S.o_in, p.in_pixels, S.in_bytes);
 	for (size_t i = 0; i < S.luts.size(); i++) up(S.o_lut[i], S.luts[i].data(), S.luts[i].size() * 8);
 	auto up_w = [&](const iwx::WeightList &wl, const WOff &o) {
 		up(o.start, wl.start.data(), wl.start.size() * 4);
@@ -2553,6 +2560,8 @@ bool upload_job(const Staged &S, const iwamd_plan &p, char *base, hipStream_t st
 		up(q.pre, S.et[o].pre.data(), S.et[o].pre.size() * 8);
 		up(q.suf, S.et[o].suf.data(), S.et[o].suf.size() * 8);
 	}
+	if (!S.pack.empty())
+		ok &= hipMemcpyAsync(base + t0, S.pack.data(), S.pack.size(), hipMemcpyHostToDevice, st) == hipSuccess;
 	return ok;
 }
 
