@@ -295,7 +295,40 @@ double round_up_point(double lo, double hi)
 	return bitsd(b);
 }
 
+namespace {
+struct QuantKey {
+	int cstype;
+	double gamma, maxcc;
+	int P, color_count, nc, fold;
+	bool operator<(const QuantKey &o) const
+	{
+		return std::tie(cstype, gamma, maxcc, P, color_count, nc, fold) <
+		       std::tie(o.cstype, o.gamma, o.maxcc, o.P, o.color_count, o.nc, o.fold);
+	}
+};
+std::mutex g_quant_mutex;
+std::map<QuantKey, QuantTable> g_quant_cache;
+QuantTable build_quant_table_uncached(const iwamd_ochan &oc, bool want_nc, bool fold_none);
+}  // namespace
+
+// A table depends only on the output transfer function, the code range and
+// the options; cached per process (most jobs of a batch share one).
 QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc, bool fold_none)
+{
+	const QuantKey key = {oc.out_cs.cstype, oc.out_cs.cstype == IW_CSTYPE_GAMMA ? oc.out_cs.gamma : 0.0,
+		oc.maxcolorcode_dbl, oc.maxcolorcode_int, oc.color_count, want_nc ? 1 : 0, fold_none ? 1 : 0};
+	{
+		std::lock_guard<std::mutex> lock(g_quant_mutex);
+		auto it = g_quant_cache.find(key);
+		if (it != g_quant_cache.end()) return it->second;
+	}
+	QuantTable q = build_quant_table_uncached(oc, want_nc, fold_none);
+	std::lock_guard<std::mutex> lock(g_quant_mutex);
+	return g_quant_cache.emplace(key, std::move(q)).first->second;
+}
+
+namespace {
+QuantTable build_quant_table_uncached(const iwamd_ochan &oc, bool want_nc, bool fold_none)
 {
 	QuantTable q;
 	const double M = oc.maxcolorcode_dbl;
@@ -356,6 +389,7 @@ QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc, bool fold_none
 	q.ix[NB] = (uint16_t)n;
 	return q;
 }
+}  // namespace
 
 DenseWeights dense_weights(const WeightList &wl)
 {
@@ -437,14 +471,27 @@ extern "C" int iwamd_quant_thresholds(int cstype, double gamma, int P, double *T
 
 namespace iwx {
 
+namespace {
+std::mutex g_lut_mutex;
+std::map<std::tuple<int, int, double, double>, std::vector<double>> g_lut_cache;
+}  // namespace
+
+// Cached per process: a 16-bit table costs 65536 libm calls.
 std::vector<double> build_input_lut(int nvalues, double maxcc, const struct iw_csdescr &cs)
 {
+	const auto key = std::make_tuple(nvalues, cs.cstype, cs.cstype == IW_CSTYPE_GAMMA ? cs.gamma : 0.0, maxcc);
+	{
+		std::lock_guard<std::mutex> lock(g_lut_mutex);
+		auto it = g_lut_cache.find(key);
+		if (it != g_lut_cache.end()) return it->second;
+	}
 	std::vector<double> lut(nvalues);
 	for (int v = 0; v < nvalues; v++) {
 		double s = ((double)v) / maxcc;
 		lut[v] = (cs.cstype == IW_CSTYPE_LINEAR) ? s : iwx_x_to_linear(s, &cs);
 	}
-	return lut;
+	std::lock_guard<std::mutex> lock(g_lut_mutex);
+	return g_lut_cache.emplace(key, std::move(lut)).first->second;
 }
 
 // ---- PRNG jump-ahead ------------------------------------------------------
