@@ -1070,7 +1070,7 @@ __global__ void reset_counters_kernel(int *c, int n)
 // Tiles whose band and span fit these run in a kernel with a third of the
 // LDS, four workgroups per CU instead of two.
 #define FAST_S_KV 96
-#define FAST_S_XT 72
+#define FAST_S_XT 64
 
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -1095,7 +1095,35 @@ struct __attribute__((aligned(16))) FastLds {
 	_Float16 wv[2][FAST_ROWS][KV];                        // hi, lo
 	float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
 	_Float16 inter[2][IWAMD_MAX_CH][FAST_ROWS][XT + 16];  // hi, lo
+	float lut[IWAMD_MAX_CH + 1][256];  // 8-bit input tables, FP32: channels, then alpha
 };
+
+// fast_fetch for LEAN plans of 8-bit input through the band's FP32 copies of
+// the input tables in LDS (lut[c], alpha last): the chosen form rounds the
+// sample to FP32 anyway, so the table values and the premultiply are taken in
+// FP32 (within 1.5 FP32 ulps of the FP64 route).
+template <int DEPTH>
+__device__ __forceinline__ void fast_fetch_lds(const DevPlan &P, const FastCin &F, const float (*lut)[256],
+	long long cb, int p, float v[IWAMD_MAX_CH])
+{
+	const int pin = p < 0 ? 0 : (p >= P.ih ? P.ih - 1 : p);
+	const long long off = cb + (long long)pin * F.dy;
+	const g_u8 *in = (const g_u8 *)P.in;
+	unsigned int raw[IWAMD_MAX_CH], araw;
+	if (DEPTH == 32) {
+		const unsigned int word = *(const __attribute__((address_space(1))) unsigned int *)(in + off);
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = (word >> (8 * F.chb[c])) & 0xffu;
+		araw = (word >> (8 * F.ab)) & 0xffu;
+	} else {
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = in[off + F.chb[c]];
+		araw = in[off + F.ab];
+	}
+	const float a = lut[IWAMD_MAX_CH][araw];
+#pragma unroll
+	for (int c = 0; c < IWAMD_MAX_CH; c++) v[c] = lut[c][raw[c]] * (F.mode[c] ? a : 1.0f);
+}
 
 __device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo)
 {
@@ -1137,7 +1165,12 @@ __device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &
 			const half4_t al = *(const half4_t *)&S.wv[1][lr][kk + 4 * lq];
 			float v[4][IWAMD_MAX_CH];
 #pragma unroll
-			for (int t = 0; t < 4; t++) fast_fetch<DEPTH, LEAN>(P, F, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+			for (int t = 0; t < 4; t++) {
+				if constexpr (LEAN && DEPTH != 16 && DEPTH != 0)
+					fast_fetch_lds<DEPTH>(P, F, S.lut, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+				else
+					fast_fetch<DEPTH, LEAN>(P, F, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+			}
 			half4_t bh[IWAMD_MAX_CH], bl[IWAMD_MAX_CH];
 #pragma unroll
 			for (int t = 0; t < 4; t++) {
@@ -1210,6 +1243,17 @@ __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__re
 			}
 			split16(v, S.wv[0][r][k], S.wv[1][r][k]);
 		}
+	}
+	{
+		// FP32 copies of the 8-bit input tables for the lean direct colour-in.
+		FastCin F0;
+		const int d0 = fast_cin_depth(P, F0);
+		if ((d0 == 8 || d0 == 32) && F0.lean)
+			for (int t = tid; t < 256; t += blockDim.x) {
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++) S.lut[c][t] = (float)((const g_cf64 *)F0.lut[c])[t];
+				S.lut[IWAMD_MAX_CH][t] = (float)((const g_cf64 *)F0.alut)[t];
+			}
 	}
 	const int rows = min(FAST_ROWS, P.Hp - w0.j0);
 	for (int tt = 0; tt < count; tt++) {
