@@ -83,6 +83,7 @@ _SIGS = {
     "iw_set_output_canvas_size": (None, [_c.c_void_p, _c.c_int, _c.c_int]),
     "iw_set_output_image_size": (None, [_c.c_void_p, _c.c_double, _c.c_double]),
     "iw_set_input_crop": (None, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    "iw_reorient_image": (None, [_c.c_void_p, _c.c_uint]),
     "iw_set_resize_alg": (None, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_double, _c.c_double, _c.c_double]),
     "iw_set_dither_type": (None, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int]),
     "iw_set_color_count": (None, [_c.c_void_p, _c.c_int, _c.c_int]),
@@ -271,6 +272,10 @@ class Library:
         img.pixels = mem
         img.bpr = bpr
         d.iw_set_input_image(ctx, ctypes.byref(img))
+        if "crop" in job:
+            d.iw_set_input_crop(ctx, *job["crop"])
+        if "reorient" in job:
+            d.iw_reorient_image(ctx, job["reorient"])
         if "input_cs" in job:
             cs = CsDescr(job["input_cs"][0], 0, job["input_cs"][1])
             d.iw_set_input_colorspace(ctx, ctypes.byref(cs))
