@@ -1,7 +1,7 @@
 """Orientation and crop through the direct colour-in: the band kernel and the
 chosen-precision kernel read input pixels through an affine byte map built
 from the same logical-to-physical function as the generic reader
-(src/imagew-main.c:129-276 reorientation and crop).  Every orientation of
+(src/imagew-main.c:129-276; the settings: src/imagew-api.c:366, :660).  Every orientation of
 iw_reorient_image, with and without a crop, for RGBA8 (one 32-bit load per
 pixel), RGB8 (byte loads) and RGBA16 inputs: the plain form is byte-identical
 to the reference core, the chosen form within 1 LSB of the plain form."""
