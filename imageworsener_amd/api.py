@@ -99,6 +99,8 @@ _SIGS = {
     "iw_set_allow_opt": (None, [_c.c_void_p, _c.c_int, _c.c_int]),
     "iw_set_max_malloc": (None, [_c.c_void_p, _c.c_size_t]),
     "iw_set_grayscale_weights": (None, [_c.c_void_p, _c.c_double, _c.c_double, _c.c_double]),
+    "iw_set_input_max_color_code": (None, [_c.c_void_p, _c.c_int, _c.c_int]),
+    "iw_set_output_max_color_code": (None, [_c.c_void_p, _c.c_int, _c.c_int]),
     "iw_get_errorflag": (_c.c_int, [_c.c_void_p]),
     "iw_get_errormsg": (_c.c_char_p, [_c.c_void_p, _c.c_char_p, _c.c_int]),
     "iw_get_version_int": (_c.c_int, []),
@@ -139,6 +141,10 @@ _EXT_SIGS = {
     "iwamd_server_request_ms": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(_c.c_double),
                                            _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
     "iwamd_server_destroy": (None, [_c.c_void_p]),
+    "iwamd_server_set_timeout": (None, [_c.c_void_p, _c.c_double]),
+    "iwamd_server_error": (_c.c_char_p, [_c.c_void_p]),
+    "iwamd_server_stats": (None, [_c.c_void_p, _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t),
+                                  _c.POINTER(_c.c_int)]),
 }
 
 # Byte offsets into struct iwamd_plan used by executors driven from Python.
@@ -291,6 +297,10 @@ class Library:
             d.iw_set_dither_type(ctx, ct, fam, sub)
         for ct, n in job.get("color_count", []):
             d.iw_set_color_count(ctx, ct, n)
+        for ch, n in job.get("input_maxcc", []):
+            d.iw_set_input_max_color_code(ctx, ch, n)
+        for ct, n in job.get("output_maxcc", []):
+            d.iw_set_output_max_color_code(ctx, ct, n)
         if "random_seed" in job:
             d.iw_set_random_seed(ctx, 0, job["random_seed"])
         for code, v in job.get("values", []):
@@ -411,11 +421,12 @@ class Server:
     overlapping batches.  results() returns every job's result in submit
     order, with its submit/start/done times (ms)."""
 
-    def __init__(self, lib, hbm_budget=0, streams=4):
+    def __init__(self, lib, hbm_budget=0, streams=4, timeout_s=120.0):
         self.lib = lib
         self.h = lib.dll.iwamd_server_create(hbm_budget, streams)
         if not self.h:
             raise RuntimeError("iwamd_server_create failed")
+        lib.dll.iwamd_server_set_timeout(self.h, timeout_s * 1e3)
         self.ctxs, self.ids = [], []
 
     def submit(self, job):
@@ -432,8 +443,20 @@ class Server:
     def drain(self):
         return bool(self.lib.dll.iwamd_server_drain(self.h))
 
+    def error(self):
+        e = self.lib.dll.iwamd_server_error(self.h)
+        return e.decode(errors="replace") if e else None
+
+    def stats(self):
+        held, peak, n = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
+        self.lib.dll.iwamd_server_stats(self.h, ctypes.byref(held), ctypes.byref(peak), ctypes.byref(n))
+        return {"bytes_held": held.value, "max_bytes_held": peak.value, "flights": n.value}
+
     def results(self):
-        self.drain()
+        """Every result in submit order; raises if the loop timed out (the
+        message names what was pending)."""
+        if not self.drain() and self.error():
+            raise TimeoutError(self.error())
         out = []
         for ctx, rid in zip(self.ctxs, self.ids):
             ok = rid >= 0 and bool(self.lib.dll.iwamd_server_wait(self.h, rid))

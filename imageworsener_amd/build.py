@@ -19,8 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libimageworsener_amd.so")
 BUILD_DIR = os.path.join(ROOT, "_build", "lib")
 
-SOURCES = ["iw_api.cpp", "iw_prepare.cpp", "iw_finish.cpp", "iw_tables.cpp", "iw_pnm.cpp", "iw_gpu.hip",
-           "iw_fast.hip", "iw_batch.cpp"]
+SOURCES = ["iw_api.cpp", "iw_prepare.cpp", "iw_finish.cpp", "iw_tables.cpp", "iw_pnm.cpp", "iw_gpu.hip"]
 
 ARCH = os.environ.get("IWAMD_OFFLOAD_ARCH", "gfx950")
 
@@ -53,7 +52,7 @@ def build(verbose=False):
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
-            continue
+            raise RuntimeError("missing source: " + path)
         obj = os.path.join(BUILD_DIR, src + ".o")
         cmd = [cc] + COMMON + ["-c", path, "-o", obj]
         if src.endswith(".hip"):

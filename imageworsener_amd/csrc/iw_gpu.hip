@@ -28,6 +28,7 @@
 
 #include "iw_ctx.h"
 #include "iw_tables.h"
+#include "iw_server.h"
 
 #include <algorithm>
 #include <chrono>
@@ -167,6 +168,7 @@ __device__ __forceinline__ unsigned int raw_int(const DevPlan &P, int rx, int ry
 
 __device__ __forceinline__ double raw_float(const DevPlan &P, int rx, int ry, int ch)
 {
+	if (P.bit_depth != 32) return 0.0;
 	float f;
 	memcpy(&f, P.in + (size_t)ry * P.in_bpr + (size_t)(P.nch_phys * rx + ch) * 4, 4);
 	return (double)f;
@@ -2161,11 +2163,15 @@ void *dev_alloc(size_t bytes)
 	}
 	void *p = nullptr;
 	if (hipMalloc(&p, bytes) != hipSuccess) {
-		// Release the cache and retry once.
-		std::lock_guard<std::mutex> lock(R.mu);
-		for (auto &b : R.free_blocks) hipFree(b.second);
-		R.free_blocks.clear();
-		R.cached_bytes = 0;
+		// Release the cache and retry once.  hipFree waits for the device,
+		// so the blocks are taken out under the lock and freed outside it.
+		std::multimap<size_t, void *> blocks;
+		{
+			std::lock_guard<std::mutex> lock(R.mu);
+			blocks.swap(R.free_blocks);
+			R.cached_bytes = 0;
+		}
+		for (auto &b : blocks) hipFree(b.second);
 		if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
 	}
 	return p;
@@ -2175,13 +2181,15 @@ void dev_release(void *p, size_t bytes)
 {
 	if (!p) return;
 	Runtime &R = rt();
-	std::lock_guard<std::mutex> lock(R.mu);
-	if (R.cached_bytes + bytes > kCacheLimit) {
-		hipFree(p);
-		return;
+	{
+		std::lock_guard<std::mutex> lock(R.mu);
+		if (R.cached_bytes + bytes <= kCacheLimit) {
+			R.free_blocks.emplace(bytes, p);
+			R.cached_bytes += bytes;
+			return;
+		}
 	}
-	R.free_blocks.emplace(bytes, p);
-	R.cached_bytes += bytes;
+	hipFree(p);
 }
 
 // Bump layout inside one device block.
@@ -2384,16 +2392,18 @@ void plan_fast(const iwamd_plan &p, Staged &S, int want_ti = 0)
 bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path, bool out_elsewhere = false)
 {
 	using namespace iwx;
-	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT) {
+	// Floating-point input: 32-bit samples are read as stored; the reference
+	// reads any other stored depth as 0.0 (src/imagew-main.c:267-272), which
+	// every transfer function maps to 0.0.  A 32-bit sample in a non-linear
+	// colour space would need libm's pow per sample (src/imagew-main.c:
+	// 53-66, :359-365), which the device does not evaluate: such jobs are
+	// refused rather than run with other bits.
+	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT && p.in_bit_depth == 32) {
 		for (int c = 0; c < p.inter_nch; c++) {
 			if (p.ich[c].in_cs.cstype != IW_CSTYPE_LINEAR) {
 				iw_set_error(ctx, "Floating-point input needs a linear input colorspace on the GPU path");
 				return false;
 			}
-		}
-		if (p.in_bit_depth != 32) {
-			iw_set_error(ctx, "Unsupported floating-point input depth");
-			return false;
 		}
 	}
 	S.wv.resize(p.inter_nch);
@@ -3622,320 +3632,324 @@ extern "C" void iwamd_batch_destroy(struct iwamd_batch *B)
 
 // ---- request loop (continuous batching) -------------------------------------
 // iwamd_server_submit() runs a request's host decisions and table building on
-// the caller's thread and queues it.  One worker thread admits queued
-// requests while their device bytes fit the HBM budget and launches every
-// request admitted at the same moment as one batch on a free stream (one
-// device block, persistent band kernels, error-diffusion planes, the outputs
-// copied back on the same stream).  Up to `streams` batches run at once, so a
-// request that arrives while others are on the GPU starts as soon as memory
-// and a stream free up, and its work overlaps theirs.  A finished batch
-// releases its memory and completes its requests' contexts.  Each request's
-// bytes depend only on its own plan.
-
-struct iwamd_server {
-	struct Req {
-		int id = 0, widx = 0;
-		iw_context *ctx = nullptr;
-		iwamd_plan plan;
-		Staged S;
-		size_t off = 0, out_off = 0;
-		double t_submit = 0, t_start = 0, t_done = 0;
-		double out_pixels = 0;
-		int flag = -1;  // index of its completion flag in the flight's host memory; -1: none
-		bool done = false, ok = false, skip = false, finished = false;
-	};
-	struct Flight {
-		std::vector<Req *> reqs;
-		WorkSet W;
-		void *block = nullptr;
-		size_t bytes = 0, out_base = 0, out_total = 0;
-		uint8_t *host = nullptr;  // pinned, coherent: outputs, then one flag per request
-		int *flags = nullptr;
-		hipStream_t st = nullptr;
-		hipEvent_t ev = nullptr;
-		bool failed = false;
-	};
-	std::mutex mu;
-	std::condition_variable cv_work, cv_done;
-	std::thread worker;
-	bool stop = false;
-	size_t budget = 0, used = 0;
-	std::vector<hipStream_t> streams_free;
-	std::vector<hipStream_t> streams_all;
-	std::deque<Req *> queue;
-	std::vector<Flight *> flights;
-	std::map<int, Req *> reqs;
-	int next_id = 0;
-	int device = 0;
-	std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
-	double now() const
-	{
-		return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-	}
-};
+// the caller's thread and queues it.  The loop (iw_server.h: one worker
+// thread) admits queued requests while their device bytes fit the HBM budget
+// and launches every request admitted at the same moment as one flight on a
+// free stream (one device block, persistent band kernels, error-diffusion
+// planes, outputs copied back on the same stream).  Up to `streams` flights
+// run at once, so a request that arrives while others are on the GPU starts
+// as soon as memory and a stream free up, and its work overlaps theirs.  A
+// request with error diffusion completes on its own: the last of its planes
+// copies its output to coherent pinned host memory and raises a flag there.
+// The rest complete with their flight.  Each request's bytes depend only on
+// its own plan.
+//
+// Pinned host buffers and the per-slot events are made once and reused:
+// freeing pinned memory waits for every stream of the device, which would
+// tie each flight's end to all the others'.
 
 namespace {
 
+struct SrvReq {
+	iw_context *ctx = nullptr;
+	iwamd_plan plan;
+	Staged S;
+	size_t off = 0, out_off = 0;
+	int widx = 0;
+	bool skip = false;
+};
+
+struct SrvFlight {
+	WorkSet W;
+	void *block = nullptr;
+	size_t bytes = 0, out_base = 0, out_total = 0;
+	uint8_t *host = nullptr;  // pinned, coherent: outputs, then one flag per request
+	size_t host_bytes = 0;
+	int *flags = nullptr;
+	hipEvent_t ev = nullptr;
+};
+
+// Pinned host buffers by size, reused across flights (worker thread only).
+struct PinnedPool {
+	std::multimap<size_t, uint8_t *> free;
+	size_t cached = 0;
+	static constexpr size_t kLimit = (size_t)4 << 30;
+	uint8_t *get(size_t bytes, size_t &got)
+	{
+		auto it = free.lower_bound(bytes);
+		if (it != free.end() && it->first <= 2 * bytes + (1 << 20)) {
+			uint8_t *p = it->second;
+			got = it->first;
+			cached -= it->first;
+			free.erase(it);
+			return p;
+		}
+		uint8_t *p = nullptr;
+		if (hipHostMalloc((void **)&p, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return nullptr;
+		got = bytes;
+		return p;
+	}
+	void put(uint8_t *p, size_t bytes)
+	{
+		if (!p) return;
+		if (cached + bytes > kLimit) {
+			// The pool is full: this buffer goes back to the system.
+			(void)hipHostFree(p);
+			return;
+		}
+		free.emplace(bytes, p);
+		cached += bytes;
+	}
+	void clear()
+	{
+		for (auto &kv : free) (void)hipHostFree(kv.second);
+		free.clear();
+		cached = 0;
+	}
+};
+
+struct ServerExec {
+	typedef SrvReq ReqData;
+	typedef SrvFlight FlightData;
+	typedef iwx::RequestLoop<ServerExec> Loop;
+	int device = 0;
+	std::vector<hipStream_t> streams;
+	std::vector<hipEvent_t> events;  // one per stream slot (one flight per slot at a time)
+	PinnedPool pool;
+
+	void worker_start() { (void)hipSetDevice(device); }
+	bool launch(Loop::Flight &F, int slot);
+	bool flag_raised(Loop::Flight &F, int k)
+	{
+		return __atomic_load_n(&F.data.flags[k], __ATOMIC_ACQUIRE) == 1;
+	}
+	bool flight_done(Loop::Flight &F)
+	{
+		const hipError_t e = hipEventQuery(F.data.ev);
+		if (e == hipErrorNotReady) return false;
+		if (e != hipSuccess) F.failed = true;  // the flight cannot complete normally
+		return true;
+	}
+	bool finish(Loop::Flight &F, Loop::Req &R, bool failed);
+	void release(Loop::Flight &F);
+};
+
 // Bytes a request holds on the device while it runs.
-size_t req_bytes(const iwamd_server::Req &R)
+size_t req_bytes(const Staged &S)
 {
-	return R.S.bytes + ((R.S.out_bytes + 255) & ~(size_t)255) + 64 * 1024;
+	return S.bytes + ((S.out_bytes + 255) & ~(size_t)255) + 64 * 1024;
 }
 
-void server_launch(iwamd_server::Flight *F)
+bool ServerExec::launch(Loop::Flight &F, int slot)
 {
+	SrvFlight &D = F.data;
+	const hipStream_t st = streams[slot];
+	D.ev = events[slot];
 	size_t off = 0;
-	for (auto *R : F->reqs) {
-		R->off = off;
-		off += R->S.bytes;
+	for (auto *R : F.reqs) {
+		R->data.off = off;
+		off += R->data.S.bytes;
 	}
-	F->out_base = off;
+	D.out_base = off;
 	size_t arena = 0;
-	for (auto *R : F->reqs) {
-		R->out_off = arena;
-		arena += (R->S.out_bytes + 255) & ~(size_t)255;
-		tune_apply(R->plan, R->S, F->st, false);
-		R->widx = ws_add(F->W, R->S, R->plan);
+	for (auto *R : F.reqs) {
+		R->data.out_off = arena;
+		arena += (R->data.S.out_bytes + 255) & ~(size_t)255;
+		tune_apply(R->data.plan, R->data.S, st, false);
+		R->data.widx = ws_add(D.W, R->data.S, R->data.plan);
 	}
-	F->out_total = arena;
-	ws_layout(F->W);
-	F->bytes = off + arena + F->W.bytes;
-	F->block = dev_alloc(F->bytes);
-	const size_t host_bytes = arena + sizeof(int) * F->reqs.size() + 16;
-	bool ok = F->block &&
-		hipHostMalloc((void **)&F->host, host_bytes, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess;
+	D.out_total = arena;
+	ws_layout(D.W);
+	D.bytes = off + arena + D.W.bytes;
+	D.block = dev_alloc(D.bytes);
+	const size_t host_bytes = arena + sizeof(int) * F.reqs.size() + 16;
+	bool ok = D.block != nullptr;
+	if (ok) {
+		D.host = pool.get(host_bytes, D.host_bytes);
+		ok = D.host != nullptr;
+	}
 	uint8_t *dhost = nullptr;
 	if (ok) {
-		F->flags = (int *)(F->host + arena);
-		for (size_t k = 0; k < F->reqs.size(); k++) F->flags[k] = 0;
-		ok = hipHostGetDevicePointer((void **)&dhost, F->host, 0) == hipSuccess;
+		D.flags = (int *)(D.host + arena);
+		for (size_t k = 0; k < F.reqs.size(); k++) __atomic_store_n(&D.flags[k], 0, __ATOMIC_RELEASE);
+		ok = hipHostGetDevicePointer((void **)&dhost, D.host, 0) == hipSuccess;
 	}
 	if (ok) {
-		char *base = (char *)F->block;
-		for (size_t k = 0; k < F->reqs.size(); k++) {
-			iwamd_server::Req *R = F->reqs[k];
-			DevPlan &D = F->W.plans[R->widx];
-			fill_dev_plan(R->S, R->plan, base + R->off, D);
-			D.out = (uint8_t *)(base + F->out_base + R->out_off);
-			if (R->S.n_ed > 0 && !R->skip) {
+		char *base = (char *)D.block;
+		for (size_t k = 0; k < F.reqs.size(); k++) {
+			SrvReq &R = F.reqs[k]->data;
+			DevPlan &P = D.W.plans[R.widx];
+			fill_dev_plan(R.S, R.plan, base + R.off, P);
+			P.out = (uint8_t *)(base + D.out_base + R.out_off);
+			if (R.S.n_ed > 0 && !R.skip) {
 				// The request completes on its own when its last plane ends.
-				D.ed_done = (int *)(base + F->out_base + F->out_total + F->W.o_done) + R->widx;
-				D.host_out = dhost + R->out_off;
-				D.host_flag = (int *)(dhost + arena) + k;
-				D.out_bytes = R->S.out_bytes;
-				R->flag = (int)k;
+				P.ed_done = (int *)(base + D.out_base + D.out_total + D.W.o_done) + R.widx;
+				P.host_out = dhost + R.out_off;
+				P.host_flag = (int *)(dhost + arena) + k;
+				P.out_bytes = R.S.out_bytes;
+				F.reqs[k]->flag = (int)k;
 			}
-			ok &= upload_job(R->S, R->plan, base + R->off, F->st);
+			ok &= upload_job(R.S, R.plan, base + R.off, st);
 		}
-		ok &= ws_upload(F->W, base + F->out_base + F->out_total, F->st);
+		ok &= ws_upload(D.W, base + D.out_base + D.out_total, st);
 	}
 	if (ok) {
-		ws_enqueue_resample(F->W, F->st, true);
-		ws_enqueue_errdiff(F->W, F->st);
+		ws_enqueue_resample(D.W, st, true);
+		ws_enqueue_errdiff(D.W, st);
 		// Requests without error diffusion complete with the flight.
-		for (auto *R : F->reqs)
-			if (R->flag < 0 && !R->skip)
-				ok &= hipMemcpyAsync(F->host + R->out_off, (char *)F->block + F->out_base + R->out_off,
-					R->S.out_bytes, hipMemcpyDeviceToHost, F->st) == hipSuccess;
+		for (auto *R : F.reqs)
+			if (R->flag < 0 && !R->data.skip)
+				ok &= hipMemcpyAsync(D.host + R->data.out_off, (char *)D.block + D.out_base + R->data.out_off,
+					R->data.S.out_bytes, hipMemcpyDeviceToHost, st) == hipSuccess;
 	}
-	F->failed = !ok;
-	hipEventRecord(F->ev, F->st);
+	// The event always ends the flight, also after a failed set-up.
+	if (hipEventRecord(D.ev, st) != hipSuccess) ok = false;
+	return ok;
 }
 
 // Completes one request from the flight's host memory.
-void server_finish_req(iwamd_server *V, iwamd_server::Flight *F, iwamd_server::Req *R)
+bool ServerExec::finish(Loop::Flight &F, Loop::Req &R, bool failed)
 {
-	bool ok = !F->failed;
-	if (ok) {
-		std::vector<uint8_t> zeros;
-		const uint8_t *img2 = F->host + R->out_off;
-		if (R->skip) {
-			zeros.assign(R->plan.out_bpr * (size_t)R->plan.out_height + 1, 0);
-			img2 = zeros.data();
-		}
-		ok = iwamd_finish(R->ctx, img2) != 0;
-	} else {
-		iw_set_error(R->ctx, "Request could not be run on the GPU");
+	if (failed) {
+		iw_set_error(R.data.ctx, "Request could not be run on the GPU");
+		return false;
 	}
-	R->finished = true;
-	const double t = V->now();
-	std::lock_guard<std::mutex> lock(V->mu);
-	R->ok = ok;
-	R->done = true;
-	R->t_done = t;
-	V->cv_done.notify_all();
+	std::vector<uint8_t> zeros;
+	const uint8_t *img2 = F.data.host + R.data.out_off;
+	if (R.data.skip) {
+		zeros.assign(R.data.plan.out_bpr * (size_t)R.data.plan.out_height + 1, 0);
+		img2 = zeros.data();
+	}
+	return iwamd_finish(R.data.ctx, img2) != 0;
 }
 
-// Completes a finished flight's remaining requests and frees it.
-void server_complete(iwamd_server *V, iwamd_server::Flight *F)
+void ServerExec::release(Loop::Flight &F)
 {
-	(void)hipEventSynchronize(F->ev);
-	for (auto *R : F->reqs)
-		if (!R->finished) server_finish_req(V, F, R);
-	for (auto *R : F->reqs) R->S = Staged();
-	dev_release(F->block, F->bytes);
-	if (F->host) (void)hipHostFree(F->host);
-	hipEventDestroy(F->ev);
-}
-
-void server_loop(iwamd_server *V)
-{
-	(void)hipSetDevice(V->device);
-	std::unique_lock<std::mutex> lock(V->mu);
-	for (;;) {
-		// Requests whose error diffusion has ended complete at once.
-		for (iwamd_server::Flight *F : V->flights) {
-			if (F->failed) continue;
-			for (auto *R : F->reqs) {
-				if (R->finished || R->flag < 0) continue;
-				if (__atomic_load_n(&F->flags[R->flag], __ATOMIC_ACQUIRE) == 1) {
-					lock.unlock();
-					server_finish_req(V, F, R);
-					lock.lock();
-				}
-			}
-		}
-		// Completed flights: they free memory and streams.
-		for (size_t k = 0; k < V->flights.size();) {
-			iwamd_server::Flight *F = V->flights[k];
-			if (hipEventQuery(F->ev) == hipSuccess) {
-				V->flights.erase(V->flights.begin() + k);
-				lock.unlock();
-				server_complete(V, F);
-				lock.lock();
-				size_t b = 0;
-				for (auto *R : F->reqs) b += req_bytes(*R);
-				V->used -= std::min(V->used, b);
-				V->streams_free.push_back(F->st);
-				delete F;
-				V->cv_done.notify_all();
-			} else {
-				k++;
-			}
-		}
-		// Admit what fits, as one batch on one stream.  Requests with error
-		// diffusion complete individually (their last plane copies the
-		// output to host memory and raises a flag), so sizes may mix.
-		if (!V->queue.empty() && !V->streams_free.empty()) {
-			iwamd_server::Flight *F = new iwamd_server::Flight();
-			while (!V->queue.empty()) {
-				iwamd_server::Req *R = V->queue.front();
-				const size_t b = req_bytes(*R);
-				// An idle device takes a request even above the budget.
-				if (V->used + b > V->budget && !(V->used == 0 && F->reqs.empty())) break;
-				V->queue.pop_front();
-				V->used += b;
-				R->t_start = V->now();
-				F->reqs.push_back(R);
-			}
-			if (F->reqs.empty()) {
-				delete F;
-			} else {
-				F->st = V->streams_free.back();
-				V->streams_free.pop_back();
-				hipEventCreateWithFlags(&F->ev, hipEventDisableTiming);
-				V->flights.push_back(F);
-				lock.unlock();
-				server_launch(F);
-				lock.lock();
-				continue;
-			}
-		}
-		if (V->stop && V->queue.empty() && V->flights.empty()) break;
-		// Wait for work or poll running flights.
-		if (V->flights.empty())
-			V->cv_work.wait(lock, [&] { return V->stop || !V->queue.empty(); });
-		else
-			V->cv_work.wait_for(lock, std::chrono::microseconds(200));
-	}
+	SrvFlight &D = F.data;
+	// flight_done() saw the event complete: the stream is idle.
+	(void)hipEventSynchronize(D.ev);
+	for (auto *R : F.reqs) R->data.S = Staged();
+	dev_release(D.block, D.bytes);
+	pool.put(D.host, D.host_bytes);
+	D.host = nullptr;
+	D.block = nullptr;
 }
 
 }  // namespace
 
+struct iwamd_server {
+	ServerExec x;
+	ServerExec::Loop *loop = nullptr;
+	double timeout_ms = 300000.0;
+	std::mutex err_mu;
+	std::string error;
+	void set_error(const std::string &s)
+	{
+		std::lock_guard<std::mutex> lock(err_mu);
+		error = s;
+		fprintf(stderr, "iwamd request loop: %s\n", s.c_str());
+	}
+};
+
 extern "C" struct iwamd_server *iwamd_server_create(size_t hbm_budget_bytes, int streams)
 {
 	iwamd_server *V = new iwamd_server();
-	V->budget = hbm_budget_bytes ? hbm_budget_bytes : ((size_t)64 << 30);
 	if (streams < 1) streams = 1;
 	if (streams > 16) streams = 16;
-	if (hipGetDevice(&V->device) != hipSuccess) V->device = 0;
+	if (hipGetDevice(&V->x.device) != hipSuccess) V->x.device = 0;
 	for (int k = 0; k < streams; k++) {
 		hipStream_t st = nullptr;
+		hipEvent_t ev = nullptr;
 		if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) break;
-		V->streams_all.push_back(st);
-		V->streams_free.push_back(st);
+		if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+			hipStreamDestroy(st);
+			break;
+		}
+		V->x.streams.push_back(st);
+		V->x.events.push_back(ev);
 	}
-	if (V->streams_all.empty()) {
+	if (V->x.streams.empty()) {
 		delete V;
 		return nullptr;
 	}
-	V->worker = std::thread(server_loop, V);
+	V->loop = new ServerExec::Loop(V->x, hbm_budget_bytes ? hbm_budget_bytes : ((size_t)64 << 30),
+		(int)V->x.streams.size());
+	V->loop->start();
 	return V;
 }
 
 extern "C" int iwamd_server_submit(struct iwamd_server *V, struct iw_context *ctx)
 {
-	iwamd_server::Req *R = new iwamd_server::Req();
-	R->ctx = ctx;
-	R->t_submit = V->now();
-	if (!iwamd_prepare(ctx, &R->plan) || !runtime_init(ctx) ||
-		!stage_job(ctx, R->plan, R->S, iw_get_option(ctx, "amd:resample"), true)) {
+	ServerExec::Loop::Req *R = new ServerExec::Loop::Req();
+	R->data.ctx = ctx;
+	if (!iwamd_prepare(ctx, &R->data.plan) || !runtime_init(ctx) ||
+		!stage_job(ctx, R->data.plan, R->data.S, iw_get_option(ctx, "amd:resample"), true)) {
 		delete R;
 		return -1;
 	}
-	R->skip = R->S.skip_pixels;
-	R->out_pixels = (double)R->plan.out_width * R->plan.out_height;
-	std::lock_guard<std::mutex> lock(V->mu);
-	R->id = V->next_id++;
-	V->reqs[R->id] = R;
-	V->queue.push_back(R);
-	V->cv_work.notify_one();
-	return R->id;
+	R->data.skip = R->data.S.skip_pixels;
+	R->bytes = req_bytes(R->data.S);
+	return V->loop->submit(R);
+}
+
+extern "C" void iwamd_server_set_timeout(struct iwamd_server *V, double ms)
+{
+	V->timeout_ms = ms;
 }
 
 extern "C" int iwamd_server_wait(struct iwamd_server *V, int id)
 {
-	std::unique_lock<std::mutex> lock(V->mu);
-	auto it = V->reqs.find(id);
-	if (it == V->reqs.end()) return 0;
-	iwamd_server::Req *R = it->second;
-	V->cv_done.wait(lock, [&] { return R->done; });
-	return R->ok ? 1 : 0;
+	std::string why;
+	const int r = V->loop->wait(id, V->timeout_ms, &why);
+	if (r < 0) {
+		V->set_error(why);
+		return 0;
+	}
+	return r;
 }
 
 extern "C" int iwamd_server_drain(struct iwamd_server *V)
 {
-	std::unique_lock<std::mutex> lock(V->mu);
-	V->cv_done.wait(lock, [&] { return V->queue.empty() && V->flights.empty(); });
-	int ok = 1;
-	for (auto &kv : V->reqs) ok &= kv.second->ok ? 1 : 0;
-	return ok;
+	std::string why;
+	const int r = V->loop->drain(V->timeout_ms, &why);
+	if (r < 0) {
+		V->set_error(why);
+		return 0;
+	}
+	return r;
+}
+
+extern "C" const char *iwamd_server_error(struct iwamd_server *V)
+{
+	std::lock_guard<std::mutex> lock(V->err_mu);
+	return V->error.empty() ? nullptr : V->error.c_str();
 }
 
 extern "C" int iwamd_server_request_ms(struct iwamd_server *V, int id, double *submit, double *start, double *done)
 {
-	std::lock_guard<std::mutex> lock(V->mu);
-	auto it = V->reqs.find(id);
-	if (it == V->reqs.end() || !it->second->done) return 0;
-	if (submit) *submit = it->second->t_submit;
-	if (start) *start = it->second->t_start;
-	if (done) *done = it->second->t_done;
-	return 1;
+	return V->loop->times(id, submit, start, done) ? 1 : 0;
+}
+
+extern "C" void iwamd_server_stats(struct iwamd_server *V, size_t *bytes_held, size_t *max_bytes_held,
+	int *flights)
+{
+	if (bytes_held) *bytes_held = V->loop->used();
+	if (max_bytes_held) *max_bytes_held = V->loop->max_used();
+	if (flights) *flights = V->loop->flights_launched();
 }
 
 extern "C" void iwamd_server_destroy(struct iwamd_server *V)
 {
 	if (!V) return;
-	{
-		std::lock_guard<std::mutex> lock(V->mu);
-		V->stop = true;
-	}
-	V->cv_work.notify_all();
-	if (V->worker.joinable()) V->worker.join();
-	for (hipStream_t st : V->streams_all) hipStreamDestroy(st);
-	for (auto &kv : V->reqs) delete kv.second;
+	V->loop->stop();
+	V->loop->delete_requests();
+	delete V->loop;
+	V->x.pool.clear();
+	for (hipEvent_t ev : V->x.events) hipEventDestroy(ev);
+	for (hipStream_t st : V->x.streams) hipStreamDestroy(st);
 	delete V;
 }
+
 
 int iwx_gpu_device_count(void)
 {

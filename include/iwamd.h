@@ -253,6 +253,16 @@ IW_EXPORT(int) iwamd_server_drain(struct iwamd_server *server);
 IW_EXPORT(int) iwamd_server_request_ms(struct iwamd_server *server, int id, double *submit, double *start,
 	double *done);
 IW_EXPORT(void) iwamd_server_destroy(struct iwamd_server *server);
+/* iwamd_server_wait() and iwamd_server_drain() wait at most `ms`
+ * milliseconds (default 300000; <= 0: no limit).  On a time-out they return 0
+ * and iwamd_server_error() names what is still pending (queued requests,
+ * running flights, bytes held); the message is also written to stderr. */
+IW_EXPORT(void) iwamd_server_set_timeout(struct iwamd_server *server, double ms);
+IW_EXPORT(const char*) iwamd_server_error(struct iwamd_server *server);
+/* Device bytes held by running flights now and at most so far, and the
+ * number of flights launched. */
+IW_EXPORT(void) iwamd_server_stats(struct iwamd_server *server, size_t *bytes_held, size_t *max_bytes_held,
+	int *flights);
 
 /* Library/device information.  Returns the number of visible HIP devices
  * (0 when none), and fills name (may be NULL). */
