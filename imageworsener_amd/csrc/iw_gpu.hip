@@ -86,9 +86,8 @@ struct DevOChan {
 	int ix_log2;
 	const double *D;                   // QuantTable::D (null: not folded)
 	int code_is_level;
-	const double *M, *ebpre, *ebsuf;  // error-diffusion decision data (EdTables)
-	double eb_max;
-	int ed_ordered;                    // EdTables::ordered
+	const double *M;                   // error diffusion: round-up points (EdTables)
+	int ed_warm;                       // error diffusion: warm-up pixels of a segment's walk
 	double bk1, bk2;
 	const uint64_t *prng_row, *prng_col;
 };
@@ -1393,31 +1392,36 @@ __global__ void __launch_bounds__(256, WAVES_PER_SIMD) fast_kernel(const DevPlan
 // ---- error diffusion ------------------------------------------------------
 //
 // A plane is one dependency chain: rows alternate direction (src/imagew-main.c:
-// 1072-1077), so every pixel waits for the one before it in scan order and a
-// row cannot start before the previous row has finished.  One workgroup runs
-// one plane: wave 0 walks the chain while waves 1-3 ("helpers") prepare and
-// complete the chunks around the one being walked.  Per chunk of a row:
-//   stage A (helpers, one pixel each): from the sample b and the error E
-//      carried from earlier rows, the levels the pixel can reach when the two
-//      in-row carries are bounded globally, and from those the pixel's own
-//      bound on |error| (the largest eb of those levels);
-//   stage B: the tighter window from the two previous pixels' own bounds,
-//      and the decision data of its one or two levels, to LDS.  Both windows
-//      are rigorous (|error| <= eb of its level; rounding is monotone), so s
-//      always lies in them.  Windows of three or more levels are "wide";
-//   walk (one lane): cell = (E + e[x-2]*m1) + e[x-1]*m0, s = clamp(b + cell),
-//      the level by one compare, the reference's floor/ceil choice and error
-//      by compares and selects; only s is stored.  Wide pixels search;
-//   finish (helpers): code and error recomputed from s by the same formulas
-//      (so they carry the walk's bits), output stored, errors kept.
-// While wave 0 walks chunk c, the helpers finish c-1, run stage B of c+1 and
-// stage A of c+2.  At a row's end the errors reach the next two rows as an
-// ordered gather -- for each target cell, the sources in the row's scan
-// order -- which adds the same terms in the same order as the reference's
-// per-pixel scatter (src/imagew-main.c:497-567).  The error rows live in LDS
-// for rows up to ED_ROWS_LDS wide.  An exact pixel (no error in the
-// reference) carries +0.0 here: adding a signed zero to an accumulator that
-// is never -0.0 leaves it unchanged, so the bits agree without flags.
+// 1072-1077), the sample of each pixel takes the errors of the one or two
+// pixels before it in its row (src/imagew-main.c:497-567), and a row needs
+// the errors of the rows above it.  Within a row the chain is cut by
+// speculation that is exact by construction:
+//   - the row is split into segments, one per lane; each lane walks its
+//     segment starting `warm` pixels early with no in-row carry;
+//   - what the rest of a row depends on at a pixel is the error of the one or
+//     two pixels before it (the walk's state).  Once a speculative walk's
+//     state equals the true chain's at some pixel, bit for bit, every later
+//     pixel of it is the true one.  Two walks that take the same levels see
+//     their difference shrink by the in-row weights at every step (7/16 for
+//     Floyd-Steinberg), so their sums round to the same doubles within a few
+//     dozen pixels;
+//   - segment k's state at its first pixel is compared with segment k-1's
+//     after its last; where they differ, one lane walks the true chain on
+//     from the last verified segment until its state equals the speculative
+//     walk's, overwriting what it walked, and the checks resume from there.
+// Each row then leaves its errors for the next two rows as an ordered
+// gather (stage A): for each target cell, the row's sources in its scan
+// order, so the same terms are added in the same order as the reference's
+// per-pixel scatter.  An exact pixel (no error in the reference) carries
+// +0.0: adding a signed zero to an accumulator that is never -0.0 leaves it
+// unchanged, so the bits agree without flags.
+//
+// The decision at a pixel is the reference's floor/ceil choice
+// (src/imagew-main.c:712-748) through tables built on the host (no libm on
+// the device): level k = #{T <= s}, found through a 4096-bin index and, in
+// the common case, one compare; exact below U[k] (error 0); else rounded up
+// from M[k], the least s where (lin[k+1]-s) <= (s-lin[k]) holds in FP64;
+// error s - lin[level].
 
 // Error-diffusion weights; slots 0,1: x+1,x+2 of this row; 2..6: x-2..x+2 of
 // the next row; 7..11: the row after (src/imagew-main.c:510-535).
@@ -1431,114 +1435,27 @@ __constant__ double c_ed_matrix[8][12] = {
 	{2.0/4, 0.0, 0.0, 1.0/4, 1.0/4, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                 // Sierra 4-2-1
 	{1.0/8, 1.0/8, 0.0, 1.0/8, 1.0/8, 1.0/8, 0.0, 0.0, 0.0, 1.0/8, 0.0, 0.0}};          // Atkinson
 
-#define ED_THREADS 256     // wave 0 walks; waves 1-3 help
-#define ED_CH 192          // pixels per chunk = helper threads
+#define ED_THREADS 256     // one workgroup per plane; up to one segment per thread
 #define ED_LDS_TABLE 256   // largest level count - 1 whose tables live in LDS
-#define ED_ROWS_LDS 4096   // widest row whose error rows live in LDS
-#define ED_QBINS 2048      // bins of the level index
+#define ED_ROWS_LDS 4096   // widest row whose working rows live in LDS
+#define ED_NB 4096         // bins of the level index (s * 4096 is exact)
+#define ED_SEG_MIN 8       // fewest pixels per segment
 
-// One pixel's operands for the walk, in the order it consumes them.
-struct __align__(16) EdPix {
-	double b, e;    // clamped sample; error carried from earlier rows
-	double t1, u0;  // s >= t1: level klo+1 (2.0: unreachable); level klo exact below u0
-	double m0, u1;  // level klo rounds up from m0; level klo+1 exact below u1
-	double m1, l0;  // level klo+1 rounds up from m1; lin[klo]
-	double l1, l2;  // lin[klo+1], lin[klo+2]
-};
-
-// One pixel's operands for the fast walk (weights without an x+2 carry):
-// the reference's decision for every s its window can reach, as one
-// round-up threshold between two bases and one exact interval [sa, sb).
-struct __align__(16) EdFast {
-	double b, e;             // clamped sample; error carried from earlier rows
-	double sm, lo;           // base: s >= sm ? hi : lo
-	double hi;
-	long long nsa;           // -bits(sa)
-	unsigned long long wdt;  // bits(sb) - bits(sa), 0 when the interval is empty
-	double pad;
-};
-
-__host__ __device__ inline long long ed_bits(double d) { return __builtin_bit_cast(long long, d); }
-
-// The decision data for a window [sL, sH] whose levels are klo..khi (the
-// levels of sL and sH).  Samples are >= +0.0, so comparing bit patterns as
-// integers orders them like doubles.  Level k's decision (src/imagew-main.c:
-// 712-748): exact below U[k] (error 0), else the base is lin[k + 1] from
-// M[k] on and lin[k] below it, and the error is s - base.
-//   one level k:  base by M[k]; exact iff s < U[k] (s >= T[k-1] holds);
-//   two levels k, k+1: the window must not reach U[k] (no exact sample of
-//      level k) nor hold both M[k] and M[k+1].  With M[k] <= T[k] <= M[k+1]
-//      ("ordered"), a sample of level k+1 is past M[k] and one of level k is
-//      below M[k+1], so one threshold -- M[k] if the window starts below it,
-//      else M[k+1] -- picks the base, and the exact samples are [T[k],
-//      U[k+1]).
-// False for any other window: the walk then searches the tables.
-__host__ __device__ inline bool ed_fast_form(const double *T, const double *U, const double *M,
-	const double *lin, int P, bool ordered, int klo, int khi, double sL, double sH, EdFast &f)
+// Per-plane working rows in HBM, in doubles: R1 (the share of row j-1's
+// errors for row j+1), err, (sample, carried error) pairs, levels (u16).
+__host__ __device__ inline size_t ed_row_doubles(int W)
 {
-	const int k = klo;
-	double sa, sb;
-	if (khi == klo) {
-		f.sm = M[k];
-		f.lo = lin[k];
-		f.hi = lin[k < P ? k + 1 : P];
-		sa = 0.0;
-		sb = U[k];
-	} else if (khi == klo + 1 && ordered && sL >= U[k] && !(sL < M[k] && sH >= M[k + 1])) {
-		if (sL < M[k]) {
-			f.sm = M[k];
-			f.lo = lin[k];
-			f.hi = lin[k + 1];
-		} else {
-			f.sm = M[k + 1];
-			f.lo = lin[k + 1];
-			f.hi = lin[k + 2 <= P ? k + 2 : P];
-		}
-		sa = T[k];
-		sb = U[k + 1];
-	} else {
-		return false;
-	}
-	const long long ba = ed_bits(sa), bb = ed_bits(sb);
-	f.nsa = -ba;
-	f.wdt = bb > ba ? (unsigned long long)(bb - ba) : 0ull;
-	return true;
+	return 4 * (size_t)W + 2 * (size_t)((W + 7) / 8);
 }
 
-// The error of sample s under ed_fast_form's data (the walk's step).
-__host__ __device__ inline double ed_fast_err(const EdFast &f, double s)
-{
-	const double base = s >= f.sm ? f.hi : f.lo;
-	const bool exact = (unsigned long long)ed_bits(s) + (unsigned long long)f.nsa < f.wdt;
-	return exact ? 0.0 : s - base;
-}
-
-template <bool LDS_ROWS>
-struct EdLds {
-	union {
-		EdPix px[2][ED_CH + 2];             // chunk c in px[c % 2]
-		EdFast fpx[2][ED_CH + 2];           // ... for the fast walk
-	};
-	double s[3][ED_CH];                     // walked samples, chunk c in [c % 3]
-	double eb[3][ED_CH];                    // bound on |error| per pixel
-	double b[2][ED_CH], e[2][ED_CH];        // stage A -> stage B
-	unsigned long long wide[2][ED_CH / 64];
-	unsigned short klo[3][ED_CH], khi[3][ED_CH];  // levels the pixel can reach
-	unsigned short kg0[2][ED_CH], kg1[2][ED_CH];  // ... under the global bound
-	double T[ED_LDS_TABLE], U[ED_LDS_TABLE + 1], M[ED_LDS_TABLE + 1], lin[ED_LDS_TABLE + 1];
-	double cd[ED_LDS_TABLE + 1], pre[ED_LDS_TABLE + 1], suf[ED_LDS_TABLE + 1];
-	unsigned short Q[ED_QBINS + 1];         // level index (ed_level)
-	double rows[LDS_ROWS ? 2 * ED_ROWS_LDS : 1];  // R1, ERR
-};
-
-struct EdTab {
-	const double *T, *U, *M, *lin, *cd, *pre, *suf;
-	const unsigned short *Q;
-	int P;
+// Level k's decision data as one record: T = the threshold at which level
+// k+1 starts (2.0 past the last level), U, M and lin as in the tables.
+struct __align__(16) EdRec {
+	double T, U, M, lin;
 };
 
 // Number of thresholds T[k], k in [a, b), that are <= s, plus a.
-__device__ __forceinline__ int level_in(const double *T, int a, int b, double s)
+__host__ __device__ inline int level_in(const double *T, int a, int b, double s)
 {
 	int n = b - a;
 	while (n > 0) {
@@ -1553,71 +1470,100 @@ __device__ __forceinline__ int level_in(const double *T, int a, int b, double s)
 	return a;
 }
 
-// Level of s in [0, 1] (the number of thresholds <= s, as level_in over all
-// of T).  Q[q] is the level of q / ED_QBINS, so a sample of bin q has a level
-// in [Q[q], Q[q+1]]: up to four thresholds are compared at once, more are
-// bisected.
-__device__ __forceinline__ int ed_level(const EdTab &t, double s)
-{
-	const int q = min((int)(s * ED_QBINS), ED_QBINS - 1);  // s * 2^11 is exact
-	const int lo = t.Q[q], hi = t.Q[q + 1];
-	if (hi == lo) return lo;
-	if (hi - lo <= 4) {
-		const double t0 = t.T[lo], t1 = t.T[min(lo + 1, hi - 1)];
-		const double t2 = t.T[min(lo + 2, hi - 1)], t3 = t.T[min(lo + 3, hi - 1)];
-		return lo + (t0 <= s ? 1 : 0) + ((lo + 1 < hi && t1 <= s) ? 1 : 0) + ((lo + 2 < hi && t2 <= s) ? 1 : 0) +
-		       ((lo + 3 < hi && t3 <= s) ? 1 : 0);
-	}
-	return level_in(t.T, lo, hi, s);
-}
+// The tables one plane's walk reads.  With the records in LDS (up to 257
+// levels) the common decision reads two bin bounds, then three records;
+// else the level comes from a bisection over the global thresholds.
+struct EdDecide {
+	const EdRec *rec;             // levels 0..P and two guards (LDS)
+	const unsigned short *Q;      // Q[q] = level of q / ED_NB, q = 0..ED_NB
+	const double *T, *U, *M, *lin;  // global tables (no records)
+	int P;
+};
 
-// The reference's decision for a sample s whose level is k: exact -> error
-// 0 and code k; else floor or ceil by distance (src/imagew-main.c:712-748).
-__device__ __forceinline__ double ed_error(const EdTab &t, int k, double s, int &level)
+// The reference's decision for a clamped sample s: its error (0.0 when
+// exact) and output level.
+template <bool TAB>
+__host__ __device__ inline double ed_decide(const EdDecide &d, double s, int &level)
 {
-	const bool exact = s < t.U[k];  // U[P] = +inf
-	const bool ceil_ = s >= t.M[k];
+	const int q = (int)(s * ED_NB) < ED_NB - 1 ? (int)(s * ED_NB) : ED_NB - 1;
+	const int k0 = d.Q[q], k1 = d.Q[q + 1];
+	int k;
+	double U, M, lk, lk1;
+	if (TAB) {
+		if (k1 <= k0 + 1) {
+			// At most one threshold in the bin: T[k0].
+			const EdRec r0 = d.rec[k0], r1 = d.rec[k0 + 1];
+			const double l2 = d.rec[k0 + 2].lin;
+			const bool up = s >= r0.T;
+			k = k0 + (up ? 1 : 0);
+			U = up ? r1.U : r0.U;
+			M = up ? r1.M : r0.M;
+			lk = up ? r1.lin : r0.lin;
+			lk1 = up ? l2 : r1.lin;
+		} else {
+			int a = k0, n = k1 - k0;
+			while (n > 0) {
+				const int half = n >> 1;
+				if (d.rec[a + half].T <= s) {
+					a += half + 1;
+					n -= half + 1;
+				} else {
+					n = half;
+				}
+			}
+			k = a;
+			U = d.rec[a].U;
+			M = d.rec[a].M;
+			lk = d.rec[a].lin;
+			lk1 = d.rec[a + 1].lin;
+		}
+	} else {
+		k = (k1 == k0) ? k0 : level_in(d.T, k0, k1, s);
+		U = d.U[k];  // U[P] = +inf: the top level is exact
+		M = d.M[k];  // M[P] = 2.0: never rounds up
+		lk = d.lin[k];
+		lk1 = d.lin[k < d.P ? k + 1 : d.P];
+	}
+	const bool exact = s < U, ceil_ = s >= M;
 	level = (exact || !ceil_) ? k : k + 1;
-	return exact ? 0.0 : s - t.lin[level];
+	return exact ? 0.0 : s - (ceil_ ? lk1 : lk);
 }
 
-// Level of a wide pixel's sample: the thresholds of its window are compared
-// eight at a time (independent LDS reads), the rest by bisection.
-__device__ __forceinline__ int wide_level(const double *T, int klo, int khi, double s)
+// One step of a walk: the pixel's (sample, error from earlier rows) and the
+// errors of the one or two pixels before it in its row -> its error and
+// level.  The carried terms are added in the reference's order, x-2 before
+// x-1 (separately rounded FP64 operations); the clamp is exact (the sum is
+// never NaN or -0.0).
+template <bool TAB, bool HAS_M1>
+__device__ __forceinline__ double ed_step(const EdDecide &d, double2 be, double p1, double p2, double m0, double m1,
+	int &level)
 {
-	int lv = klo;
-#pragma unroll
-	for (int q = 0; q < 8; q++) {
-		const int k = min(klo + q, khi - 1);
-		lv += (klo + q < khi && T[k] <= s) ? 1 : 0;
-	}
-	if (lv == klo + 8 && khi > klo + 8) lv = level_in(T, klo + 8, khi, s);
-	return lv;
+	const double cell = HAS_M1 ? (be.y + p2 * m1) + p1 * m0 : be.y + p1 * m0;
+	const double s = fmin(fmax(be.x + cell, 0.0), 1.0);
+	return ed_decide<TAB>(d, s, level);
 }
 
-template <bool HAS_M1>
-__device__ __forceinline__ double ed_step(const EdPix &p, double &e1, double &e2, double m0, double m1)
-{
-	const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
-	const double s = fmin(fmax(p.b + cell, 0.0), 1.0);  // never NaN or -0.0: exact clamp
-	const bool up = s >= p.t1;
-	const double uu = up ? p.u1 : p.u0, mm = up ? p.m1 : p.m0;
-	const double lo = up ? p.l1 : p.l0, hi = up ? p.l2 : p.l1;
-	const double base = (s >= mm) ? hi : lo;
-	const double err = (s < uu) ? 0.0 : s - base;
-	e2 = e1;
-	e1 = err;
-	return s;
-}
+template <bool TAB, bool ROWS>
+struct EdLds {
+	EdRec rec[TAB ? ED_LDS_TABLE + 3 : 1];         // levels 0..P, two guards
+	double cd[TAB ? ED_LDS_TABLE + 1 : 1];         // output code per level
+	unsigned short Q[ED_NB + 2];
+	double2 be[ROWS ? ED_ROWS_LDS : 1];            // (sample, carried error) by scan position
+	double err[ROWS ? ED_ROWS_LDS : 1];            // the row's errors by x
+	unsigned short lvl[ROWS ? ED_ROWS_LDS : 1];    // the row's levels by scan position
+	double sw[ED_THREADS][2], se[ED_THREADS][2];   // segment states: at its first pixel, after its last
+	unsigned long long fail[ED_THREADS / 64];      // boundaries whose states differ
+};
 
 struct EdCtx {
-	EdTab t;
-	g_cf64 *samp;
-	const double *m;     // the weights (c_ed_matrix row)
-	double *R1, *ERR;
-	double m0, m1, G1, G2;
-	int W, Pq, o, ordered;
-	// Output image, read once from the plan.
+	EdDecide d;
+	const double *cd;          // output code per level
+	g_cf64 *samp;              // the plane's clamped samples [H][W]
+	double *R1, *err;
+	double2 *be;
+	unsigned short *lvl;
+	double m0, m1;
+	int W, o;
 	g_u8 *out;
 	size_t out_bpr;
 	int out_nch, out_depth;
@@ -1637,335 +1583,130 @@ __device__ __forceinline__ void ed_store(const EdCtx &c, int i, int j, double s_
 	}
 }
 
-template <bool LDS_ROWS>
-__device__ __forceinline__ void ed_stage_a(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int g, int ch, int i,
-	bool prefetched = false)
+// Stage A of row j, all threads: each pixel's sample and the error carried
+// to it from the rows above -- what row j-2 left in R1, then row j-1's
+// sources x-2f, x-f, x, x+f, x+2f in that row's scan order with weights
+// m[6..2] (the reference's scatter, src/imagew-main.c:497-567, as an ordered
+// gather); R1[x] then takes row j-1's share for row j+1 (weights m[11..7]).
+__device__ __forceinline__ void ed_stage_a(const EdCtx &c, const double *m, int j)
 {
-	const int z = ch * ED_CH + i;
-	if (z >= c.W) return;
-	const int x = (j % 2) ? c.W - 1 - z : z;
-	const double b = prefetched ? S.b[g % 2][i] : c.samp[(size_t)j * c.W + x];
-	// The error carried to x from the rows above: what row j-2 left in R1,
-	// then row j-1's sources x-2f, x-f, x, x+f, x+2f in its scan order with
-	// weights m[6..2] -- the reference's per-pixel scatter (src/imagew-main.c:
-	// 497-567) as an ordered gather.  R1[x] then takes row j-1's share for
-	// row j+1 (weights m[11..7]).  Row j's own errors reach ERR only at
-	// chunks already walked, two chunks behind this stage.
-	double e = 0.0;
-	if (j > 0) {
-		const int fwd = ((j - 1) % 2) ? -1 : 1;
-		double a2 = 0.0;
-		e = c.R1[x];
-		for (int d = -2; d <= 2; d++) {
-			const int p = x + d * fwd;
-			if (p < 0 || p >= c.W) continue;
-			const double er = c.ERR[p];
-			e = e + er * c.m[4 - d];
-			a2 = a2 + er * c.m[9 - d];
-		}
-		c.R1[x] = a2;
-	}
-	const int kg0 = ed_level(c.t, clamp01(b + ((e - c.G2) - c.G1)));
-	const int kg1 = max(ed_level(c.t, clamp01(b + ((e + c.G2) + c.G1))), kg0);
-	S.eb[g % 3][i] = fmin(c.t.pre[kg1], c.t.suf[kg0]);
-	S.b[g % 2][i] = b;
-	S.e[g % 2][i] = e;
-	S.kg0[g % 2][i] = (unsigned short)kg0;
-	S.kg1[g % 2][i] = (unsigned short)kg1;
-}
-
-// Run by all 64 lanes of a helper wave (the wide flags are a ballot).  FAST:
-// the pixel's data for the fast walk (EdFast); a pixel is wide when
-// ed_fast_form has no data for its window.  Else EdPix; wide = three or more
-// levels.
-template <bool LDS_ROWS, bool FAST>
-__device__ __forceinline__ void ed_stage_b(EdLds<LDS_ROWS> &S, const EdCtx &c, int g, int ch, int i)
-{
-	const int z = ch * ED_CH + i;
-	bool wide = false;
-	if (z < c.W) {
-		const double b = S.b[g % 2][i], e = S.e[g % 2][i];
-		const int kg0 = S.kg0[g % 2][i], kg1 = S.kg1[g % 2][i];
-		// Bounds of the previous two pixels in scan order (none before the
-		// row's first pixel); chunk ch-1 is full when ch > 0.
-		const double eb1 = (i >= 1) ? S.eb[g % 3][i - 1] : (z >= 1 ? S.eb[(g + 2) % 3][ED_CH - 1] : 0.0);
-		const double eb2 = (i >= 2) ? S.eb[g % 3][i - 2] : (z >= 2 ? S.eb[(g + 2) % 3][ED_CH - 2 + i] : 0.0);
-		const double B1 = eb1 * c.m0, B2 = eb2 * c.m1;
-		const double sL = clamp01(b + ((e - B2) - B1)), sH = clamp01(b + ((e + B2) + B1));
-		// level_in(T, a, b, s) is level(s) clamped to [a, b].
-		const int klo = min(max(ed_level(c.t, sL), kg0), kg1);
-		const int khi = min(max(ed_level(c.t, sH), klo), kg1);
-		if (FAST) {
-			EdFast &f = S.fpx[g % 2][i];
-			f.b = b;
-			f.e = e;
-			wide = !ed_fast_form(c.t.T, c.t.U, c.t.M, c.t.lin, c.Pq, c.ordered != 0, klo, khi, sL, sH, f);
-		} else {
-			const int k1 = min(klo + 1, c.Pq), k2 = min(klo + 2, c.Pq);
-			EdPix &p = S.px[g % 2][i];
-			p.b = b;
-			p.e = e;
-			p.t1 = (khi > klo) ? c.t.T[klo] : 2.0;
-			p.u0 = c.t.U[klo];
-			p.m0 = c.t.M[klo];
-			p.u1 = c.t.U[k1];
-			p.m1 = c.t.M[k1];
-			p.l0 = c.t.lin[klo];
-			p.l1 = c.t.lin[k1];
-			p.l2 = c.t.lin[k2];
-			wide = khi > klo + 1;
-		}
-		S.klo[g % 3][i] = (unsigned short)klo;
-		S.khi[g % 3][i] = (unsigned short)khi;
-	}
-	const unsigned long long wm = __ballot(wide);
-	if ((i & 63) == 0) S.wide[g % 2][i >> 6] = wm;
-}
-
-// Completes pixel i of chunk (row j, chunk ch; slot g): the output code from
-// the walked s, and with WRITE_ERR the error (the fast walk writes it
-// itself).
-template <bool LDS_ROWS, bool WRITE_ERR>
-__device__ __forceinline__ void ed_finish(EdLds<LDS_ROWS> &S, const EdCtx &c, const DevPlan &P, int j, int g,
-	int ch, int i)
-{
-	const int z = ch * ED_CH + i;
-	if (z >= c.W) return;
-	const int x = (j % 2) ? c.W - 1 - z : z;
-	const double s = S.s[g % 3][i];
-	const int klo = S.klo[g % 3][i], khi = S.khi[g % 3][i];
-	const int lv = (khi > klo) ? wide_level(c.t.T, klo, khi, s) : klo;
-	int level;
-	const double err = ed_error(c.t, lv, s, level);
-	if (WRITE_ERR) c.ERR[x] = err;
-	ed_store(c, x, j, c.t.cd[level]);
-}
-
-template <bool LDS_ROWS, bool HAS_M1>
-__device__ __forceinline__ void ed_walk(EdLds<LDS_ROWS> &S, const EdCtx &c, int g, int ch, double &e1, double &e2)
-{
-	const int n = min(ED_CH, c.W - ch * ED_CH);
-	const EdPix *px = S.px[g % 2];
-	double *so = S.s[g % 3];
-	const unsigned long long *wm = S.wide[g % 2];
-	const double m0 = c.m0, m1 = c.m1;
-	// First wide pixel at or after a (n: none).
-	auto next_wide = [&](int a) {
-		for (int w = a >> 6; w < (n + 63) >> 6; w++) {
-			unsigned long long mk = wm[w];
-			if (w == (a >> 6)) mk &= ~0ull << (a & 63);
-			if (mk) return min(n, w * 64 + __ffsll((unsigned long long)mk) - 1);
-		}
-		return n;
-	};
-	// Runs of ordinary pixels between wide ones, four pixels per turn with
-	// operands loaded two pixels ahead (px has ED_CH + 2 entries, so index
-	// stop + 1 is in bounds).  Each run loads its first operands itself: the
-	// loop's loads then always have the same pending pattern, which keeps
-	// the compiler's LDS waits partial.
-	int a = 0;
-	while (a < n) {
-		const int stop = next_wide(a);
-		EdPix pa = px[a], pb = px[a + 1];
-		int k = a;
-		for (; k + 4 <= stop; k += 4) {
-			const EdPix pc = px[k + 2], pd = px[k + 3];
-			so[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
-			so[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
-			pa = px[k + 4];
-			pb = px[k + 5];
-			so[k + 2] = ed_step<HAS_M1>(pc, e1, e2, m0, m1);
-			so[k + 3] = ed_step<HAS_M1>(pd, e1, e2, m0, m1);
-		}
-		if (k < stop) so[k] = ed_step<HAS_M1>(pa, e1, e2, m0, m1);
-		if (k + 1 < stop) so[k + 1] = ed_step<HAS_M1>(pb, e1, e2, m0, m1);
-		if (k + 2 < stop) so[k + 2] = ed_step<HAS_M1>(px[k + 2], e1, e2, m0, m1);
-		if (stop < n) {
-			const EdPix &p = px[stop];
-			const double cell = HAS_M1 ? (p.e + e2 * m1) + e1 * m0 : p.e + e1 * m0;
-			const double s = fmin(fmax(p.b + cell, 0.0), 1.0);
-			const int lv = wide_level(c.t.T, S.klo[g % 3][stop], S.khi[g % 3][stop], s);
-			int level;
-			const double err = ed_error(c.t, lv, s, level);
-			so[stop] = s;
-			e2 = e1;
-			e1 = err;
-		}
-		a = stop + 1;
-	}
-}
-
-__device__ __forceinline__ double lane_f64(double v, int k)
-{
-	const unsigned long long u = __double_as_longlong(v);
-	const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, k);
-	const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), k);
-	return __longlong_as_double(((unsigned long long)hi << 32) | lo);
-}
-
-// One walk step of ed_fast_run (assembler text).
-#define ED_STEP \
-	"v_mul_f64 v[100:101], s[40:41], %[m0]\n\t" \
-	"v_add_f64 v[100:101], %[e], v[100:101]\n\t" \
-	"v_add_f64 %[s], %[b], v[100:101] clamp\n\t" \
-	"v_cmp_ge_f64 vcc, %[s], %[sm]\n\t" \
-	"v_lshl_add_u64 %[q], %[s], 0, %[nsa]\n\t" \
-	"s_nop 0\n\t" \
-	"v_cndmask_b32 v100, v104, v106, vcc\n\t" \
-	"v_cndmask_b32 v101, v105, v107, vcc\n\t" \
-	"v_cmp_gt_u64 %[mask], %[wdt], %[q]\n\t" \
-	"v_add_f64 v[102:103], %[s], -v[100:101]\n\t" \
-	"s_nop 0\n\t" \
-	"v_cndmask_b32 v102, v102, 0, %[mask]\n\t" \
-	"v_cndmask_b32 v103, v103, 0, %[mask]\n\t" \
-	"v_readfirstlane_b32 s40, v102\n\t" \
-	"v_readfirstlane_b32 s41, v103\n\t" \
-	"s_ff1_i32_b64 %[lane], exec\n\t" \
-	"s_bitset0_b64 exec, %[lane]\n\t"
-
-// Steps 2^bit of ed_fast_run when bit `bit` of the run length is set.
-#define ED_STEPS(bit, n, skip) \
-	"s_bitcmp1_b32 %[len], " #bit "\n\t" \
-	"s_cbranch_scc0 " #skip "f\n\t" \
-	".rept " #n "\n\t" ED_STEP ".endr\n\t" \
-	#skip ":\n\t"
-
-// The 64 steps of a full run (len = 64: no lower bit is set).
-#define ED_STEPS64(skip, done) \
-	"s_bitcmp1_b32 %[len], 6\n\t" \
-	"s_cbranch_scc0 " #skip "f\n\t" \
-	".rept 64\n\t" ED_STEP ".endr\n\t" \
-	"s_branch " #done "f\n\t" \
-	#skip ":\n\t"
-
-// The walk steps of a run of `len` ordinary pixels (1..64), one per lane:
-// exec is the run's lanes, each step walks the first active lane -- its
-// error becomes the carried x (v_readfirstlane) -- and then clears that lane
-// from exec, so the lane keeps its s and error.  The steps are straight-line
-// groups of 64, 32, ..., 1 taken by the bits of len, so exactly len steps
-// run and x ends as the run's last error.  Per step:
-//   s = clamp(b + (e + x*m0)); base = s >= sm ? hi : lo;
-//   err = (bits(s) - bits(sa) < wdt, unsigned) ? 0 : s - base.
-// The products and sums are separately rounded FP64 operations in the
-// reference's order (src/imagew-main.c:702-709); v_add_f64's clamp is the
-// [0,1] clamp (the sum is never NaN or -0.0).  Two wait states separate each
-// VALU write of an SGPR (compare, v_readfirstlane) from its VALU reader.
-// Fixed registers: x in s[40:41]; v[100:101] the base, v[102:103] the error,
-// v[104:105] lo, v[106:107] hi.
-__device__ __forceinline__ void ed_fast_run(const EdFast &f, double m0, unsigned long long run, int len, double &x,
-	double &s, double &err)
-{
-	unsigned long long save, q, mask;
-	unsigned lane;
-	double t;
-	{
-		// x is the same in every lane; say so, for the SGPR operand.
-		const unsigned long long u = __double_as_longlong(x);
-		const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
-		const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
-		x = __longlong_as_double(((unsigned long long)hi << 32) | lo);
-	}
-	asm volatile(
-		"s_mov_b64 %[save], exec\n\t"
-		"s_and_b64 exec, exec, %[run]\n\t"
-		"s_nop 0\n\t"
-		ED_STEPS64(2, 9)
-		ED_STEPS(5, 32, 3)
-		ED_STEPS(4, 16, 4)
-		ED_STEPS(3, 8, 5)
-		ED_STEPS(2, 4, 6)
-		ED_STEPS(1, 2, 7)
-		ED_STEPS(0, 1, 8)
-		"9:\n\t"
-		"s_mov_b64 exec, %[save]\n\t"
-		"s_nop 1"
-		: [x] "+{s[40:41]}"(x), [s] "+v"(s), "+{v[102:103]}"(err), "=&{v[100:101]}"(t), [save] "=&s"(save),
-		  [q] "=&v"(q), [mask] "=&s"(mask), [lane] "=&s"(lane)
-		: [m0] "v"(m0), [e] "v"(f.e), [b] "v"(f.b), [sm] "v"(f.sm), [nsa] "v"(f.nsa), [wdt] "v"(f.wdt),
-		  "{v[104:105]}"(f.lo), "{v[106:107]}"(f.hi), [run] "s"(run), [len] "s"(len)
-		: "vcc", "scc");
-	(void)t;
-}
-
-#undef ED_STEPS64
-#undef ED_STEPS
-#undef ED_STEP
-
-// The walk of chunk ch of row j (slot g) for weights without an x+2 carry,
-// by all 64 lanes of wave 0 (lane l: pixel 64*blk + l of the chunk).  Runs of ordinary pixels
-// go through ed_fast_run.  A wide pixel between them is stepped by all
-// lanes: they compare its sample with 64 thresholds of its window at a time,
-// and the count of those <= s (thresholds increase) gives its level.
-template <bool LDS_ROWS>
-__device__ __forceinline__ void ed_walk_fast(EdLds<LDS_ROWS> &S, const EdCtx &c, int j, int g, int ch, double &x)
-{
-	static_assert(ED_CH == 3 * 64, "three blocks per chunk");
-	const int lane = threadIdx.x;
-	const int n = min(ED_CH, c.W - ch * ED_CH);
-	const EdFast *px = S.fpx[g % 2];
-	// The chunk's wide masks, then the first block's operands: LDS returns
-	// in order, so each later wait covers only what it needs.
-	const unsigned long long wide[3] = {S.wide[g % 2][0], S.wide[g % 2][1], S.wide[g % 2][2]};
-	EdFast f = px[lane];
-	for (int blk = 0; blk * 64 < n; blk++) {
-		const int base = blk * 64, nb = min(64, n - base);
-		// Operands of the next block, loaded while this one is walked (px has
-		// ED_CH + 2 entries; base + 64 + lane is in bounds for blk < 2).
-		EdFast fn = f;
-		if (base + 64 < n) fn = px[base + 64 + lane];
-		const unsigned long long wm = wide[blk] & (nb == 64 ? ~0ull : ((1ull << nb) - 1));
-		double s = 0.0, err = 0.0;
-		if (wm == 0) {
-			// The common block: one run over all its pixels.
-			ed_fast_run(f, c.m0, nb == 64 ? ~0ull : ((1ull << nb) - 1), nb, x, s, err);
-		} else {
-			const int klo = S.klo[g % 3][base + lane], khi = S.khi[g % 3][base + lane];
-			int a = 0;
-			while (a < nb) {
-				const unsigned long long rest = wm >> a;
-				const int stop = rest ? min(nb, a + __ffsll((unsigned long long)rest) - 1) : nb;
-				if (stop > a) {
-					const int len = stop - a;
-					const unsigned long long run = (len == 64) ? ~0ull : (((1ull << len) - 1) << a);
-					ed_fast_run(f, c.m0, run, len, x, s, err);
-				}
-				if (stop < nb) {
-					const double sl = fmin(fmax(f.b + (f.e + x * c.m0), 0.0), 1.0);
-					const double sk = lane_f64(sl, stop);
-					const int klw = __builtin_amdgcn_readlane(klo, stop), khw = __builtin_amdgcn_readlane(khi, stop);
-					int lv = klw;
-					for (int k0 = klw; k0 < khw; k0 += 64) {
-						const int k = k0 + lane;
-						const int cnt = __popcll(__ballot(k < khw && c.t.T[k] <= sk));
-						lv += cnt;
-						if (cnt < 64) break;
-					}
-					// The reference's decision at level lv (as ed_error).
-					const double u = c.t.U[lv], m = c.t.M[lv], l0 = c.t.lin[lv], l1 = c.t.lin[min(lv + 1, c.Pq)];
-					x = (sk < u) ? 0.0 : sk - ((sk >= m) ? l1 : l0);
-					s = (lane == stop) ? sl : s;
-					err = (lane == stop) ? x : err;
-				}
-				a = stop + 1;
+	const int W = c.W;
+	const bool rev = (j & 1) != 0;
+	const int fwd = (j & 1) ? 1 : -1;  // row j-1's direction
+	for (int z = threadIdx.x; z < W; z += ED_THREADS) {
+		const int x = rev ? W - 1 - z : z;
+		const double b = c.samp[(size_t)j * W + x];
+		double e = 0.0;
+		if (j > 0) {
+			double a2 = 0.0;
+			e = c.R1[x];
+			for (int dd = -2; dd <= 2; dd++) {
+				const int p = x + dd * fwd;
+				if (p < 0 || p >= W) continue;
+				const double er = c.err[p];
+				e = e + er * m[4 - dd];
+				a2 = a2 + er * m[9 - dd];
 			}
+			c.R1[x] = a2;
 		}
-		// Each lane's s for the finish and its error for the rows below
-		// (stage A reads ERR two chunks ahead of the walk).
-		if (lane < nb) {
-			S.s[g % 3][base + lane] = s;
-			const int z = ch * ED_CH + base + lane;
-			c.ERR[(j % 2) ? c.W - 1 - z : z] = err;
+		c.be[z] = make_double2(b, e);
+	}
+}
+
+// The speculative walk of segment t: pixels [zs, ze) of the row in scan
+// order, started `warm` pixels early with no in-row carry; the pixels of the
+// segment keep their errors and levels, and the states at zs and after the
+// last pixel go to LDS for the checks.
+template <bool TAB, bool HAS_M1, class Lds>
+__device__ __forceinline__ void ed_walk_seg(const EdCtx &c, Lds &S, bool rev, int t, int seg, int warm)
+{
+	const int W = c.W;
+	const int zs = t * seg, ze = min(W, zs + seg), z0 = max(0, zs - warm);
+	double p1 = 0.0, p2 = 0.0;
+	int level;
+	double2 v = c.be[z0];
+	int z = z0;
+	for (; z < zs; z++) {
+		const double2 nv = c.be[z + 1];  // z + 1 <= zs < W
+		const double e = ed_step<TAB, HAS_M1>(c.d, v, p1, p2, c.m0, c.m1, level);
+		p2 = p1;
+		p1 = e;
+		v = nv;
+	}
+	S.sw[t][0] = p1;
+	S.sw[t][1] = p2;
+	for (; z < ze; z++) {
+		const double2 nv = c.be[min(z + 1, W - 1)];
+		const double e = ed_step<TAB, HAS_M1>(c.d, v, p1, p2, c.m0, c.m1, level);
+		c.err[rev ? W - 1 - z : z] = e;
+		c.lvl[z] = (unsigned short)level;
+		p2 = p1;
+		p1 = e;
+		v = nv;
+	}
+	S.se[t][0] = p1;
+	S.se[t][1] = p2;
+}
+
+// Wave 0: the boundary checks, then (lane 0) the true chain through every
+// segment whose start state differs from its predecessor's end state.
+template <bool TAB, bool HAS_M1, class Lds>
+__device__ __forceinline__ void ed_resolve(const EdCtx &c, Lds &S, bool rev, int nseg, int seg)
+{
+	const int lane = threadIdx.x;
+	for (int k0 = 0; k0 < nseg; k0 += 64) {
+		const int k = k0 + lane;
+		const bool bad = k >= 1 && k < nseg &&
+			!(S.se[k - 1][0] == S.sw[k][0] && (!HAS_M1 || S.se[k - 1][1] == S.sw[k][1]));
+		const unsigned long long mk = __ballot(bad);
+		if (lane == 0) S.fail[k0 >> 6] = mk;
+	}
+	if (lane != 0) return;
+	auto next_fail = [&](int k) {
+		while (k < nseg) {
+			const unsigned long long mk = S.fail[k >> 6] >> (k & 63);
+			if (mk) return k + __ffsll((unsigned long long)mk) - 1;
+			k = (k | 63) + 1;
 		}
-		f = fn;
+		return nseg;
+	};
+	const int W = c.W;
+	int k = next_fail(1);
+	while (k < nseg) {
+		// Segments before k are verified, so segment k-1's end state is the
+		// true chain's.  Walk on until the state equals the speculative one
+		// at the same pixel (for the x+2 carry, both errors, from the same
+		// segment's walk); that segment is right from there to its end.
+		double p1 = S.se[k - 1][0], p2 = S.se[k - 1][1];
+		double pe = 0.0, po = 0.0;
+		int z = k * seg, own = k;
+		bool met = false;
+		for (; z < W; z++) {
+			own = z / seg;
+			const int x = rev ? W - 1 - z : z;
+			int level;
+			const double old = c.err[x];
+			const double e = ed_step<TAB, HAS_M1>(c.d, c.be[z], p1, p2, c.m0, c.m1, level);
+			c.err[x] = e;
+			c.lvl[z] = (unsigned short)level;
+			met = e == old && (!HAS_M1 || (z > own * seg && pe == po));
+			pe = e;
+			po = old;
+			p2 = p1;
+			p1 = e;
+			if (met) break;
+		}
+		if (!met) break;
+		k = next_fail(own + 1);
 	}
 }
 
 // One workgroup per plane: planes[b] = (plan, output channel).
-template <bool LDS_TAB, bool HAS_M1, bool LDS_ROWS>
+template <bool TAB, bool HAS_M1, bool ROWS>
 __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__restrict__ plans,
 	const int2 *__restrict__ planes)
 {
-	__shared__ EdLds<LDS_ROWS> S;
+	__shared__ EdLds<TAB, ROWS> S;
 	const int tid = threadIdx.x;
 	const int2 pl = planes[blockIdx.x];
 	const DevPlan &P = plans[pl.x];
@@ -1976,106 +1717,82 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 
 	EdCtx c;
 	c.W = W;
-	c.Pq = Pq;
 	c.o = o;
 	c.m0 = m[0];
 	c.m1 = m[1];
-	c.ordered = q.ed_ordered;
-	// Global bounds on the in-row carries, for the first window.
-	c.G1 = q.eb_max * c.m0;
-	c.G2 = q.eb_max * c.m1;
 	c.samp = (g_cf64 *)(P.ed_samples + (size_t)q.ed_index * H * W);
 	c.out = (g_u8 *)P.out;
 	c.out_bpr = P.out_bpr;
 	c.out_nch = P.out_nch;
 	c.out_depth = P.out_depth;
-	c.m = m;
-	if (LDS_ROWS) {
-		c.R1 = S.rows;
-		c.ERR = S.rows + ED_ROWS_LDS;
+	double *rows = P.ed_err + (size_t)q.ed_index * ed_row_doubles(W);
+	c.R1 = rows;
+	if (ROWS) {
+		c.err = S.err;
+		c.be = S.be;
+		c.lvl = S.lvl;
 	} else {
-		c.R1 = P.ed_err + ((size_t)q.ed_index * 4 + 1) * W;
-		c.ERR = P.ed_err + ((size_t)q.ed_index * 4 + 2) * W;
+		c.err = rows + W;
+		c.be = (double2 *)(rows + 2 * (size_t)W);
+		c.lvl = (unsigned short *)(rows + 4 * (size_t)W);
 	}
-	c.t.P = Pq;
-	if (LDS_TAB) {
-		for (int k = tid; k <= Pq; k += ED_THREADS) {
-			if (k < Pq) S.T[k] = ((g_cf64 *)q.T)[k];
-			S.U[k] = ((g_cf64 *)q.U)[k];
-			S.M[k] = ((g_cf64 *)q.M)[k];
-			S.lin[k] = ((g_cf64 *)q.lin)[k];
-			S.cd[k] = ((g_cf64 *)q.code)[k];
-			S.pre[k] = ((g_cf64 *)q.ebpre)[k];
-			S.suf[k] = ((g_cf64 *)q.ebsuf)[k];
+	c.d.P = Pq;
+	c.d.T = q.T;
+	c.d.U = q.U;
+	c.d.M = q.M;
+	c.d.lin = q.lin;
+	if (TAB) {
+		for (int k = tid; k <= Pq + 2; k += ED_THREADS) {
+			EdRec r;
+			if (k <= Pq) {
+				r.T = k < Pq ? ((g_cf64 *)q.T)[k] : 2.0;
+				r.U = ((g_cf64 *)q.U)[k];
+				r.M = ((g_cf64 *)q.M)[k];
+				r.lin = ((g_cf64 *)q.lin)[k];
+				S.cd[k] = ((g_cf64 *)q.code)[k];
+			} else {
+				r.T = 2.0;
+				r.U = 2.0;
+				r.M = 2.0;
+				r.lin = ((g_cf64 *)q.lin)[Pq];
+			}
+			S.rec[k] = r;
 		}
-		c.t.T = S.T; c.t.U = S.U; c.t.M = S.M; c.t.lin = S.lin; c.t.cd = S.cd; c.t.pre = S.pre; c.t.suf = S.suf;
+		c.d.rec = S.rec;
+		c.cd = S.cd;
 	} else {
-		c.t.T = q.T; c.t.U = q.U; c.t.M = q.M; c.t.lin = q.lin; c.t.cd = q.code; c.t.pre = q.ebpre; c.t.suf = q.ebsuf;
+		c.d.rec = nullptr;
+		c.cd = q.code;
 	}
+	for (int k = tid; k <= ED_NB; k += ED_THREADS)
+		S.Q[k] = (unsigned short)level_in(q.T, 0, Pq, k / (double)ED_NB);
+	c.d.Q = S.Q;
 	for (int x = tid; x < W; x += ED_THREADS) c.R1[x] = 0.0;
 	__syncthreads();
-	for (int k = tid; k <= ED_QBINS; k += ED_THREADS) S.Q[k] = (unsigned short)level_in(c.t.T, 0, Pq, k / (double)ED_QBINS);
-	c.t.Q = S.Q;
+
+	int nseg = min(ED_THREADS, max(1, W / ED_SEG_MIN));
+	const int seg = (W + nseg - 1) / nseg;
+	nseg = (W + seg - 1) / seg;
+	const int warm = q.ed_warm;
+
+	ed_stage_a(c, m, 0);
 	__syncthreads();
-
-	const bool helper = tid >= 64;
-	const int h = tid - 64;  // helper's pixel in a chunk
-	const int nch = (W + ED_CH - 1) / ED_CH;
-	double e1 = 0.0, e2 = 0.0;  // walker: errors of the previous two pixels
-
-	// Chunks are numbered across rows (g = j*nch + ch) and take their
-	// buffers by g, so the pipeline runs through row ends: at a row's start
-	// only stage A of its first chunk, then stage B of it (with stage A of
-	// the second) precede the walk.  The fast walk writes the errors itself,
-	// so the first chunk's stage A needs only the previous row's walk, and
-	// that row's last chunk is completed in the row's first step; the x+2
-	// walk completes it first.
 	for (int j = 0; j < H; j++) {
-		const int g0 = j * nch;
-		if (HAS_M1 && j > 0) {
-			// The x+2 walk leaves the errors to the finish.
-			if (helper) ed_finish<LDS_ROWS, true>(S, c, P, j - 1, g0 - 1, nch - 1, h);
-			__syncthreads();
-		}
-		if (helper) ed_stage_a(S, c, j, g0, 0, h, j > 0);
+		const bool rev = (j & 1) != 0;
+		if (tid < nseg) ed_walk_seg<TAB, HAS_M1>(c, S, rev, tid, seg, warm);
 		__syncthreads();
-		if (helper) {
-			ed_stage_b<LDS_ROWS, !HAS_M1>(S, c, g0, 0, h);
-			if (nch > 1) ed_stage_a(S, c, j, g0 + 1, 1, h, j > 0);
-		}
+		if (tid < 64) ed_resolve<TAB, HAS_M1>(c, S, rev, nseg, seg);
 		__syncthreads();
-		e1 = 0.0;
-		e2 = 0.0;
-		for (int ch = 0; ch < nch; ch++) {
-			const int g = g0 + ch;
-			if (HAS_M1 ? tid == 0 : tid < 64) {
-				if constexpr (HAS_M1) ed_walk<LDS_ROWS, HAS_M1>(S, c, g, ch, e1, e2);
-				else ed_walk_fast(S, c, j, g, ch, e1);
-			} else if (helper) {
-				if (ch >= 1) ed_finish<LDS_ROWS, HAS_M1>(S, c, P, j, g - 1, ch - 1, h);
-				else if (!HAS_M1 && j >= 1) ed_finish<LDS_ROWS, false>(S, c, P, j - 1, g - 1, nch - 1, h);
-				if (ch + 1 < nch) ed_stage_b<LDS_ROWS, !HAS_M1>(S, c, g + 1, ch + 1, h);
-				if (ch + 2 < nch) ed_stage_a(S, c, j, g + 2, ch + 2, h);
-				if (ch == nch - 1 && j + 1 < H) {
-					// No stage reads S.b during a row's last step: load the
-					// next row's first two chunks of samples (slots g+1,
-					// g+2) ahead of the row's end.
-					const int jn = j + 1;
-					for (int q = 0; q < 2 && q < nch; q++) {
-						const int z = q * ED_CH + h;
-						if (z < W) S.b[(g + 1 + q) % 2][h] = c.samp[(size_t)jn * W + ((jn % 2) ? W - 1 - z : z)];
-					}
-				}
-			}
-			__syncthreads();
-		}
+		// Row j's codes, and the next row's samples and carried errors
+		// (stage A reads row j's errors; the next walk overwrites them).
+		for (int z = tid; z < W; z += ED_THREADS) ed_store(c, rev ? W - 1 - z : z, j, c.cd[c.lvl[z]]);
+		if (j + 1 < H) ed_stage_a(c, m, j + 1);
+		__syncthreads();
 	}
-	if (helper) ed_finish<LDS_ROWS, HAS_M1>(S, c, P, H - 1, H * nch - 1, nch - 1, h);
 	if (P.ed_done) {
 		// Completion of the request: the last of its planes to finish copies
 		// the whole output to the host and raises the flag.
 		__shared__ int last;
-		__syncthreads();
 		if (tid == 0) {
 			__threadfence();
 			last = atomicAdd(P.ed_done, 1) == P.n_ed - 1;
@@ -2232,7 +1949,7 @@ namespace {
 // Host side of one job's device data: the tables, where each lives in the
 // job's part of a device block, and the resampling path.
 struct WOff { size_t start, src, w, copy, dfirst, dcount, doff, dw; };
-struct QOff { size_t T, U, lin, code, nc, ix, D, prow, pcol, M, pre, suf; };
+struct QOff { size_t T, U, lin, code, nc, ix, D, prow, pcol, M; };
 
 struct Staged {
 	std::vector<iwx::WeightList> wv, wh;
@@ -2523,7 +2240,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	S.out_bytes = p.out_bpr * (size_t)p.out_height;
 	S.o_out = out_elsewhere ? 0 : L.add<uint8_t>(S.out_bytes);
 	S.o_eds = L.add<double>((size_t)S.n_ed * p.out_height * p.out_width);
-	S.o_ede = L.add<double>((size_t)S.n_ed * 4 * p.out_width);
+	S.o_ede = L.add<double>((size_t)S.n_ed * ed_row_doubles(p.out_width));
 	S.o_tables = L.add<uint8_t>(0);
 	S.o_lut.resize(S.luts.size());
 	for (size_t i = 0; i < S.luts.size(); i++) S.o_lut[i] = L.add<double>(S.luts[i].size());
@@ -2560,8 +2277,6 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 		q.prow = L.add<uint64_t>(S.pt[o].row_q.size());
 		q.pcol = L.add<uint64_t>(S.pt[o].col_mont.size());
 		q.M = L.add<double>(S.et[o].M.size());
-		q.pre = L.add<double>(S.et[o].pre.size());
-		q.suf = L.add<double>(S.et[o].suf.size());
 	}
 	S.bytes = (L.off + 255) & ~(size_t)255;
 	return true;
@@ -2611,8 +2326,6 @@ bool upload_job(Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
 		up(q.prow, S.pt[o].row_q.data(), S.pt[o].row_q.size() * 8);
 		up(q.pcol, S.pt[o].col_mont.data(), S.pt[o].col_mont.size() * 8);
 		up(q.M, S.et[o].M.data(), S.et[o].M.size() * 8);
-		up(q.pre, S.et[o].pre.data(), S.et[o].pre.size() * 8);
-		up(q.suf, S.et[o].suf.data(), S.et[o].suf.size() * 8);
 	}
 	if (!S.pack.empty())
 		ok &= hipMemcpyAsync(base + t0, S.pack.data(), S.pack.size(), hipMemcpyHostToDevice, st) == hipSuccess;
@@ -2620,6 +2333,20 @@ bool upload_job(Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
 }
 
 // Kernel arguments of a staged job placed at base.
+// Pixels each segment's walk takes before its own first pixel.  Walks with
+// equal levels close their gap by the in-row weights at every pixel, so the
+// lead that makes a segment's start state the true one is set by the
+// weights (slower with an x+2 carry) and by how often the walks' levels
+// differ (more often with few, widely spaced levels); a start state that is
+// still off costs a serial walk to where it meets the speculative one.
+int ed_warm(const iwamd_ochan &o, int P)
+{
+	const int sub = (o.dithersubtype >= 0 && o.dithersubtype <= 7) ? o.dithersubtype : 0;
+	const bool m1 = sub != 0 && sub != 6;
+	if (P < 16) return 128;
+	return m1 ? 96 : 64;
+}
+
 void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 {
 	memset(&D, 0, sizeof(D));
@@ -2722,10 +2449,7 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 			d.prng_row = (const uint64_t *)(base + q.prow);
 			d.prng_col = (const uint64_t *)(base + q.pcol);
 			d.M = (const double *)(base + q.M);
-			d.ebpre = (const double *)(base + q.pre);
-			d.ebsuf = (const double *)(base + q.suf);
-			d.eb_max = S.et[o].eb_max;
-			d.ed_ordered = S.et[o].ordered ? 1 : 0;
+			d.ed_warm = ed_warm(s, S.qt[o].P);
 		}
 	}
 	D.ed_samples = (double *)(base + S.o_eds);
@@ -3358,22 +3082,35 @@ extern "C" int iwamd_tune_entries(struct iwamd_tune_entry *out, int max)
 	return n;
 }
 
-// ed_fast_form and ed_fast_err on the host, for tests: the error of each
-// sample s[i] in [sL, sH] under the fast walk's data for that window.
-// Returns 1, 0 when the window needs the table search, -1 for a
-// nearest-colour plan.
-extern "C" int iwamd_ed_fast_check(const struct iwamd_ochan *och, double sL, double sH, int n, const double *s,
-	double *err)
+// ed_decide on the host, for tests: the error and level of each sample
+// s[i] in [0, 1] through the same tables the walk reads -- the level records
+// and bin index (tab = 1, as for up to 257 levels) or the bisection over the
+// thresholds (tab = 0).  Returns 1, 0 when tab = 1 and the channel has more
+// levels than the records hold, -1 for a nearest-colour plan.
+extern "C" int iwamd_ed_decide_check(const struct iwamd_ochan *och, int tab, int n, const double *s, double *err,
+	int *level)
 {
 	if (och->use_nc_table) return -1;
 	const iwx::QuantTable q = iwx::build_quant_table(*och, false);
 	const iwx::EdTables t = iwx::build_ed_tables(q);
-	auto level = [&](double v) { return (int)(std::upper_bound(q.T.begin(), q.T.end(), v) - q.T.begin()); };
-	EdFast f;
-	if (!ed_fast_form(q.T.data(), q.U.data(), t.M.data(), q.lin.data(), q.P, t.ordered, level(sL), level(sH), sL, sH,
-		    f))
-		return 0;
-	for (int i = 0; i < n; i++) err[i] = ed_fast_err(f, s[i]);
+	const int P = q.P;
+	if (tab && P > ED_LDS_TABLE) return 0;
+	std::vector<unsigned short> Q(ED_NB + 2);
+	for (int k = 0; k <= ED_NB; k++) Q[k] = (unsigned short)level_in(q.T.data(), 0, P, k / (double)ED_NB);
+	std::vector<EdRec> rec(P + 3);
+	for (int k = 0; k <= P + 2; k++) {
+		if (k <= P) rec[k] = EdRec{k < P ? q.T[k] : 2.0, q.U[k], t.M[k], q.lin[k]};
+		else rec[k] = EdRec{2.0, 2.0, 2.0, q.lin[P]};
+	}
+	EdDecide d;
+	d.rec = rec.data();
+	d.Q = Q.data();
+	d.T = q.T.data();
+	d.U = q.U.data();
+	d.M = t.M.data();
+	d.lin = q.lin.data();
+	d.P = P;
+	for (int i = 0; i < n; i++) err[i] = tab ? ed_decide<true>(d, s[i], level[i]) : ed_decide<false>(d, s[i], level[i]);
 	return 1;
 }
 

@@ -424,37 +424,7 @@ EdTables build_ed_tables(const QuantTable &q)
 	EdTables t;
 	const int P = q.P;
 	t.M.assign(P + 1, 2.0);
-	t.eb.assign(P + 1, 0.0);
-	for (int k = 0; k < P; k++) {
-		const double lo = q.lin[k], hi = q.lin[k + 1];
-		const double m = round_up_point(lo, hi);
-		t.M[k] = m;
-		// Non-exact samples of level k: [U[k], T[k]).
-		const double u = q.U[k], tk = q.T[k];
-		double e = 0.0;
-		auto at = [&](double s, double base) { e = std::max(e, fabs(s - base)); };
-		const double fend = std::min(m, tk);
-		if (u < fend) {
-			at(u, lo);
-			at(nextafter(fend, -1.0), lo);
-		}
-		const double cbeg = std::max(u, m);
-		if (cbeg < tk) {
-			at(cbeg, hi);
-			at(nextafter(tk, -1.0), hi);
-		}
-		t.eb[k] = e;
-	}
-	t.pre.resize(P + 1);
-	t.suf.resize(P + 1);
-	double run = 0.0;
-	for (int k = 0; k <= P; k++) t.pre[k] = run = std::max(run, t.eb[k]);
-	run = 0.0;
-	for (int k = P; k >= 0; k--) t.suf[k] = run = std::max(run, t.eb[k]);
-	t.eb_max = t.pre[P];
-	t.ordered = true;
-	for (int k = 0; k < P; k++)
-		if (!(t.M[k] <= q.T[k] && q.T[k] <= t.M[k + 1])) t.ordered = false;
+	for (int k = 0; k < P; k++) t.M[k] = round_up_point(q.lin[k], q.lin[k + 1]);
 	return t;
 }
 
@@ -574,7 +544,7 @@ PrngTables build_prng_tables(int seed, int width, int height)
 
 }  // namespace iwx
 
-extern "C" int iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, double *M, double *eb)
+extern "C" int iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, double *M)
 {
 	if (och->use_nc_table) return -1;
 	const iwx::QuantTable q = iwx::build_quant_table(*och, false);
@@ -582,7 +552,6 @@ extern "C" int iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, doubl
 	for (int k = 0; k <= q.P; k++) {
 		lin[k] = q.lin[k];
 		M[k] = t.M[k];
-		eb[k] = t.eb[k];
 	}
 	return q.P;
 }

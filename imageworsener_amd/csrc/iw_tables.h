@@ -86,16 +86,9 @@ QuantTable build_quant_table(const iwamd_ochan &oc, bool want_nc, bool fold_none
 // Decision data for error diffusion over a QuantTable (src/imagew-main.c:
 // 730-748).  A non-exact sample s at level k is rounded up iff
 // (lin[k+1] - s) <= (s - lin[k]) in FP64; both differences are monotone in s,
-// so that holds exactly for s >= M[k] (M[P] = 2: never).  eb[k] is the largest
-// |error| any s at level k can leave -- found exactly, as the error is
-// monotone on each of the floor and ceil ranges -- and pre/suf are its prefix
-// and suffix maxima, so max eb over levels [a, b] <= min(pre[b], suf[a]).
+// so that holds exactly for s >= M[k] (M[P] = 2: never).
 struct EdTables {
-	std::vector<double> M, eb, pre, suf;
-	double eb_max = 0.0;
-	// M[k] <= T[k] <= M[k+1] for every k < P: a sample past level k's top
-	// rounds up from k, and one below it does not round up from k+1.
-	bool ordered = false;
+	std::vector<double> M;
 };
 
 EdTables build_ed_tables(const QuantTable &q);

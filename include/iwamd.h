@@ -143,10 +143,10 @@ IW_EXPORT(int) iwamd_quant_thresholds(int cstype, double gamma, int P, double *T
 /* The error-diffusion decision data for an output channel (och as in a plan,
  * quantised to P+1 levels): lin[k] the level's value in linear light,
  * M[k] the least s at which a non-exact sample of level k rounds up
- * ((lin[k+1]-s) <= (s-lin[k]) in FP64; 2.0 when never, M[P] = 2.0), eb[k] the
- * largest |error| a sample of level k can leave.  Arrays hold P+1 entries;
- * returns P, or -1 for a floating-point or nearest-colour channel. */
-IW_EXPORT(int) iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, double *M, double *eb);
+ * ((lin[k+1]-s) <= (s-lin[k]) in FP64; 2.0 when never, M[P] = 2.0).  Arrays
+ * hold P+1 entries; returns P, or -1 for a floating-point or nearest-colour
+ * channel. */
+IW_EXPORT(int) iwamd_ed_tables(const struct iwamd_ochan *och, double *lin, double *M);
 /* The autotuner's table (no counterpart in the reference).  Band height of
  * the band kernel and tile width of the chosen-precision form change only
  * how the work is cut, not the bytes, so each class of jobs -- form,
@@ -167,13 +167,16 @@ struct iwamd_tune_entry {
 };
 IW_EXPORT(int) iwamd_tune_entries(struct iwamd_tune_entry *out, int max);
 
-/* The fast error-diffusion walk's decision for samples s[0..n) of a window
- * [sL, sH]: err[i] as the walk computes it.  Returns 1, 0 when the window
- * needs the table search, -1 for a nearest-colour plan.  No counterpart in
- * the reference: it checks the device's restatement of the decision at
+/* The error-diffusion walk's decision for samples s[0..n) in [0, 1]: err[i]
+ * and the output level level[i] as the walk computes them, through the level
+ * records and bin index it keeps in LDS (tab = 1) or the bisection it runs
+ * over the thresholds for channels with more than 257 levels (tab = 0).
+ * Returns 1, 0 when tab = 1 and the channel has more levels than the records
+ * hold, -1 for a nearest-colour plan.  No counterpart in the reference: it
+ * checks the device's restatement of the decision at
  * src/imagew-main.c:712-748. */
-IW_EXPORT(int) iwamd_ed_fast_check(const struct iwamd_ochan *och, double sL, double sH, int n, const double *s,
-	double *err);
+IW_EXPORT(int) iwamd_ed_decide_check(const struct iwamd_ochan *och, int tab, int n, const double *s, double *err,
+	int *level);
 /* The no-dither quantiser for samples s[0..n) in [0, 1] as the kernels run
  * it (folded boundaries, bin index): code[i].  Returns 1, -1 for a
  * nearest-colour plan.  No counterpart in the reference: it checks the
