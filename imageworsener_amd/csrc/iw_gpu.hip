@@ -1435,17 +1435,63 @@ __constant__ double c_ed_matrix[8][12] = {
 	{2.0/4, 0.0, 0.0, 1.0/4, 1.0/4, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                 // Sierra 4-2-1
 	{1.0/8, 1.0/8, 0.0, 1.0/8, 1.0/8, 1.0/8, 0.0, 0.0, 0.0, 1.0/8, 0.0, 0.0}};          // Atkinson
 
-#define ED_THREADS 256     // one workgroup per plane; up to one segment per thread
-#define ED_LDS_TABLE 256   // largest level count - 1 whose tables live in LDS
-#define ED_ROWS_LDS 4096   // widest row whose working rows live in LDS
-#define ED_NB 4096         // bins of the level index (s * 4096 is exact)
-#define ED_SEG_MIN 8       // fewest pixels per segment
+#define ED_WALKERS 256    // waves 0-3: up to one segment per thread
+#define ED_THREADS 512    // waves 4-7 help: next row's samples, previous row's codes
+#define ED_LDS_TABLE 256  // largest level count - 1 whose tables live in LDS
+#define ED_ROWS_LDS 3072  // widest row whose working rows live in LDS
+#define ED_NB 4096        // bins of the level index (s * 4096 is exact)
+#define ED_SEG_MIN 8      // fewest pixels per segment
 
-// Per-plane working rows in HBM, in doubles: R1 (the share of row j-1's
-// errors for row j+1), err, (sample, carried error) pairs, levels (u16).
-__host__ __device__ inline size_t ed_row_doubles(int W)
+// How a row of W pixels is cut: nseg segments of seg pixels (the last one
+// shorter).  Rows are stored transposed -- pixel z (scan order) at
+// (z % seg) * nsp + z / seg -- so that the segments' walks, which advance in
+// step, read and write consecutive elements; nsp (odd) spreads the row
+// stage's strided accesses over the LDS banks.
+struct EdCut {
+	int nseg, seg, nsp;
+	float inv;  // 1 / seg: (z + 0.5) * inv truncates to z / seg exactly for the row widths here
+};
+
+__host__ __device__ inline EdCut ed_cut(int W)
 {
-	return 4 * (size_t)W + 2 * (size_t)((W + 7) / 8);
+	EdCut c;
+	int n = W / ED_SEG_MIN;
+	n = n < 1 ? 1 : (n > ED_WALKERS ? ED_WALKERS : n);
+	c.seg = (W + n - 1) / n;
+	c.nseg = (W + c.seg - 1) / c.seg;
+	c.nsp = c.nseg | 1;
+	c.inv = 1.0f / (float)c.seg;
+	return c;
+}
+
+// Elements of a transposed row (>= W).
+__host__ __device__ inline int ed_cap(int W)
+{
+	const EdCut c = ed_cut(W);
+	return c.seg * c.nsp;
+}
+
+// Per-plane working rows in HBM, offsets in doubles, for rows wider than
+// ED_ROWS_LDS: R1 and the next row's samples (W each, by x and by scan
+// position), then transposed: the errors, (sample, carried error) pairs
+// (16-byte aligned) and two level rows (u16).  total is even, so every
+// plane's rows start 16-byte aligned.
+struct EdRowsOff {
+	size_t bnext, err, be, lvl0, lvl1, total;
+};
+
+__host__ __device__ inline EdRowsOff ed_rows_off(int W)
+{
+	const size_t cap = (size_t)ed_cap(W), l4 = (cap + 3) / 4;
+	EdRowsOff o;
+	o.bnext = (size_t)W;
+	o.err = 2 * (size_t)W;
+	o.be = o.err + cap + (cap & 1);
+	o.lvl0 = o.be + 2 * cap;
+	o.lvl1 = o.lvl0 + l4;
+	o.total = o.lvl1 + l4;
+	o.total += o.total & 1;
+	return o;
 }
 
 // Level k's decision data as one record: T = the threshold at which level
@@ -1543,31 +1589,41 @@ __device__ __forceinline__ double ed_step(const EdDecide &d, double2 be, double 
 	return ed_decide<TAB>(d, s, level);
 }
 
+#define ED_CAP_LDS (ED_ROWS_LDS + 32)  // >= ed_cap(W) for W <= ED_ROWS_LDS
+
 template <bool TAB, bool ROWS>
 struct EdLds {
 	EdRec rec[TAB ? ED_LDS_TABLE + 3 : 1];         // levels 0..P, two guards
-	double cd[TAB ? ED_LDS_TABLE + 1 : 1];         // output code per level
 	unsigned short Q[ED_NB + 2];
-	double2 be[ROWS ? ED_ROWS_LDS : 1];            // (sample, carried error) by scan position
-	double err[ROWS ? ED_ROWS_LDS : 1];            // the row's errors by x
-	unsigned short lvl[ROWS ? ED_ROWS_LDS : 1];    // the row's levels by scan position
-	double sw[ED_THREADS][2], se[ED_THREADS][2];   // segment states: at its first pixel, after its last
-	unsigned long long fail[ED_THREADS / 64];      // boundaries whose states differ
+	double2 be[ROWS ? ED_CAP_LDS : 1];             // (sample, carried error), transposed
+	double err[ROWS ? ED_CAP_LDS : 1];             // the row's errors, transposed
+	unsigned short lvl[2][ROWS ? ED_CAP_LDS : 1];  // levels of rows j (j % 2), transposed
+	double bnext[ROWS ? ED_ROWS_LDS : 1];          // the next row's samples by scan position
+	double R1[ROWS ? ED_ROWS_LDS : 1];             // row j-1's share for row j+1, by x
+	double sw[ED_WALKERS][2], se[ED_WALKERS][2];   // segment states: at its first pixel, after its last
+	unsigned long long fail[ED_WALKERS / 64];      // boundaries whose states differ
 };
 
 struct EdCtx {
 	EdDecide d;
-	const double *cd;          // output code per level
+	const double *code;        // output code per level (global)
 	g_cf64 *samp;              // the plane's clamped samples [H][W]
-	double *R1, *err;
+	double *R1, *err, *bnext;
 	double2 *be;
-	unsigned short *lvl;
+	unsigned short *lvl[2];
 	double m0, m1;
 	int W, o;
+	EdCut cut;
 	g_u8 *out;
 	size_t out_bpr;
 	int out_nch, out_depth;
 };
+
+__device__ __forceinline__ int ed_idx(const EdCut &k, int z)
+{
+	const int q = (int)(((float)z + 0.5f) * k.inv);
+	return (z - q * k.seg) * k.nsp + q;
+}
 
 // emit_store() through the context's global view of the output.
 __device__ __forceinline__ void ed_store(const EdCtx &c, int i, int j, double s_full)
@@ -1583,63 +1639,95 @@ __device__ __forceinline__ void ed_store(const EdCtx &c, int i, int j, double s_
 	}
 }
 
-// Stage A of row j, all threads: each pixel's sample and the error carried
-// to it from the rows above -- what row j-2 left in R1, then row j-1's
-// sources x-2f, x-f, x, x+f, x+2f in that row's scan order with weights
-// m[6..2] (the reference's scatter, src/imagew-main.c:497-567, as an ordered
-// gather); R1[x] then takes row j-1's share for row j+1 (weights m[11..7]).
+// Row j's codes (its levels are in lvl[j % 2]), threads [t0, t0 + nt).
+__device__ __forceinline__ void ed_store_row(const EdCtx &c, int j, int t, int nt)
+{
+	const int W = c.W;
+	const bool rev = (j & 1) != 0;
+	const unsigned short *lv = c.lvl[j & 1];
+	for (int z = t; z < W; z += nt) ed_store(c, rev ? W - 1 - z : z, j, c.code[lv[ed_idx(c.cut, z)]]);
+}
+
+// Row j's samples by scan position, threads [t0, t0 + nt).
+__device__ __forceinline__ void ed_load_row(const EdCtx &c, int j, int t, int nt)
+{
+	const int W = c.W;
+	const bool rev = (j & 1) != 0;
+	const g_cf64 *row = c.samp + (size_t)j * W;
+	for (int z = t; z < W; z += nt) c.bnext[z] = row[rev ? W - 1 - z : z];
+}
+
+// Stage A of row j, all threads: each pixel's sample (prefetched) and the
+// error carried to it from the rows above -- what row j-2 left in R1, then
+// row j-1's sources x-2f, x-f, x, x+f, x+2f in that row's scan order with
+// weights m[6..2] (the reference's scatter, src/imagew-main.c:497-567, as an
+// ordered gather); R1[x] then takes row j-1's share for row j+1 (weights
+// m[11..7]).  Row j-1 runs the other way, so pixel z of row j is its scan
+// position W-1-z, and its sources are that position's neighbours.
 __device__ __forceinline__ void ed_stage_a(const EdCtx &c, const double *m, int j)
 {
 	const int W = c.W;
 	const bool rev = (j & 1) != 0;
-	const int fwd = (j & 1) ? 1 : -1;  // row j-1's direction
 	for (int z = threadIdx.x; z < W; z += ED_THREADS) {
 		const int x = rev ? W - 1 - z : z;
-		const double b = c.samp[(size_t)j * W + x];
 		double e = 0.0;
 		if (j > 0) {
+			const int zp = W - 1 - z;
 			double a2 = 0.0;
 			e = c.R1[x];
 			for (int dd = -2; dd <= 2; dd++) {
-				const int p = x + dd * fwd;
+				const int p = zp + dd;
 				if (p < 0 || p >= W) continue;
-				const double er = c.err[p];
+				const double er = c.err[ed_idx(c.cut, p)];
 				e = e + er * m[4 - dd];
 				a2 = a2 + er * m[9 - dd];
 			}
 			c.R1[x] = a2;
 		}
-		c.be[z] = make_double2(b, e);
+		c.be[ed_idx(c.cut, z)] = make_double2(c.bnext[z], e);
 	}
 }
 
-// The speculative walk of segment t: pixels [zs, ze) of the row in scan
-// order, started `warm` pixels early with no in-row carry; the pixels of the
-// segment keep their errors and levels, and the states at zs and after the
-// last pixel go to LDS for the checks.
+// The speculative walk of segment t of row j: pixels [zs, ze) in scan order,
+// started `warm` pixels early with no in-row carry.  The pixels of the
+// segment keep their errors and levels; the states at zs and after the last
+// pixel go to LDS for the checks.  Position (r, q) of pixel z in the
+// transposed row advances without divisions.
 template <bool TAB, bool HAS_M1, class Lds>
-__device__ __forceinline__ void ed_walk_seg(const EdCtx &c, Lds &S, bool rev, int t, int seg, int warm)
+__device__ __forceinline__ void ed_walk_seg(const EdCtx &c, Lds &S, int j, int t, int warm)
 {
-	const int W = c.W;
+	const int W = c.W, seg = c.cut.seg, nsp = c.cut.nsp;
+	unsigned short *lv = c.lvl[j & 1];
 	const int zs = t * seg, ze = min(W, zs + seg), z0 = max(0, zs - warm);
 	double p1 = 0.0, p2 = 0.0;
 	int level;
-	double2 v = c.be[z0];
+	int q = (int)(((float)z0 + 0.5f) * c.cut.inv), r = z0 - q * seg;
+	double2 v = c.be[r * nsp + q];
 	int z = z0;
 	for (; z < zs; z++) {
-		const double2 nv = c.be[z + 1];  // z + 1 <= zs < W
+		// Next pixel (z + 1 <= zs < W).
+		int qn = q, rn = r + 1;
+		if (rn == seg) {
+			rn = 0;
+			qn++;
+		}
+		const double2 nv = c.be[rn * nsp + qn];
 		const double e = ed_step<TAB, HAS_M1>(c.d, v, p1, p2, c.m0, c.m1, level);
 		p2 = p1;
 		p1 = e;
 		v = nv;
+		q = qn;
+		r = rn;
 	}
 	S.sw[t][0] = p1;
 	S.sw[t][1] = p2;
-	for (; z < ze; z++) {
-		const double2 nv = c.be[min(z + 1, W - 1)];
+	// The segment's own pixels: q == t, r = 0 .. ze - zs - 1.
+	const int n = ze - zs;
+	for (int k = 0; k < n; k++) {
+		const double2 nv = c.be[min(k + 1, n - 1) * nsp + t];
 		const double e = ed_step<TAB, HAS_M1>(c.d, v, p1, p2, c.m0, c.m1, level);
-		c.err[rev ? W - 1 - z : z] = e;
-		c.lvl[z] = (unsigned short)level;
+		c.err[k * nsp + t] = e;
+		lv[k * nsp + t] = (unsigned short)level;
 		p2 = p1;
 		p1 = e;
 		v = nv;
@@ -1651,9 +1739,10 @@ __device__ __forceinline__ void ed_walk_seg(const EdCtx &c, Lds &S, bool rev, in
 // Wave 0: the boundary checks, then (lane 0) the true chain through every
 // segment whose start state differs from its predecessor's end state.
 template <bool TAB, bool HAS_M1, class Lds>
-__device__ __forceinline__ void ed_resolve(const EdCtx &c, Lds &S, bool rev, int nseg, int seg)
+__device__ __forceinline__ void ed_resolve(const EdCtx &c, Lds &S, int j)
 {
 	const int lane = threadIdx.x;
+	const int nseg = c.cut.nseg, seg = c.cut.seg, nsp = c.cut.nsp;
 	for (int k0 = 0; k0 < nseg; k0 += 64) {
 		const int k = k0 + lane;
 		const bool bad = k >= 1 && k < nseg &&
@@ -1670,6 +1759,7 @@ __device__ __forceinline__ void ed_resolve(const EdCtx &c, Lds &S, bool rev, int
 		}
 		return nseg;
 	};
+	unsigned short *lv = c.lvl[j & 1];
 	const int W = c.W;
 	int k = next_fail(1);
 	while (k < nseg) {
@@ -1679,29 +1769,35 @@ __device__ __forceinline__ void ed_resolve(const EdCtx &c, Lds &S, bool rev, int
 		// segment's walk); that segment is right from there to its end.
 		double p1 = S.se[k - 1][0], p2 = S.se[k - 1][1];
 		double pe = 0.0, po = 0.0;
-		int z = k * seg, own = k;
+		int q = k, r = 0;  // pixel z = q * seg + r
 		bool met = false;
-		for (; z < W; z++) {
-			own = z / seg;
-			const int x = rev ? W - 1 - z : z;
+		for (int z = k * seg; z < W; z++) {
+			const int ix = r * nsp + q;
 			int level;
-			const double old = c.err[x];
-			const double e = ed_step<TAB, HAS_M1>(c.d, c.be[z], p1, p2, c.m0, c.m1, level);
-			c.err[x] = e;
-			c.lvl[z] = (unsigned short)level;
-			met = e == old && (!HAS_M1 || (z > own * seg && pe == po));
+			const double old = c.err[ix];
+			const double e = ed_step<TAB, HAS_M1>(c.d, c.be[ix], p1, p2, c.m0, c.m1, level);
+			c.err[ix] = e;
+			lv[ix] = (unsigned short)level;
+			met = e == old && (!HAS_M1 || (r > 0 && pe == po));
 			pe = e;
 			po = old;
 			p2 = p1;
 			p1 = e;
 			if (met) break;
+			if (++r == seg) {
+				r = 0;
+				q++;
+			}
 		}
 		if (!met) break;
-		k = next_fail(own + 1);
+		k = next_fail(q + 1);
 	}
 }
 
-// One workgroup per plane: planes[b] = (plan, output channel).
+// One workgroup per plane: planes[b] = (plan, output channel).  Per row:
+// waves 0-3 walk the segments while waves 4-7 store the previous row's codes
+// and load the next row's samples; wave 0 checks the boundaries and walks
+// what differs; then every thread runs stage A for the next row.
 template <bool TAB, bool HAS_M1, bool ROWS>
 __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__restrict__ plans,
 	const int2 *__restrict__ planes)
@@ -1720,21 +1816,29 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	c.o = o;
 	c.m0 = m[0];
 	c.m1 = m[1];
+	c.cut = ed_cut(W);
 	c.samp = (g_cf64 *)(P.ed_samples + (size_t)q.ed_index * H * W);
+	c.code = q.code;
 	c.out = (g_u8 *)P.out;
 	c.out_bpr = P.out_bpr;
 	c.out_nch = P.out_nch;
 	c.out_depth = P.out_depth;
-	double *rows = P.ed_err + (size_t)q.ed_index * ed_row_doubles(W);
-	c.R1 = rows;
 	if (ROWS) {
+		c.R1 = S.R1;
+		c.bnext = S.bnext;
 		c.err = S.err;
 		c.be = S.be;
-		c.lvl = S.lvl;
+		c.lvl[0] = S.lvl[0];
+		c.lvl[1] = S.lvl[1];
 	} else {
-		c.err = rows + W;
-		c.be = (double2 *)(rows + 2 * (size_t)W);
-		c.lvl = (unsigned short *)(rows + 4 * (size_t)W);
+		const EdRowsOff ro = ed_rows_off(W);
+		double *rows = P.ed_err + (size_t)q.ed_index * ro.total;
+		c.R1 = rows;
+		c.bnext = rows + ro.bnext;
+		c.err = rows + ro.err;
+		c.be = (double2 *)(rows + ro.be);
+		c.lvl[0] = (unsigned short *)(rows + ro.lvl0);
+		c.lvl[1] = (unsigned short *)(rows + ro.lvl1);
 	}
 	c.d.P = Pq;
 	c.d.T = q.T;
@@ -1749,7 +1853,6 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 				r.U = ((g_cf64 *)q.U)[k];
 				r.M = ((g_cf64 *)q.M)[k];
 				r.lin = ((g_cf64 *)q.lin)[k];
-				S.cd[k] = ((g_cf64 *)q.code)[k];
 			} else {
 				r.T = 2.0;
 				r.U = 2.0;
@@ -1759,36 +1862,36 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 			S.rec[k] = r;
 		}
 		c.d.rec = S.rec;
-		c.cd = S.cd;
 	} else {
 		c.d.rec = nullptr;
-		c.cd = q.code;
 	}
 	for (int k = tid; k <= ED_NB; k += ED_THREADS)
 		S.Q[k] = (unsigned short)level_in(q.T, 0, Pq, k / (double)ED_NB);
 	c.d.Q = S.Q;
 	for (int x = tid; x < W; x += ED_THREADS) c.R1[x] = 0.0;
+	ed_load_row(c, 0, tid, ED_THREADS);
 	__syncthreads();
-
-	int nseg = min(ED_THREADS, max(1, W / ED_SEG_MIN));
-	const int seg = (W + nseg - 1) / nseg;
-	nseg = (W + seg - 1) / seg;
-	const int warm = q.ed_warm;
-
 	ed_stage_a(c, m, 0);
 	__syncthreads();
+
+	const int warm = q.ed_warm;
 	for (int j = 0; j < H; j++) {
-		const bool rev = (j & 1) != 0;
-		if (tid < nseg) ed_walk_seg<TAB, HAS_M1>(c, S, rev, tid, seg, warm);
+		if (tid < c.cut.nseg) {
+			ed_walk_seg<TAB, HAS_M1>(c, S, j, tid, warm);
+		} else if (tid >= ED_WALKERS) {
+			if (j > 0) ed_store_row(c, j - 1, tid - ED_WALKERS, ED_THREADS - ED_WALKERS);
+			if (j + 1 < H) ed_load_row(c, j + 1, tid - ED_WALKERS, ED_THREADS - ED_WALKERS);
+		}
 		__syncthreads();
-		if (tid < 64) ed_resolve<TAB, HAS_M1>(c, S, rev, nseg, seg);
+		if (tid < 64) ed_resolve<TAB, HAS_M1>(c, S, j);
 		__syncthreads();
-		// Row j's codes, and the next row's samples and carried errors
-		// (stage A reads row j's errors; the next walk overwrites them).
-		for (int z = tid; z < W; z += ED_THREADS) ed_store(c, rev ? W - 1 - z : z, j, c.cd[c.lvl[z]]);
-		if (j + 1 < H) ed_stage_a(c, m, j + 1);
-		__syncthreads();
+		if (j + 1 < H) {
+			ed_stage_a(c, m, j + 1);
+			__syncthreads();
+		}
 	}
+	ed_store_row(c, H - 1, tid, ED_THREADS);
+	__syncthreads();
 	if (P.ed_done) {
 		// Completion of the request: the last of its planes to finish copies
 		// the whole output to the host and raises the flag.
@@ -2191,6 +2294,11 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 					iw_set_error(ctx, "Too many output levels for error diffusion on the GPU path");
 					return false;
 				}
+				if (p.out_width >= (1 << 22)) {
+					// ed_cut's float division is exact below 2^22 pixels.
+					iw_set_error(ctx, "Rows too wide for error diffusion on the GPU path");
+					return false;
+				}
 				S.ed_of[o] = S.n_ed;
 				S.ed_list[S.n_ed++] = o;
 			}
@@ -2240,7 +2348,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	S.out_bytes = p.out_bpr * (size_t)p.out_height;
 	S.o_out = out_elsewhere ? 0 : L.add<uint8_t>(S.out_bytes);
 	S.o_eds = L.add<double>((size_t)S.n_ed * p.out_height * p.out_width);
-	S.o_ede = L.add<double>((size_t)S.n_ed * ed_row_doubles(p.out_width));
+	S.o_ede = L.add<double>((size_t)S.n_ed * ed_rows_off(p.out_width).total);
 	S.o_tables = L.add<uint8_t>(0);
 	S.o_lut.resize(S.luts.size());
 	for (size_t i = 0; i < S.luts.size(); i++) S.o_lut[i] = L.add<double>(S.luts[i].size());
@@ -2494,13 +2602,15 @@ const EdKernel kEdKernels[8] = {
 
 // Kernel variant of one error-diffusion plane: decision tables in LDS for up
 // to 257 levels, the x+2 carry only where its weight is nonzero (not
-// Floyd-Steinberg or Sierra 4-2-1), error rows in LDS up to ED_ROWS_LDS wide.
+// Floyd-Steinberg or Sierra 4-2-1), working rows in LDS up to ED_ROWS_LDS
+// wide.
 int ed_variant(const Staged &S, const iwamd_plan &p, int o)
 {
 	const int subtype = p.och[o].dithersubtype;
 	const int sub = (subtype >= 0 && subtype <= 7) ? subtype : 0;
 	const bool m1 = sub != 0 && sub != 6;
-	return (S.qt[o].P <= ED_LDS_TABLE ? 4 : 0) + (m1 ? 2 : 0) + (p.out_width <= ED_ROWS_LDS ? 1 : 0);
+	const bool rows = p.out_width <= ED_ROWS_LDS && ed_cap(p.out_width) <= ED_CAP_LDS;
+	return (S.qt[o].P <= ED_LDS_TABLE ? 4 : 0) + (m1 ? 2 : 0) + (rows ? 1 : 0);
 }
 
 struct WorkSet {
