@@ -1536,17 +1536,17 @@ __host__ __device__ inline double ed_decide(const EdDecide &d, double s, int &le
 	int k;
 	double U, M, lk, lk1;
 	if (TAB) {
-		if (k1 <= k0 + 1) {
-			// At most one threshold in the bin: T[k0].
-			const EdRec r0 = d.rec[k0], r1 = d.rec[k0 + 1];
-			const double l2 = d.rec[k0 + 2].lin;
-			const bool up = s >= r0.T;
-			k = k0 + (up ? 1 : 0);
-			U = up ? r1.U : r0.U;
-			M = up ? r1.M : r0.M;
-			lk = up ? r1.lin : r0.lin;
-			lk1 = up ? l2 : r1.lin;
-		} else {
+		// The common bin holds at most one threshold, T[k0]; a bin with more
+		// (levels narrower than a bin) bisects its thresholds afterwards.
+		const EdRec r0 = d.rec[k0], r1 = d.rec[k0 + 1];
+		const double l2 = d.rec[k0 + 2].lin;
+		const bool up = s >= r0.T;
+		k = k0 + (up ? 1 : 0);
+		U = up ? r1.U : r0.U;
+		M = up ? r1.M : r0.M;
+		lk = up ? r1.lin : r0.lin;
+		lk1 = up ? l2 : r1.lin;
+		if (k1 > k0 + 1) {
 			int a = k0, n = k1 - k0;
 			while (n > 0) {
 				const int half = n >> 1;
@@ -1606,11 +1606,11 @@ struct EdLds {
 
 struct EdCtx {
 	EdDecide d;
-	const double *code;        // output code per level (global)
+	g_cf64 *code;              // output code per level
 	g_cf64 *samp;              // the plane's clamped samples [H][W]
 	double *R1, *err, *bnext;
 	double2 *be;
-	unsigned short *lvl[2];
+	unsigned short *lvl0, *lvl1;  // level rows (HBM working rows only)
 	double m0, m1;
 	int W, o;
 	EdCut cut;
@@ -1639,12 +1639,11 @@ __device__ __forceinline__ void ed_store(const EdCtx &c, int i, int j, double s_
 	}
 }
 
-// Row j's codes (its levels are in lvl[j % 2]), threads [t0, t0 + nt).
-__device__ __forceinline__ void ed_store_row(const EdCtx &c, int j, int t, int nt)
+// Row j's codes from its levels lv, threads t, t + nt, ...
+__device__ __forceinline__ void ed_store_row(const EdCtx &c, const unsigned short *lv, int j, int t, int nt)
 {
 	const int W = c.W;
 	const bool rev = (j & 1) != 0;
-	const unsigned short *lv = c.lvl[j & 1];
 	for (int z = t; z < W; z += nt) ed_store(c, rev ? W - 1 - z : z, j, c.code[lv[ed_idx(c.cut, z)]]);
 }
 
@@ -1694,10 +1693,9 @@ __device__ __forceinline__ void ed_stage_a(const EdCtx &c, const double *m, int 
 // pixel go to LDS for the checks.  Position (r, q) of pixel z in the
 // transposed row advances without divisions.
 template <bool TAB, bool HAS_M1, class Lds>
-__device__ __forceinline__ void ed_walk_seg(const EdCtx &c, Lds &S, int j, int t, int warm)
+__device__ __forceinline__ void ed_walk_seg(const EdCtx &c, Lds &S, unsigned short *lv, int t, int warm)
 {
 	const int W = c.W, seg = c.cut.seg, nsp = c.cut.nsp;
-	unsigned short *lv = c.lvl[j & 1];
 	const int zs = t * seg, ze = min(W, zs + seg), z0 = max(0, zs - warm);
 	double p1 = 0.0, p2 = 0.0;
 	int level;
@@ -1739,7 +1737,7 @@ __device__ __forceinline__ void ed_walk_seg(const EdCtx &c, Lds &S, int j, int t
 // Wave 0: the boundary checks, then (lane 0) the true chain through every
 // segment whose start state differs from its predecessor's end state.
 template <bool TAB, bool HAS_M1, class Lds>
-__device__ __forceinline__ void ed_resolve(const EdCtx &c, Lds &S, int j)
+__device__ __forceinline__ void ed_resolve(const EdCtx &c, Lds &S, unsigned short *lv)
 {
 	const int lane = threadIdx.x;
 	const int nseg = c.cut.nseg, seg = c.cut.seg, nsp = c.cut.nsp;
@@ -1759,7 +1757,6 @@ __device__ __forceinline__ void ed_resolve(const EdCtx &c, Lds &S, int j)
 		}
 		return nseg;
 	};
-	unsigned short *lv = c.lvl[j & 1];
 	const int W = c.W;
 	int k = next_fail(1);
 	while (k < nseg) {
@@ -1818,7 +1815,7 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	c.m1 = m[1];
 	c.cut = ed_cut(W);
 	c.samp = (g_cf64 *)(P.ed_samples + (size_t)q.ed_index * H * W);
-	c.code = q.code;
+	c.code = (g_cf64 *)q.code;
 	c.out = (g_u8 *)P.out;
 	c.out_bpr = P.out_bpr;
 	c.out_nch = P.out_nch;
@@ -1828,8 +1825,7 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 		c.bnext = S.bnext;
 		c.err = S.err;
 		c.be = S.be;
-		c.lvl[0] = S.lvl[0];
-		c.lvl[1] = S.lvl[1];
+		c.lvl0 = c.lvl1 = nullptr;
 	} else {
 		const EdRowsOff ro = ed_rows_off(W);
 		double *rows = P.ed_err + (size_t)q.ed_index * ro.total;
@@ -1837,8 +1833,8 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 		c.bnext = rows + ro.bnext;
 		c.err = rows + ro.err;
 		c.be = (double2 *)(rows + ro.be);
-		c.lvl[0] = (unsigned short *)(rows + ro.lvl0);
-		c.lvl[1] = (unsigned short *)(rows + ro.lvl1);
+		c.lvl0 = (unsigned short *)(rows + ro.lvl0);
+		c.lvl1 = (unsigned short *)(rows + ro.lvl1);
 	}
 	c.d.P = Pq;
 	c.d.T = q.T;
@@ -1874,23 +1870,29 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	ed_stage_a(c, m, 0);
 	__syncthreads();
 
+	// The level row of row j (two alternate), indexed straight from the LDS
+	// array so that its accesses stay LDS instructions.
+	auto lvl_of = [&](int jj) -> unsigned short * {
+		if constexpr (ROWS) return &S.lvl[jj & 1][0];
+		else return (jj & 1) ? c.lvl1 : c.lvl0;
+	};
 	const int warm = q.ed_warm;
 	for (int j = 0; j < H; j++) {
 		if (tid < c.cut.nseg) {
-			ed_walk_seg<TAB, HAS_M1>(c, S, j, tid, warm);
+			ed_walk_seg<TAB, HAS_M1>(c, S, lvl_of(j), tid, warm);
 		} else if (tid >= ED_WALKERS) {
-			if (j > 0) ed_store_row(c, j - 1, tid - ED_WALKERS, ED_THREADS - ED_WALKERS);
+			if (j > 0) ed_store_row(c, lvl_of(j - 1), j - 1, tid - ED_WALKERS, ED_THREADS - ED_WALKERS);
 			if (j + 1 < H) ed_load_row(c, j + 1, tid - ED_WALKERS, ED_THREADS - ED_WALKERS);
 		}
 		__syncthreads();
-		if (tid < 64) ed_resolve<TAB, HAS_M1>(c, S, j);
+		if (tid < 64) ed_resolve<TAB, HAS_M1>(c, S, lvl_of(j));
 		__syncthreads();
 		if (j + 1 < H) {
 			ed_stage_a(c, m, j + 1);
 			__syncthreads();
 		}
 	}
-	ed_store_row(c, H - 1, tid, ED_THREADS);
+	ed_store_row(c, lvl_of(H - 1), H - 1, tid, ED_THREADS);
 	__syncthreads();
 	if (P.ed_done) {
 		// Completion of the request: the last of its planes to finish copies
