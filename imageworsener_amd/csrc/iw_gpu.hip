@@ -115,6 +115,7 @@ struct DevPlan {
 	double *ed_samples;
 	double *ed_err;
 	int n_ed;
+	int ed_seg;             // fewest pixels per error-diffusion segment
 	// Per-request completion (request loop only; null otherwise): the last
 	// error-diffusion plane of a request to finish copies the output image to
 	// host memory and raises the flag there.
@@ -1440,7 +1441,7 @@ __constant__ double c_ed_matrix[8][12] = {
 #define ED_LDS_TABLE 256  // largest level count - 1 whose tables live in LDS
 #define ED_ROWS_LDS 3072  // widest row whose working rows live in LDS
 #define ED_NB 4096        // bins of the level index (s * 4096 is exact)
-#define ED_SEG_MIN 8      // fewest pixels per segment
+#define ED_SEG_MIN 8      // fewest pixels per segment (default)
 
 // How a row of W pixels is cut: nseg segments of seg pixels (the last one
 // shorter).  Rows are stored transposed -- pixel z (scan order) at
@@ -1452,10 +1453,10 @@ struct EdCut {
 	float inv;  // 1 / seg: (z + 0.5) * inv truncates to z / seg exactly for the row widths here
 };
 
-__host__ __device__ inline EdCut ed_cut(int W)
+__host__ __device__ inline EdCut ed_cut(int W, int seg_min)
 {
 	EdCut c;
-	int n = W / ED_SEG_MIN;
+	int n = W / seg_min;
 	n = n < 1 ? 1 : (n > ED_WALKERS ? ED_WALKERS : n);
 	c.seg = (W + n - 1) / n;
 	c.nseg = (W + c.seg - 1) / c.seg;
@@ -1465,9 +1466,9 @@ __host__ __device__ inline EdCut ed_cut(int W)
 }
 
 // Elements of a transposed row (>= W).
-__host__ __device__ inline int ed_cap(int W)
+__host__ __device__ inline int ed_cap(int W, int seg_min)
 {
-	const EdCut c = ed_cut(W);
+	const EdCut c = ed_cut(W, seg_min);
 	return c.seg * c.nsp;
 }
 
@@ -1480,9 +1481,9 @@ struct EdRowsOff {
 	size_t bnext, err, be, lvl0, lvl1, total;
 };
 
-__host__ __device__ inline EdRowsOff ed_rows_off(int W)
+__host__ __device__ inline EdRowsOff ed_rows_off(int W, int seg_min)
 {
-	const size_t cap = (size_t)ed_cap(W), l4 = (cap + 3) / 4;
+	const size_t cap = (size_t)ed_cap(W, seg_min), l4 = (cap + 3) / 4;
 	EdRowsOff o;
 	o.bnext = (size_t)W;
 	o.err = 2 * (size_t)W;
@@ -1813,7 +1814,7 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 	c.o = o;
 	c.m0 = m[0];
 	c.m1 = m[1];
-	c.cut = ed_cut(W);
+	c.cut = ed_cut(W, P.ed_seg);
 	c.samp = (g_cf64 *)(P.ed_samples + (size_t)q.ed_index * H * W);
 	c.code = (g_cf64 *)q.code;
 	c.out = (g_u8 *)P.out;
@@ -1827,7 +1828,7 @@ __global__ void __launch_bounds__(ED_THREADS) errdiff_kernel(const DevPlan *__re
 		c.be = S.be;
 		c.lvl0 = c.lvl1 = nullptr;
 	} else {
-		const EdRowsOff ro = ed_rows_off(W);
+		const EdRowsOff ro = ed_rows_off(W, P.ed_seg);
 		double *rows = P.ed_err + (size_t)q.ed_index * ro.total;
 		c.R1 = rows;
 		c.bnext = rows + ro.bnext;
@@ -2068,6 +2069,8 @@ struct Staged {
 	std::vector<iwx::PrngTables> pt;
 	int ed_of[4] = {-1, -1, -1, -1};
 	int n_ed = 0;
+	int ed_warm = 0;           // error diffusion: warm-up pixels (0: by the channel's weights and levels)
+	int ed_seg = ED_SEG_MIN;   // error diffusion: fewest pixels per segment
 	int ed_list[4] = {-1, -1, -1, -1};
 	bool in_int = true;
 	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
@@ -2331,6 +2334,10 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 		return v ? atoi(v) : 0;
 	};
 	const int want_ti = opt_int("amd:tile"), want_rows = opt_int("amd:band_rows");
+	// Options amd:ed_warm and amd:ed_seg set the error-diffusion walks' warm-up
+	// and the fewest pixels per segment; they change the time, not the bytes.
+	S.ed_warm = std::max(0, opt_int("amd:ed_warm"));
+	if (opt_int("amd:ed_seg") >= 1) S.ed_seg = opt_int("amd:ed_seg");
 	if (p.precision == IWAMD_PRECISION_FAST && !S.skip_pixels)
 		plan_fast(p, S, (want_ti == 16 || want_ti == 32) ? want_ti : 0);
 	if (!S.fast.empty()) S.shape_forced = S.fast_ti == want_ti;
@@ -2350,7 +2357,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	S.out_bytes = p.out_bpr * (size_t)p.out_height;
 	S.o_out = out_elsewhere ? 0 : L.add<uint8_t>(S.out_bytes);
 	S.o_eds = L.add<double>((size_t)S.n_ed * p.out_height * p.out_width);
-	S.o_ede = L.add<double>((size_t)S.n_ed * ed_rows_off(p.out_width).total);
+	S.o_ede = L.add<double>((size_t)S.n_ed * ed_rows_off(p.out_width, S.ed_seg).total);
 	S.o_tables = L.add<uint8_t>(0);
 	S.o_lut.resize(S.luts.size());
 	for (size_t i = 0; i < S.luts.size(); i++) S.o_lut[i] = L.add<double>(S.luts[i].size());
@@ -2559,12 +2566,13 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 			d.prng_row = (const uint64_t *)(base + q.prow);
 			d.prng_col = (const uint64_t *)(base + q.pcol);
 			d.M = (const double *)(base + q.M);
-			d.ed_warm = ed_warm(s, S.qt[o].P);
+			d.ed_warm = S.ed_warm > 0 ? S.ed_warm : ed_warm(s, S.qt[o].P);
 		}
 	}
 	D.ed_samples = (double *)(base + S.o_eds);
 	D.ed_err = (double *)(base + S.o_ede);
 	D.n_ed = S.n_ed;
+	D.ed_seg = S.ed_seg;
 	D.apply_bkgd = p.apply_bkgd;
 	D.late = p.bkgd_strategy_late;
 	D.checker = p.bkgd_checkerboard;
@@ -2611,7 +2619,7 @@ int ed_variant(const Staged &S, const iwamd_plan &p, int o)
 	const int subtype = p.och[o].dithersubtype;
 	const int sub = (subtype >= 0 && subtype <= 7) ? subtype : 0;
 	const bool m1 = sub != 0 && sub != 6;
-	const bool rows = p.out_width <= ED_ROWS_LDS && ed_cap(p.out_width) <= ED_CAP_LDS;
+	const bool rows = p.out_width <= ED_ROWS_LDS && ed_cap(p.out_width, S.ed_seg) <= ED_CAP_LDS;
 	return (S.qt[o].P <= ED_LDS_TABLE ? 4 : 0) + (m1 ? 2 : 0) + (rows ? 1 : 0);
 }
 
