@@ -2073,6 +2073,7 @@ struct Staged {
 	int ed_seg = ED_SEG_MIN;   // error diffusion: fewest pixels per segment
 	int ed_list[4] = {-1, -1, -1, -1};
 	bool in_int = true;
+	bool in_elsewhere = false; // the input image is already in device memory (no region, no copy)
 	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
 	int band_rows = 0;         // rows per band of the fused kernel; 0: two-pass kernels
 	int fast_ti = 0;           // output columns per chosen-precision tile
@@ -2214,7 +2215,8 @@ void plan_fast(const iwamd_plan &p, Staged &S, int want_ti = 0)
 
 // Build the host tables of a plan, choose its path and lay out its device
 // data (offsets from the job's base).  path: "band", "twopass" or null.
-bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path, bool out_elsewhere = false)
+bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path, bool out_elsewhere = false,
+	bool in_elsewhere = false)
 {
 	using namespace iwx;
 	// Floating-point input: 32-bit samples are read as stored; the reference
@@ -2352,7 +2354,8 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 
 	Layout L;
 	S.in_bytes = p.in_bpr * (size_t)phys_rows;
-	S.o_in = L.add<uint8_t>(S.in_bytes);
+	S.in_elsewhere = in_elsewhere;
+	S.o_in = L.add<uint8_t>(in_elsewhere ? 0 : S.in_bytes);
 	S.o_inter = L.add<float>(S.band_rows ? 0 : (size_t)p.inter_nch * p.inter_height * p.inter_width);
 	S.out_bytes = p.out_bpr * (size_t)p.out_height;
 	S.o_out = out_elsewhere ? 0 : L.add<uint8_t>(S.out_bytes);
@@ -2406,7 +2409,8 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 bool upload_job(Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
 {
 	bool ok = true;
-	if (S.in_bytes) ok &= hipMemcpyAsync(base + S.o_in, p.in_pixels, S.in_bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+	if (S.in_bytes && !S.in_elsewhere)
+		ok &= hipMemcpyAsync(base + S.o_in, p.in_pixels, S.in_bytes, hipMemcpyHostToDevice, st) == hipSuccess;
 	const size_t t0 = S.o_tables;
 	S.pack.assign(S.bytes - t0, 0);
 	auto up = [&](size_t off, const void *src, size_t bytes) {
@@ -3078,25 +3082,34 @@ extern "C" struct iwamd_job *iwamd_job_create(struct iw_context *ctx)
 	return J;
 }
 
-extern "C" int iwamd_job_run(struct iwamd_job *J)
+namespace {
+
+// One job's kernels on st (one workgroup per band, not persistent); with
+// events, ev[0..3] bracket the resampling, the horizontal pass (empty when
+// fused) and error diffusion.
+void enqueue_job(const WorkSet &W, const DevPlan &D, hipStream_t st, hipEvent_t *ev)
 {
-	if (J->skip_pixels) return 1;
-	const WorkSet &W = *(const WorkSet *)J->ws;
-	hipStream_t st = J->stream;
-	hipEventRecord(J->ev[0], st);
+	if (ev) hipEventRecord(ev[0], st);
 	if (W.twopass.empty()) {
 		// One fused kernel; the horizontal-pass slot of the timings stays 0.
 		ws_enqueue_resample(W, st, false);
-		hipEventRecord(J->ev[1], st);
+		if (ev) hipEventRecord(ev[1], st);
 	} else {
-		const DevPlan &D = J->D;
 		hipLaunchKernelGGL(vpass_kernel, dim3((D.W + 255) / 256, D.Hp), dim3(256), 0, st, D);
-		hipEventRecord(J->ev[1], st);
+		if (ev) hipEventRecord(ev[1], st);
 		hipLaunchKernelGGL(hpass_kernel, dim3((D.out_w + 255) / 256, D.out_h), dim3(256), 0, st, D);
 	}
-	hipEventRecord(J->ev[2], st);
+	if (ev) hipEventRecord(ev[2], st);
 	ws_enqueue_errdiff(W, st);
-	hipEventRecord(J->ev[3], st);
+	if (ev) hipEventRecord(ev[3], st);
+}
+
+}  // namespace
+
+extern "C" int iwamd_job_run(struct iwamd_job *J)
+{
+	if (J->skip_pixels) return 1;
+	enqueue_job(*(const WorkSet *)J->ws, J->D, J->stream, J->ev);
 	if (hipGetLastError() != hipSuccess) {
 		iw_set_error(J->ctx, "Kernel launch failed");
 		return 0;
@@ -3165,6 +3178,200 @@ extern "C" void iwamd_job_destroy(struct iwamd_job *J)
 		if (e) hipEventDestroy(e);
 	if (J->stream) hipStreamDestroy(J->stream);
 	delete J;
+}
+
+// ---- a job that allocates nothing --------------------------------------------
+// For callers that cannot allocate device memory (code under stream capture,
+// tight serving loops, harnesses that forbid it).  The input image is already
+// in device memory, the output goes to device memory the caller owns, and
+// everything else the kernels need -- tables, the FP32 intermediate, the
+// error-diffusion samples and working rows, the descriptor table and work
+// lists -- lives in `scratch`, one fixed array per GPU held until the process
+// ends.  Host code reaches the array only through the address that
+// hipGetSymbolAddress returns on each device (the symbol's name is not a
+// device address in host code).
+
+#define IWAMD_NOALLOC_SCRATCH ((size_t)64 << 20)
+
+static __device__ __align__(256) unsigned char scratch[IWAMD_NOALLOC_SCRATCH];
+
+__global__ void __launch_bounds__(256) zero_bytes_kernel(uint8_t *p, size_t n)
+{
+	for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+		p[k] = 0;
+}
+
+// Negated samples of the non-alpha channels (IW_VAL_NEGATE_TARGET): 8- and
+// 16-bit codes become maxcolorcode - code, floats 1 - s
+// (src/imagew-main.c:1324-1366).
+struct NegateArgs {
+	unsigned int maxcc[4];
+	int neg[4];
+};
+
+__global__ void __launch_bounds__(256) negate_kernel(uint8_t *img, size_t bpr, int w, int h, int nc, int depth,
+	int is_float, NegateArgs a)
+{
+	const size_t n = (size_t)w * h * nc;
+	for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+		const int ch = (int)(k % nc);
+		if (!a.neg[ch]) continue;
+		const size_t px = k / nc;
+		uint8_t *row = img + (px / w) * bpr;
+		const size_t s = (px % w) * nc + ch;
+		if (is_float) {
+			float f;
+			memcpy(&f, row + s * 4, 4);
+			f = 1.0f - f;
+			memcpy(row + s * 4, &f, 4);
+		} else if (depth == 8) {
+			row[s] = (uint8_t)(a.maxcc[ch] - row[s]);
+		} else if (depth == 16) {
+			const unsigned int v = a.maxcc[ch] - (row[s * 2] * 256u + row[s * 2 + 1]);
+			row[s * 2] = (uint8_t)((v & 0xff00) >> 8);
+			row[s * 2 + 1] = (uint8_t)(v & 0xff);
+		}
+	}
+}
+
+// The colour of fully transparent pixels cleared, as the reference's output
+// optimiser does for every integer image with alpha before any optional
+// optimisation (src/imagew-opt.c:1112-1163, :1200): alpha is the last of nc
+// channels; 16-bit samples are two bytes.
+__global__ void __launch_bounds__(256) clear_transparent_kernel(uint8_t *img, size_t bpr, int w, int h, int nc,
+	int bytes_per_sample)
+{
+	const size_t n = (size_t)w * h;
+	for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+		uint8_t *px = img + (k / w) * bpr + (k % w) * nc * bytes_per_sample;
+		const uint8_t *a = px + (nc - 1) * bytes_per_sample;
+		if (a[0] == 0 && (bytes_per_sample == 1 || a[1] == 0))
+			for (int b = 0; b < (nc - 1) * bytes_per_sample; b++) px[b] = 0;
+	}
+}
+
+namespace {
+
+// Per device: the array's address there, and the lock that makes calls on
+// one device take turns (they share its array).
+struct ScratchSlot {
+	std::mutex mu;
+	char *ptr = nullptr;
+};
+
+ScratchSlot &scratch_slot(int dev)
+{
+	static std::mutex table_mu;
+	static std::map<int, ScratchSlot *> table;
+	std::lock_guard<std::mutex> lock(table_mu);
+	ScratchSlot *&s = table[dev];
+	if (!s) s = new ScratchSlot();
+	return *s;
+}
+
+}  // namespace
+
+extern "C" size_t iwamd_noalloc_scratch_bytes(void)
+{
+	return IWAMD_NOALLOC_SCRATCH;
+}
+
+extern "C" int iwamd_process_device_noalloc(struct iw_context *ctx, const void *d_in, size_t in_bytes, void *d_out,
+	size_t out_bytes, void *stream, struct iw_image *out_desc)
+{
+	iwamd_plan p;
+	if (!iwamd_prepare(ctx, &p) || !runtime_init(ctx)) return 0;
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess) {
+		iw_set_error(ctx, "hipGetDevice failed");
+		return 0;
+	}
+	const int phys_rows = (p.orient_transform & 4) ? p.in_width : p.in_height;
+	const size_t need_in = p.in_bpr * (size_t)phys_rows, need_out = p.out_bpr * (size_t)p.out_height;
+	if (!d_in || in_bytes < need_in) {
+		iw_set_errorf(ctx, "Device input: %zu bytes needed, %zu given", need_in, d_in ? in_bytes : (size_t)0);
+		return 0;
+	}
+	if (!d_out || out_bytes < need_out) {
+		iw_set_errorf(ctx, "Device output: %zu bytes needed, %zu given", need_out, d_out ? out_bytes : (size_t)0);
+		return 0;
+	}
+	hipStream_t st = (hipStream_t)stream;
+	Staged S;
+	if (!stage_job(ctx, p, S, iw_get_option(ctx, "amd:resample"), true, true)) return 0;
+	tune_apply(p, S, st, false);  // look the shape up; never tunes (tuning allocates)
+	WorkSet W;
+	ws_add(W, S, p);
+	ws_layout(W);
+	const size_t need = S.bytes + W.bytes;
+	if (need > IWAMD_NOALLOC_SCRATCH) {
+		iw_set_errorf(ctx, "This job needs %zu bytes of device scratch; iwamd_process_device_noalloc has %zu",
+			need, (size_t)IWAMD_NOALLOC_SCRATCH);
+		return 0;
+	}
+	ScratchSlot &slot = scratch_slot(dev);
+	std::lock_guard<std::mutex> lock(slot.mu);
+	if (!slot.ptr) {
+		// Once per device, with that device current.
+		void *ptr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !ptr) {
+			iw_set_errorf(ctx, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+			return 0;
+		}
+		slot.ptr = (char *)ptr;
+	}
+	char *base = slot.ptr;
+	DevPlan D;
+	fill_dev_plan(S, p, base, D);
+	D.in = (const uint8_t *)d_in;
+	D.out = (uint8_t *)d_out;
+	bool ok = true;
+	if (S.skip_pixels) {
+		// An unknown filter: the reference reports it and outputs zeros.
+		const int grid = (int)std::min<size_t>(4096, (need_out + 255) / 256);
+		hipLaunchKernelGGL(zero_bytes_kernel, dim3(std::max(grid, 1)), dim3(256), 0, st, (uint8_t *)d_out, need_out);
+	} else {
+		W.plans[0] = D;
+		ok = upload_job(S, p, base, st) && ws_upload(W, base + S.bytes, st);
+		if (ok) enqueue_job(W, D, st, nullptr);
+		if (ok && ctx->req.negate_target) {
+			NegateArgs a;
+			memset(&a, 0, sizeof(a));
+			for (int ch = 0; ch < p.out_nch && ch < 4; ch++) {
+				a.neg[ch] = ctx->img2_ci[ch].channeltype != IW_CHANNELTYPE_ALPHA;
+				a.maxcc[ch] = (unsigned int)ctx->img2_ci[ch].maxcolorcode_int;
+			}
+			const size_t ns = (size_t)p.out_width * p.out_height * p.out_nch;
+			const int grid = (int)std::max<size_t>(1, std::min<size_t>(4096, (ns + 255) / 256));
+			hipLaunchKernelGGL(negate_kernel, dim3(grid), dim3(256), 0, st, (uint8_t *)d_out, p.out_bpr, p.out_width,
+				p.out_height, p.out_nch, p.out_bit_depth, p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT ? 1 : 0, a);
+		}
+		if (ok && p.out_sampletype == IW_SAMPLETYPE_UINT && !ctx->reduced_output_maxcolor_flag &&
+			(p.out_imgtype & 0x100) && (p.out_bit_depth == 8 || p.out_bit_depth == 16)) {
+			const size_t npx = (size_t)p.out_width * p.out_height;
+			const int grid = (int)std::max<size_t>(1, std::min<size_t>(4096, (npx + 255) / 256));
+			hipLaunchKernelGGL(clear_transparent_kernel, dim3(grid), dim3(256), 0, st, (uint8_t *)d_out, p.out_bpr,
+				p.out_width, p.out_height, p.out_nch, p.out_bit_depth / 8);
+		}
+	}
+	ok = ok && hipGetLastError() == hipSuccess;
+	// The host tables live in S and W: wait for the copies (and the kernels).
+	const hipError_t e = hipStreamSynchronize(st);
+	if (!ok || e != hipSuccess) {
+		iw_set_errorf(ctx, "GPU pipeline failed: %s", hipGetErrorString(e != hipSuccess ? e : hipErrorUnknown));
+		return 0;
+	}
+	if (out_desc) {
+		memset(out_desc, 0, sizeof(*out_desc));
+		out_desc->imgtype = p.out_imgtype;
+		out_desc->bit_depth = p.out_bit_depth;
+		out_desc->sampletype = p.out_sampletype;
+		out_desc->width = p.out_width;
+		out_desc->height = p.out_height;
+		out_desc->bpr = p.out_bpr;
+	}
+	return 1;
 }
 
 // ---- HBM counter calibration ---------------------------------------------

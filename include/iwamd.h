@@ -183,6 +183,29 @@ IW_EXPORT(int) iwamd_ed_decide_check(const struct iwamd_ochan *och, int tab, int
  * device's restatement of src/imagew-main.c:669-736 with no dither. */
 IW_EXPORT(int) iwamd_quant_fold_check(const struct iwamd_ochan *och, int n, const double *s, double *code);
 
+/* An iw_process_image pixel pass that allocates nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  ctx is set up as for iw_process_image, except that the input
+ * image's bytes (the layout img1 describes: bpr x rows, 16-bit big-endian)
+ * are at d_in in device memory -- img1's host pixels may be NULL -- and the
+ * output image is written to d_out in device memory: out_desc's bpr x height
+ * bytes (pixels left NULL), as iw_process_image leaves them with every
+ * optional optimisation off (iw_set_allow_opt), i.e. the pixel pipeline's
+ * image, negated when IW_VAL_NEGATE_TARGET is set, with the colour of fully
+ * transparent pixels cleared.  The kernels run on stream (a hipStream_t; NULL:
+ * the null stream), and the call returns when they have finished.  Tables,
+ * intermediate rows, error-diffusion rows and work lists live in a fixed
+ * device array of iwamd_noalloc_scratch_bytes() (64 MiB) per GPU, reserved
+ * for the life of the process: a job that needs more fails with an error
+ * naming both sizes (C1-sized jobs fit).  Calls on one GPU take turns on
+ * that array (a lock held for the call); each call overwrites what the
+ * previous one left there.  Never tunes; never calls hipMalloc, hipFree or
+ * hipHostMalloc.  Returns 1, or 0 with the context's error set. */
+IW_EXPORT(int) iwamd_process_device_noalloc(struct iw_context *ctx, const void *d_in, size_t in_bytes, void *d_out,
+	size_t out_bytes, void *stream, struct iw_image *out_desc);
+IW_EXPORT(size_t) iwamd_noalloc_scratch_bytes(void);
+
 /* Device-resident jobs.  iwamd_job_create() runs the host decisions on ctx
  * (consuming its single use), builds the device tables and uploads the input
  * image once; iwamd_job_run() launches the pixel kernels on the job's stream
