@@ -459,6 +459,7 @@ __device__ __forceinline__ void emit_ctx(const DevPlan &P, EmitCtx &E)
 // diffusion.
 struct EmitSimple {
 	int nch, intclamp, out_depth, out_nch, out_w, out_h;
+	int pack4;  // 8-bit, four output channels, each written once, none diffused: one 32-bit store per pixel
 	size_t out_bpr;
 	uint8_t *out;
 	double *ed_samples;
@@ -495,6 +496,18 @@ __device__ __forceinline__ bool emit_simple_ctx(const DevPlan &P, EmitSimple &E)
 			(ed ? 1 << 7 : 0) | ((nc || q.code_is_level) ? 1 << 8 : 0) | (q.ix_log2 << 9);
 		if (o >= 0 && !ed && !(q.family == IW_DITHERFAMILY_NONE && q.ix && E.sa[idx] && E.n[idx] > 0)) ok = false;
 	}
+	// Rows of four-byte pixels start four-byte aligned (device blocks and
+	// output arenas are 256-byte aligned).
+	int seen = 0;
+	bool pack = P.out_depth == 8 && P.out_nch == 4 && P.inter_nch == 4 && P.out_bpr % 4 == 0 &&
+		((uintptr_t)P.out & 3) == 0;
+#pragma unroll
+	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
+		const int o = (E.f[idx] >> 2 & 7) - 1;
+		if (o < 0 || (E.f[idx] & (1 << 7)) || (seen & (1 << o))) pack = false;
+		else seen |= 1 << o;
+	}
+	E.pack4 = pack && seen == 15;
 	return ok;
 }
 
@@ -632,9 +645,12 @@ __device__ __forceinline__ void emit_px_simple(const EmitSimple &E, const int (&
 		}
 	}
 	g_u8 *px[N];
+	unsigned int word[N];
 #pragma unroll
-	for (int n = 0; n < N; n++)
+	for (int n = 0; n < N; n++) {
 		px[n] = (g_u8 *)E.out + (size_t)pj[n] * E.out_bpr + (size_t)E.out_nch * pi[n] * (E.out_depth == 8 ? 1 : 2);
+		word[n] = 0;
+	}
 #pragma unroll
 	for (int idx = 0; idx < IWAMD_MAX_CH; idx++) {
 		if (idx >= E.nch) break;
@@ -653,7 +669,9 @@ __device__ __forceinline__ void emit_px_simple(const EmitSimple &E, const int (&
 			int k = l + (m > 0 && a0[idx][n] <= x) + (m > 1 && a1[idx][n] <= x);
 			if (m > 2 || !(x >= 0.0)) k = count_le(E.sa[idx], E.n[idx], E.six[idx], f >> 9 & 31, x);
 			const double code = (f & (1 << 8)) ? (double)k : ((const g_cf64 *)E.scode[idx])[k];
-			if (E.out_depth == 8) {
+			if (E.pack4) {
+				word[n] |= (unsigned int)(uint8_t)(0.5 + code) << (8 * o);
+			} else if (E.out_depth == 8) {
 				px[n][o] = (uint8_t)(0.5 + code);
 			} else {
 				const unsigned int u = (unsigned short)(0.5 + code);
@@ -661,6 +679,11 @@ __device__ __forceinline__ void emit_px_simple(const EmitSimple &E, const int (&
 				px[n][2 * o + 1] = (uint8_t)(u & 0xff);
 			}
 		}
+	}
+	if (E.pack4) {
+#pragma unroll
+		for (int n = 0; n < N; n++)
+			if (live[n]) *(__attribute__((address_space(1))) unsigned int *)px[n] = word[n];
 	}
 }
 
@@ -1100,6 +1123,23 @@ struct __attribute__((aligned(16))) FastLds {
 	float lut[IWAMD_MAX_CH + 1][256];  // 8-bit input tables, FP32: channels, then alpha
 };
 
+// The 32-bit word of a 4-byte pixel (fast_fetch_lds<32>'s load): logical
+// row p of the column at byte offset cb, clamped to the image.
+__device__ __forceinline__ unsigned int fast_word(const DevPlan &P, const FastCin &F, long long cb, int p)
+{
+	const int pin = p < 0 ? 0 : (p >= P.ih ? P.ih - 1 : p);
+	return *(const __attribute__((address_space(1))) unsigned int *)((const g_u8 *)P.in + cb + (long long)pin * F.dy);
+}
+
+// fast_fetch_lds<32>'s conversion of a loaded word.
+__device__ __forceinline__ void fast_decode_word(const FastCin &F, const float (*lut)[256], unsigned int word,
+	float v[IWAMD_MAX_CH])
+{
+	const float a = lut[IWAMD_MAX_CH][(word >> (8 * F.ab)) & 0xffu];
+#pragma unroll
+	for (int c = 0; c < IWAMD_MAX_CH; c++) v[c] = lut[c][(word >> (8 * F.chb[c])) & 0xffu] * (F.mode[c] ? a : 1.0f);
+}
+
 // fast_fetch for LEAN plans of 8-bit input through the band's FP32 copies of
 // the input tables in LDS (lut[c], alpha last): the chosen form rounds the
 // sample to FP32 anyway, so the table values and the premultiply are taken in
@@ -1162,16 +1202,33 @@ __device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &
 			acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
 			acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
 		}
+		// 4-byte pixels through the LDS tables: the next K step's words are
+		// loaded while this step's are converted and multiplied.
+		constexpr bool WORDS = LEAN && DEPTH == 32;
+		unsigned int wcur[4], wnxt[4];
+		if constexpr (WORDS) {
+#pragma unroll
+			for (int t = 0; t < 4; t++) wcur[t] = fast_word(P, F, cb, w.p_lo + 4 * lq + t);
+		}
 		for (int kk = 0; kk < kv_pad; kk += 16) {
 			const half4_t ah = *(const half4_t *)&S.wv[0][lr][kk + 4 * lq];
 			const half4_t al = *(const half4_t *)&S.wv[1][lr][kk + 4 * lq];
 			float v[4][IWAMD_MAX_CH];
+			if constexpr (WORDS) {
+				if (kk + 16 < kv_pad) {
 #pragma unroll
-			for (int t = 0; t < 4; t++) {
-				if constexpr (LEAN && DEPTH != 16 && DEPTH != 0)
-					fast_fetch_lds<DEPTH>(P, F, S.lut, cb, w.p_lo + kk + 4 * lq + t, v[t]);
-				else
-					fast_fetch<DEPTH, LEAN>(P, F, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+					for (int t = 0; t < 4; t++) wnxt[t] = fast_word(P, F, cb, w.p_lo + kk + 16 + 4 * lq + t);
+				}
+#pragma unroll
+				for (int t = 0; t < 4; t++) fast_decode_word(F, S.lut, wcur[t], v[t]);
+			} else {
+#pragma unroll
+				for (int t = 0; t < 4; t++) {
+					if constexpr (LEAN && DEPTH != 16 && DEPTH != 0)
+						fast_fetch_lds<DEPTH>(P, F, S.lut, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+					else
+						fast_fetch<DEPTH, LEAN>(P, F, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+				}
 			}
 			half4_t bh[IWAMD_MAX_CH], bl[IWAMD_MAX_CH];
 #pragma unroll
@@ -1191,6 +1248,10 @@ __device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &
 				acc[c] = mfma16(ah, bh[c], acc[c]);
 				acl[c] = mfma16(ah, bl[c], acl[c]);
 				acl[c] = mfma16(al, bh[c], acl[c]);
+			}
+			if constexpr (WORDS) {
+#pragma unroll
+				for (int t = 0; t < 4; t++) wcur[t] = wnxt[t];
 			}
 		}
 #pragma unroll
