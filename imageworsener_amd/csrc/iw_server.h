@@ -215,19 +215,24 @@ private:
 					progress = true;
 				}
 			}
-			// Ended flights release their bytes, memory and stream slot.
+			// Ended flights complete their requests, then release their bytes,
+			// memory and stream slot.  A flight stays listed until all of
+			// that is done, so drain() cannot return while its requests are
+			// still completing or its bytes are still counted.  Only this
+			// thread changes flights_, so index k stays valid while the lock
+			// is released.
 			for (size_t k = 0; k < flights_.size();) {
 				Flight *F = flights_[k];
 				if (!x_.flight_done(*F)) {
 					k++;
 					continue;
 				}
-				flights_.erase(flights_.begin() + k);
 				lock.unlock();
 				for (Req *R : F->reqs)
 					if (!R->finished) complete_req(*F, *R);
 				x_.release(*F);
 				lock.lock();
+				flights_.erase(flights_.begin() + k);
 				used_ -= F->held;
 				free_slots_.push_back(F->slot);
 				delete F;

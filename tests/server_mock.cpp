@@ -5,6 +5,8 @@
 // writes coherent host memory) and then the flight's completion, while
 // client threads submit, wait and drain.  Checks:
 //   - every request completes, with its own result, in every scenario;
+//   - a drain returns only once every request is done and every flight has
+//     released its bytes;
 //   - the bytes held return to 0 after each drain, and never exceed the
 //     budget unless one request alone is larger (admitted on an idle device);
 //   - requests whose total bytes are many times the budget all complete (the
@@ -34,6 +36,15 @@ int g_fail = 0;
 	} while (0)
 
 struct MockExec;
+
+// Every request the loop holds is done (what a successful drain promises).
+template <class Loop>
+bool all_done(Loop &loop)
+{
+	bool all = true;
+	loop.for_each_req([&](typename Loop::Req &R) { all = all && R.done; });
+	return all;
+}
 
 struct MockReq {
 	int tag = 0;          // the value the request's result must carry
@@ -226,7 +237,7 @@ void scenario_mixed()
 	for (auto &t : th) t.join();
 	std::string why;
 	for (int i = 0; i < n; i++) CHECK(s.loop.wait(ids[i], 60000, &why) == 1);
-	CHECK(s.loop.drain(60000, &why) == 1);
+	CHECK(s.loop.drain(60000, &why) == 1); CHECK(all_done(s.loop));
 	for (int i = 0; i < n; i++) CHECK(s.results[i] == i * 7 + 1);
 	CHECK(s.loop.used() == 0);
 	CHECK(s.loop.max_used() <= 6000 + budget);
@@ -245,7 +256,7 @@ void scenario_overlap()
 		std::this_thread::sleep_for(std::chrono::microseconds(300));
 	}
 	std::string why;
-	CHECK(s.loop.drain(60000, &why) == 1);
+	CHECK(s.loop.drain(60000, &why) == 1); CHECK(all_done(s.loop));
 	for (int i = 0; i < n; i++) CHECK(s.results[i] == i * 7 + 1);
 	CHECK(s.x.max_in_flight.load() >= 2);
 	CHECK(s.x.max_in_flight.load() <= 4);
@@ -264,7 +275,7 @@ void scenario_budget_turnover()
 	std::string why;
 	for (int w = 0; w < waves; w++) {
 		for (int i = 0; i < per; i++) s.loop.submit(s.req(w * per + i, 60, 20, i % 3 == 0));
-		CHECK(s.loop.drain(30000, &why) == 1);
+		CHECK(s.loop.drain(30000, &why) == 1); CHECK(all_done(s.loop));
 		CHECK(s.loop.used() == 0);
 	}
 	for (int i = 0; i < waves * per; i++) CHECK(s.results[i] == i * 7 + 1);
@@ -288,7 +299,7 @@ void scenario_timeout()
 	CHECK(s.loop.wait(b, 30000, &why) == 1);
 	s.dev.release();
 	CHECK(s.loop.wait(a, 30000, &why) == 1);
-	CHECK(s.loop.drain(30000, &why) == 1);
+	CHECK(s.loop.drain(30000, &why) == 1); CHECK(all_done(s.loop));
 	CHECK(s.loop.used() == 0);
 }
 
