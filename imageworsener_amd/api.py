@@ -157,6 +157,57 @@ class TuneEntry(ctypes.Structure):
                 ("n", ctypes.c_int), ("cand", ctypes.c_int * 3), ("ms", ctypes.c_double * 3)]
 
 
+_READ_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                            ctypes.POINTER(ctypes.c_size_t))
+_WRITE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+_CLOSE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+_SIZE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64))
+_SEEK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int)
+_TELL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64))
+
+
+class IODescr(ctypes.Structure):
+    """struct iw_iodescr (include/imagew.h)."""
+    _fields_ = [("fp", ctypes.c_void_p), ("read_fn", _READ_FN), ("write_fn", _WRITE_FN), ("close_fn", _CLOSE_FN),
+                ("getfilesize_fn", _SIZE_FN), ("seek_fn", _SEEK_FN), ("tell_fn", _TELL_FN)]
+
+
+class MemFile:
+    """An in-memory file behind an iw_iodescr: read from `data`, or collect
+    what is written in `buf`."""
+
+    def __init__(self, data=b""):
+        self.buf = bytearray(data)
+        self.pos = 0
+
+        def rd(ctx, io, buf, n, got):
+            k = max(0, min(n, len(self.buf) - self.pos))
+            ctypes.memmove(buf, bytes(self.buf[self.pos:self.pos + k]), k)
+            self.pos += k
+            got[0] = k
+            return 1
+
+        def wr(ctx, io, buf, n):
+            self.buf += ctypes.string_at(buf, n)
+            return 1
+
+        def size(ctx, io, p):
+            p[0] = len(self.buf)
+            return 1
+
+        def seek(ctx, io, off, whence):
+            self.pos = off if whence == 0 else (self.pos + off if whence == 1 else len(self.buf) + off)
+            return 1
+
+        def tell(ctx, io, p):
+            p[0] = self.pos
+            return 1
+
+        self._cb = (_READ_FN(rd), _WRITE_FN(wr), _CLOSE_FN(lambda c, i: 1), _SIZE_FN(size), _SEEK_FN(seek),
+                    _TELL_FN(tell))
+        self.io = IODescr(None, *self._cb)
+
+
 class Library:
     """A loaded iw_* library.  mode RTLD_LOCAL keeps several such libraries
     (this one and the reference core) apart in one process."""
@@ -262,22 +313,23 @@ class Library:
 
     def _setup(self, ctx, job):
         d = self.dll
-        im = job["image"]
-        pix = np.ascontiguousarray(im["pixels"], dtype=np.uint8).reshape(-1)
-        bpr = im["bpr"]
-        mem = d.iw_malloc_large(ctx, bpr, im["height"])
-        if not mem:
-            raise MemoryError("iw_malloc_large failed")
-        ctypes.memmove(mem, pix.ctypes.data, bpr * im["height"])
-        img = Image()
-        img.imgtype = im["imgtype"]
-        img.bit_depth = im["bit_depth"]
-        img.sampletype = im.get("sampletype", 0)
-        img.width = im["width"]
-        img.height = im["height"]
-        img.pixels = mem
-        img.bpr = bpr
-        d.iw_set_input_image(ctx, ctypes.byref(img))
+        if "image" in job:
+            im = job["image"]
+            pix = np.ascontiguousarray(im["pixels"], dtype=np.uint8).reshape(-1)
+            bpr = im["bpr"]
+            mem = d.iw_malloc_large(ctx, bpr, im["height"])
+            if not mem:
+                raise MemoryError("iw_malloc_large failed")
+            ctypes.memmove(mem, pix.ctypes.data, bpr * im["height"])
+            img = Image()
+            img.imgtype = im["imgtype"]
+            img.bit_depth = im["bit_depth"]
+            img.sampletype = im.get("sampletype", 0)
+            img.width = im["width"]
+            img.height = im["height"]
+            img.pixels = mem
+            img.bpr = bpr
+            d.iw_set_input_image(ctx, ctypes.byref(img))
         if "crop" in job:
             d.iw_set_input_crop(ctx, *job["crop"])
         if "reorient" in job:
@@ -288,7 +340,8 @@ class Library:
         if "output_cs" in job:
             cs = CsDescr(job["output_cs"][0], 0, job["output_cs"][1])
             d.iw_set_output_colorspace(ctx, ctypes.byref(cs))
-        d.iw_set_output_profile(ctx, job.get("profile", PROFILE_PNG))
+        if "profile" in job or "image" in job:
+            d.iw_set_output_profile(ctx, job.get("profile", PROFILE_PNG))
         if "depth" in job:
             d.iw_set_output_depth(ctx, job["depth"])
         for dim, (fam, blur, p1, p2) in job.get("resize", {}).items():
@@ -322,7 +375,60 @@ class Library:
             d.iwamd_set_precision(ctx, job["precision"])
         for name, val in job.get("options", []):
             d.iw_set_option(ctx, name.encode(), val.encode())
-        d.iw_set_output_canvas_size(ctx, job["canvas"][0], job["canvas"][1])
+        if "canvas" in job:
+            d.iw_set_output_canvas_size(ctx, job["canvas"][0], job["canvas"][1])
+
+    def run_file(self, data, read, write, job=None, executor=None):
+        """A file through the library the way the reference's CLI drives it:
+        read `data` with the codec function named `read` (e.g.
+        "iw_read_pnm_file", or ("iw_read_file_by_fmt", fmt)), apply `job`'s
+        settings (no "image" key; its
+        "profile" defaults to the one iw_get_profile_by_fmt gives the output
+        format), process (iw_process_image, or `executor` as in run()), and
+        write with the codec function named `write`.  `write` is (name, fmt):
+        the format is set as the output format first, and passed on to a
+        *_by_fmt function.  Returns (ok, bytes, error message)."""
+        d = self.dll
+        job = dict(job or {})
+        ctx = d.iw_create_context(None)
+        if not ctx:
+            raise MemoryError("iw_create_context failed")
+        try:
+            src, dst = MemFile(data), MemFile()
+            rname, rfmt = read if isinstance(read, tuple) else (read, None)
+            rfn = getattr(d, rname)
+            rfn.restype = ctypes.c_int
+            rfn.argtypes = [ctypes.c_void_p, ctypes.POINTER(IODescr)] + ([] if rfmt is None else [ctypes.c_int])
+            wname, fmt = write
+            ok = bool(rfn(ctx, ctypes.byref(src.io)) if rfmt is None else rfn(ctx, ctypes.byref(src.io), rfmt))
+            if ok:
+                if "profile" not in job:
+                    d.iw_get_profile_by_fmt.argtypes, d.iw_get_profile_by_fmt.restype = [ctypes.c_int], ctypes.c_uint
+                    job["profile"] = d.iw_get_profile_by_fmt(fmt)
+                self._setup(ctx, job)
+                d.iw_set_value(ctx, 52, fmt)  # IW_VAL_OUTPUT_FORMAT
+                if executor is None:
+                    ok = bool(d.iw_process_image(ctx))
+                else:
+                    plan = ctypes.create_string_buffer(d.iwamd_plan_size())
+                    ok = bool(d.iwamd_prepare(ctx, plan))
+                    if ok:
+                        out_bpr, out_h = plan_output_size(plan)
+                        img2 = ctypes.create_string_buffer(out_bpr * out_h + 1)
+                        ok = bool(executor(plan, img2)) and bool(d.iwamd_finish(ctx, img2))
+            if ok:
+                wfn = getattr(d, wname)
+                wfn.restype = ctypes.c_int
+                by_fmt = wname.endswith("_by_fmt")
+                wfn.argtypes = [ctypes.c_void_p, ctypes.POINTER(IODescr)] + ([ctypes.c_int] if by_fmt else [])
+                ok = bool(wfn(ctx, ctypes.byref(dst.io), fmt) if by_fmt else wfn(ctx, ctypes.byref(dst.io)))
+            err = None
+            if d.iw_get_errorflag(ctx):
+                buf = ctypes.create_string_buffer(200)
+                err = d.iw_get_errormsg(ctx, buf, 200).decode(errors="replace")
+            return ok and not err, bytes(dst.buf), err
+        finally:
+            d.iw_destroy_context(ctx)
 
     def _collect(self, ctx, ok):
         d = self.dll

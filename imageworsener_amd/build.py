@@ -19,7 +19,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libimageworsener_amd.so")
 BUILD_DIR = os.path.join(ROOT, "_build", "lib")
 
-SOURCES = ["iw_api.cpp", "iw_prepare.cpp", "iw_finish.cpp", "iw_tables.cpp", "iw_pnm.cpp", "iw_gpu.hip"]
+SOURCES = ["iw_api.cpp", "iw_prepare.cpp", "iw_finish.cpp", "iw_tables.cpp", "iw_pnm.cpp", "iw_miff.cpp",
+           "iw_gpu.hip"]
 
 ARCH = os.environ.get("IWAMD_OFFLOAD_ARCH", "gfx950")
 

@@ -1086,8 +1086,6 @@ IWX_NO_CODEC(iw_write_jpeg_file)
 IWX_NO_CODEC(iw_read_bmp_file)
 IWX_NO_CODEC(iw_write_bmp_file)
 IWX_NO_CODEC(iw_write_tiff_file)
-IWX_NO_CODEC(iw_read_miff_file)
-IWX_NO_CODEC(iw_write_miff_file)
 IWX_NO_CODEC(iw_read_webp_file)
 IWX_NO_CODEC(iw_write_webp_file)
 IWX_NO_CODEC(iw_read_gif_file)

@@ -1,5 +1,6 @@
 // PNM (PBM/PGM/PPM, binary) and PAM reading and writing, and the format
-// dispatch iw_read_file_by_fmt / iw_write_file_by_fmt.
+// dispatch iw_read_file_by_fmt / iw_write_file_by_fmt (PNM, PAM, and MIFF
+// through iw_miff.cpp).
 //
 // The dependency-free on-disk format on either side of the pixel path.
 // Behaviour follows the reference's src/imagew-pnm.c (:1-958) and
@@ -349,6 +350,7 @@ extern "C" int iw_write_pam_file(struct iw_context *ctx, struct iw_iodescr *io)
 extern "C" int iw_read_file_by_fmt(struct iw_context *ctx, struct iw_iodescr *io, int fmt)
 {
 	if (fmt == IW_FORMAT_PNM || fmt == IW_FORMAT_PAM) return iw_read_pnm_file(ctx, io);
+	if (fmt == IW_FORMAT_MIFF) return iw_read_miff_file(ctx, io);
 	const char *s = iw_get_fmt_name(fmt);
 	if (!s) {
 		iw_set_errorf(ctx, "Attempt to read unknown file format (%d)", fmt);
@@ -368,6 +370,8 @@ extern "C" int iw_write_file_by_fmt(struct iw_context *ctx, struct iw_iodescr *i
 	case IW_FORMAT_PBM:
 	case IW_FORMAT_PAM:
 		return iw_write_pnm_file(ctx, io);
+	case IW_FORMAT_MIFF:
+		return iw_write_miff_file(ctx, io);
 	}
 	const char *s = iw_get_fmt_name(fmt);
 	if (!s) {

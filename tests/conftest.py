@@ -9,6 +9,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libiwref.so")
+REF_CODECS_LIB = os.path.join(ROOT, "oracle", "_ref", "libiwref_codecs.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "libiwo.so")
 
 
@@ -28,6 +29,15 @@ def ref():
     if not os.path.exists(REF_LIB):
         pytest.skip("reference core not built (oracle/_ref needs /root/reference at build time)")
     return api.Library(REF_LIB, extensions=False)
+
+
+@pytest.fixture(scope="session")
+def ref_codecs():
+    """The reference core with its own codecs (PNM/PAM, MIFF, ...)."""
+    from imageworsener_amd import api
+    if not os.path.exists(REF_CODECS_LIB):
+        pytest.skip("reference codecs not built (oracle/_ref needs /root/reference at build time)")
+    return api.Library(REF_CODECS_LIB, extensions=False)
 
 
 @pytest.fixture(scope="session")
