@@ -353,6 +353,7 @@ def bench_c3(a):
     if not a.no_c1:
         line["c1"] = c1_side(a, lib)
     print(json.dumps(line), flush=True)
+    copy_probe(lib, a.copy_probe)
 
 
 def c1_job(size, out):
@@ -424,6 +425,19 @@ def c1_side(a, lib):
     return out
 
 
+def copy_probe(lib, mib):
+    """One plain device-to-device copy of mib MiB after the timed work: the
+    reference point that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    if not mib:
+        return
+    n = mib << 20
+    t = ctypes.c_double()
+    lib.dll.iwamd_copy_probe.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
+    if not lib.dll.iwamd_copy_probe(n, ctypes.byref(t)):
+        raise RuntimeError("copy probe failed")
+    log("copy probe: %d bytes in %.3f ms (%.1f GB/s read+write)" % (n, t.value, 2 * n / t.value / 1e6))
+
+
 def bench_c1(a):
     """C1 on its own: one line whose value is the plain FP64 form (byte-
     identical to the reference; the chosen form differs by more than 1 LSB
@@ -452,13 +466,7 @@ def bench_c1(a):
             if "cpu_reference_one_thread" in side else None,
             "c1": side}
     print(json.dumps(line), flush=True)
-    if a.copy_probe:
-        n = a.copy_probe << 20
-        t = ctypes.c_double()
-        lib.dll.iwamd_copy_probe.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
-        if not lib.dll.iwamd_copy_probe(n, ctypes.byref(t)):
-            raise RuntimeError("copy probe failed")
-        log("copy probe: %d bytes in %.3f ms (%.1f GB/s read+write)" % (n, t.value, 2 * n / t.value / 1e6))
+    copy_probe(lib, a.copy_probe)
 
 
 def bench_c2(a):
