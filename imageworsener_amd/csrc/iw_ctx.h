@@ -201,10 +201,38 @@ void iwx_fill_plan(iw_context *ctx, iwamd_plan *plan);
 double iwx_x_to_linear(double v, const struct iw_csdescr *cs);
 double iwx_linear_to_x(double v, const struct iw_csdescr *cs);
 double iwx_color_to_grayscale(iw_context *ctx, double r, double g, double b);
+struct iwx_opt_scan;
+int iwx_finish_image(struct iw_context *ctx, const uint8_t *img2, const struct iwx_opt_scan *scan);
+
+// What the output optimiser starts from, when the device has computed it over
+// the finished image: the flags of src/imagew-opt.c:18-240, for each form a
+// binary-transparency image can take the values of its varying colour-key
+// component among opaque pixels (:418-683), and the set of distinct 8-bit
+// colours while there are at most 256 (:719-797).  The device has also
+// cleared the colour of fully transparent pixels (:1112-1163).
+#define IWX_SCAN_COLOR   1u
+#define IWX_SCAN_TRNS    2u
+#define IWX_SCAN_PARTIAL 4u
+#define IWX_SCAN_16BIT   8u
+#define IWX_SCAN_HAVE_FLAGS 1u  // iwx_opt_scan::valid bits: what was computed
+#define IWX_SCAN_HAVE_KEYS  2u
+#define IWX_SCAN_HAVE_PAL   4u
+#define IWX_SCAN_PAL_SLOTS 512
+enum { IWX_KEY_RGB8 = 0, IWX_KEY_RGB16 = 1, IWX_KEY_GRAY8 = 2, IWX_KEY_GRAY16 = 3 };
+struct iwx_opt_scan {
+	unsigned int valid;       // IWX_SCAN_HAVE_*; 0: nothing (the host scans)
+	unsigned int flags;       // IWX_SCAN_COLOR ...
+	unsigned int used[4][8];  // [IWX_KEY_*]: 256-bit sets of the key component values in use
+	unsigned int pal_count;   // colours in pal[]
+	unsigned int pal_over;    // nonzero: more than 256 distinct colours
+	// Open-addressed set of colours: 0 empty, else 2^32 | r | g<<8 | b<<16 | a<<24
+	// (the colour of a fully transparent pixel taken as 0,0,0,0).
+	unsigned long long pal[IWX_SCAN_PAL_SLOTS];
+};
 
 // iw_finish.cpp
-int iwx_finish(iw_context *ctx);
-void iwx_optimize_image(iw_context *ctx);
+int iwx_finish(iw_context *ctx, const iwx_opt_scan *scan = nullptr);
+void iwx_optimize_image(iw_context *ctx, const iwx_opt_scan *scan = nullptr);
 
 // iw_gpu.hip
 int iwx_gpu_process(iw_context *ctx);

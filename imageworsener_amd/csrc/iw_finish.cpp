@@ -116,6 +116,16 @@ inline unsigned int sample_at(const iwx_opt_ctx &o, int nc, int i, int j, int k)
 // One pass recording colour / transparency / 16-bit precision, with the
 // reference's early stop once every property it can use is known
 // (src/imagew-opt.c:18-284).
+// The image types and depths scan_pixels examines (else the reductions are
+// skipped).
+bool scan_types_ok(const iwx_opt_ctx &o)
+{
+	const int t = o.imgtype, bd = o.bit_depth;
+	if (bd != 8 && bd != 16) return false;
+	if (t == IW_IMGTYPE_GRAY && bd == 8) return false;
+	return t == IW_IMGTYPE_GRAY || t == IW_IMGTYPE_GRAYA || t == IW_IMGTYPE_RGB || t == IW_IMGTYPE_RGBA;
+}
+
 int scan_pixels(iwx_opt_ctx &o)
 {
 	const int t = o.imgtype, bd = o.bit_depth;
@@ -226,8 +236,8 @@ int find_unused(const iw_byte *flags, int count, iw_byte *out)
 
 // RGBA/GA with only fully opaque or fully transparent pixels becomes RGB/G
 // plus a colour key chosen among values the image does not use
-// (src/imagew-opt.c:416-676).
-void try_binary_trns(iw_context *ctx, iwx_opt_ctx &o)
+// (src/imagew-opt.c:418-683).
+void try_binary_trns(iw_context *ctx, iwx_opt_ctx &o, const iwx_opt_scan *scan)
 {
 	if (!(ctx->output_profile & IW_PROFILE_BINARYTRNS)) return;
 	if (!ctx->opt_binary_trns) return;
@@ -236,13 +246,19 @@ void try_binary_trns(iw_context *ctx, iwx_opt_ctx &o)
 	const int nc = rgb ? 4 : 2;
 	iw_byte used[256];
 	memset(used, 0, sizeof(used));
+	const bool have_keys = scan && (scan->valid & IWX_SCAN_HAVE_KEYS);
+	if (have_keys) {
+		// The device's sets of key components in use (the same tests below).
+		const int key = rgb ? (b16 ? IWX_KEY_RGB16 : IWX_KEY_RGB8) : (b16 ? IWX_KEY_GRAY16 : IWX_KEY_GRAY8);
+		for (int v = 0; v < 256; v++) used[v] = (scan->used[key][v >> 5] >> (v & 31)) & 1u;
+	}
 	iw_byte *mask = (iw_byte *)iw_malloc_large(ctx, o.width, o.height);
 	if (!mask) return;
 	for (int j = 0; j < o.height; j++) {
 		for (int i = 0; i < o.width; i++) {
 			const unsigned int a = sample_at(o, nc, i, j, nc - 1);
 			mask[(size_t)j * o.width + i] = (a != 0);
-			if (a == 0) continue;
+			if (a == 0 || have_keys) continue;
 			if (rgb) {
 				const unsigned int r = sample_at(o, nc, i, j, 0), g = sample_at(o, nc, i, j, 1),
 					b = sample_at(o, nc, i, j, 2);
@@ -319,15 +335,29 @@ int palette_find(const struct iw_palette *pal, const struct iw_rgba8color &c)
 	return -1;
 }
 
-int collect_palette(iwx_opt_ctx &o)
+// The distinct colours of the image, in the order of first appearance, or 0
+// when there are more than 256.  With the device's set the order differs;
+// every later use sorts the entries or only tests membership
+// (src/imagew-opt.c:871-899, :906-1004), so the outcome is the same.
+int collect_palette(iwx_opt_ctx &o, const iwx_opt_scan *scan)
 {
 	struct iw_palette *pal = o.palette;
-	for (int y = 0; y < o.height; y++) {
-		for (int x = 0; x < o.width; x++) {
-			struct iw_rgba8color c = pixel_color(o, x, y);
-			if (palette_find(pal, c) >= 0) continue;
+	if (scan && (scan->valid & IWX_SCAN_HAVE_PAL)) {
+		if (scan->pal_over) return 0;
+		for (int s = 0; s < IWX_SCAN_PAL_SLOTS; s++) {
+			const unsigned long long v = scan->pal[s];
+			if (!v) continue;
 			if (pal->num_entries >= 256) return 0;
-			pal->entry[pal->num_entries++] = c;
+			pal->entry[pal->num_entries++] = {(iw_byte)v, (iw_byte)(v >> 8), (iw_byte)(v >> 16), (iw_byte)(v >> 24)};
+		}
+	} else {
+		for (int y = 0; y < o.height; y++) {
+			for (int x = 0; x < o.width; x++) {
+				struct iw_rgba8color c = pixel_color(o, x, y);
+				if (palette_find(pal, c) >= 0) continue;
+				if (pal->num_entries >= 256) return 0;
+				pal->entry[pal->num_entries++] = c;
+			}
 		}
 	}
 	if (pal->num_entries < 1) return 0;
@@ -431,18 +461,52 @@ int palette_order(const void *p1, const void *p2)
 	return 0;
 }
 
+// Colour -> palette index, the first entry that matches as palette_find
+// gives it, through an open-addressed table of the at most 256 entries.
+struct PaletteIndex {
+	uint32_t key[1024];
+	int16_t idx[1024];
+	static uint32_t pack(const struct iw_rgba8color &c)
+	{
+		return (uint32_t)c.r | ((uint32_t)c.g << 8) | ((uint32_t)c.b << 16) | ((uint32_t)c.a << 24);
+	}
+	static unsigned slot(uint32_t k) { return (k * 2654435761u) >> 22; }
+	explicit PaletteIndex(const struct iw_palette *pal)
+	{
+		memset(idx, 0xff, sizeof(idx));
+		for (int i = 0; i < pal->num_entries; i++) {
+			const uint32_t k = pack(pal->entry[i]);
+			unsigned s = slot(k);
+			while (idx[s] >= 0 && key[s] != k) s = (s + 1) & 1023;
+			if (idx[s] < 0) {
+				key[s] = k;
+				idx[s] = (int16_t)i;
+			}
+		}
+	}
+	int find(const struct iw_rgba8color &c) const
+	{
+		const uint32_t k = pack(c);
+		for (unsigned s = slot(k);; s = (s + 1) & 1023) {
+			if (idx[s] < 0) return -1;
+			if (key[s] == k) return idx[s];
+		}
+	}
+};
+
 void convert_to_palette(iw_context *ctx, iwx_opt_ctx &o)
 {
 	size_t nbpr = o.width;
 	iw_byte *np = (iw_byte *)iw_malloc_large(ctx, nbpr, o.height);
 	if (!np) return;
+	const PaletteIndex index(o.palette);
 	for (int y = 0; y < o.height; y++) {
 		for (int x = 0; x < o.width; x++) {
 			struct iw_rgba8color c = pixel_color(o, x, y);
 			int e;
 			if (o.has_colorkey_trns && c.a == 0) e = (int)o.colorkey[IW_CHANNELTYPE_RED];
 			else {
-				e = palette_find(o.palette, c);
+				e = index.find(c);
 				if (e < 0) e = 0;
 			}
 			np[(size_t)y * nbpr + x] = (iw_byte)e;
@@ -455,7 +519,7 @@ void convert_to_palette(iw_context *ctx, iwx_opt_ctx &o)
 
 // Preference gray1, pal1, gray2, pal2, gray4, pal4, (gray8: leave), pal8
 // (src/imagew-opt.c:1010-1094).
-void try_palette(iw_context *ctx, iwx_opt_ctx &o)
+void try_palette(iw_context *ctx, iwx_opt_ctx &o, const iwx_opt_scan *scan)
 {
 	const unsigned int p = ctx->output_profile;
 	if (!(p & (IW_PROFILE_PAL1 | IW_PROFILE_PAL2 | IW_PROFILE_PAL4 | IW_PROFILE_PAL8 |
@@ -468,7 +532,7 @@ void try_palette(iw_context *ctx, iwx_opt_ctx &o)
 	int bt = 0;
 	unsigned int shade = 0;
 	bool chosen = false;
-	if (collect_palette(o)) {
+	if (collect_palette(o, scan)) {
 		static const int bpps[3] = {1, 2, 4};
 		static const unsigned int grayflag[3] = {IW_PROFILE_GRAY1, IW_PROFILE_GRAY2, IW_PROFILE_GRAY4};
 		bool stop = false;
@@ -503,7 +567,7 @@ void try_palette(iw_context *ctx, iwx_opt_ctx &o)
 	}
 }
 
-// Zero the colour of fully transparent pixels (src/imagew-opt.c:1123-1164).
+// Zero the colour of fully transparent pixels (src/imagew-opt.c:1112-1163).
 void make_transparent_pixels_black(iw_image &im)
 {
 	if (!IW_IMGTYPE_HAS_ALPHA(im.imgtype)) return;
@@ -520,8 +584,9 @@ void make_transparent_pixels_black(iw_image &im)
 
 }  // namespace
 
-void iwx_optimize_image(iw_context *ctx)
+void iwx_optimize_image(iw_context *ctx, const iwx_opt_scan *scan)
 {
+	if (scan && !(scan->valid & IWX_SCAN_HAVE_FLAGS)) scan = nullptr;
 	iwx_opt_ctx &o = ctx->optctx;
 	o.width = ctx->img2.width;
 	o.height = ctx->img2.height;
@@ -537,7 +602,7 @@ void iwx_optimize_image(iw_context *ctx)
 	if (ctx->img2.sampletype != IW_SAMPLETYPE_UINT) return;
 	if (ctx->reduced_output_maxcolor_flag) return;
 
-	make_transparent_pixels_black(ctx->img2);
+	if (!scan) make_transparent_pixels_black(ctx->img2);
 
 	if (o.has_bkgdlabel) {
 		if (o.bkgdlabel[0] != o.bkgdlabel[1] || o.bkgdlabel[0] != o.bkgdlabel[2]) o.has_color = 1;
@@ -545,7 +610,13 @@ void iwx_optimize_image(iw_context *ctx)
 			o.has_16bit_precision = 1;
 	}
 
-	if (scan_pixels(o)) {
+	if (scan ? scan_types_ok(o) : scan_pixels(o)) {
+		if (scan) {
+			if (scan->flags & IWX_SCAN_COLOR) o.has_color = 1;
+			if (scan->flags & IWX_SCAN_TRNS) o.has_transparency = 1;
+			if (scan->flags & IWX_SCAN_PARTIAL) o.has_partial_transparency = 1;
+			if (scan->flags & IWX_SCAN_16BIT) o.has_16bit_precision = 1;
+		}
 		const int t0 = o.imgtype;
 		if (o.bit_depth == 16 && !o.has_16bit_precision) reduce_16_to_8(ctx, o, iw_imgtype_num_channels(t0));
 		if (!o.has_transparency && ctx->opt_strip_alpha) {
@@ -558,17 +629,21 @@ void iwx_optimize_image(iw_context *ctx)
 		}
 	}
 
-	try_palette(ctx, o);
+	try_palette(ctx, o, scan);
 
 	if ((o.imgtype == IW_IMGTYPE_RGBA || o.imgtype == IW_IMGTYPE_GRAYA) && !o.has_partial_transparency &&
 		(o.bit_depth == 8 || o.bit_depth == 16))
-		try_binary_trns(ctx, o);
+		try_binary_trns(ctx, o, scan);
 }
 
-int iwx_finish(iw_context *ctx)
+int iwx_finish(iw_context *ctx, const iwx_opt_scan *scan)
 {
 	process_bkgd_label(ctx);
-	if (ctx->req.negate_target) negate_target_image(ctx);
-	iwx_optimize_image(ctx);
+	// The device scans only images that are not negated afterwards.
+	if (ctx->req.negate_target) {
+		negate_target_image(ctx);
+		scan = nullptr;
+	}
+	iwx_optimize_image(ctx, scan);
 	return 1;
 }

@@ -37,7 +37,9 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <tuple>
+#include <type_traits>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -2089,6 +2091,320 @@ struct Layout {
 	}
 };
 
+// ---- output optimiser scans (src/imagew-opt.c) ------------------------------
+// One pass over each finished 8/16-bit output image before it is copied
+// back.  It clears the colour of fully transparent pixels in place
+// (:1112-1163) and forms what the optimiser decides by: colour,
+// transparency, partial transparency and 16-bit precision (:18-241, :358-389); for
+// each form a binary-transparency image can take (RGB or gray, 8 or 16
+// bits, :418-683), the values of its varying colour-key component among
+// opaque pixels; and the set of distinct 8-bit colours, as the palette
+// search sees them after any 16->8 reduction, while there are at most 256
+// (:719-797).  The host optimiser then only builds the new image.
+//
+// A workgroup takes one chunk of OPT_CHUNK pixels of one image: flags and
+// key sets in LDS words, colours in an LDS hash set, then one merge into the
+// image's result with global atomics.  HBM-bound: each pixel is read once
+// and written only where a transparent pixel's colour is cleared.
+
+#define OPT_WANT_KEYS 2u  // = IWX_SCAN_HAVE_KEYS
+#define OPT_WANT_PAL 4u   // = IWX_SCAN_HAVE_PAL
+#define OPT_THREADS 256
+#define OPT_PPT 16
+#define OPT_CHUNK (OPT_THREADS * OPT_PPT)
+#define OPT_LSLOTS 512
+
+static_assert(sizeof(iwx_opt_scan) % 16 == 0, "result records are cleared in 16-byte words");
+
+struct OptJob {
+	unsigned char *px;        // width*height pixels of nc samples, rows unpadded,
+	                          // 16-bit samples big-endian
+	iwx_opt_scan *res;
+	unsigned long long npx;
+	unsigned int chunk0;      // first chunk of this image in the launch
+	int nc, b16;
+	unsigned int want;        // IWX_SCAN_HAVE_* beyond the flags
+};
+
+struct OptShared {
+	unsigned long long tab[OPT_LSLOTS];
+	unsigned int used[4][8];
+	unsigned int fl, pal_n, pal_over;
+	int job;
+};
+
+__device__ __forceinline__ unsigned int opt_hash(unsigned int c, unsigned int slots)
+{
+	return (c * 2654435761u) >> (32 - __builtin_ctz(slots));
+}
+
+// Enters colour c into an open-addressed set of `slots` entries: 1 when it
+// is new, 0 when present, -1 when the set is full.
+__device__ __forceinline__ int opt_set_add(unsigned long long *tab, unsigned int slots, unsigned int c)
+{
+	const unsigned long long key = (1ull << 32) | c;
+	unsigned int h = opt_hash(c, slots);
+	for (unsigned int n = 0; n < slots; n++, h = (h + 1) & (slots - 1)) {
+		unsigned long long s = tab[h];
+		if (s == 0) {
+			s = atomicCAS(&tab[h], 0ull, key);
+			if (s == 0) return 1;
+		}
+		if (s == key) return 0;
+	}
+	return -1;
+}
+
+__device__ __forceinline__ void opt_bit(unsigned int *set, unsigned int v)
+{
+	const unsigned int b = 1u << (v & 31);
+	unsigned int *w = set + (v >> 5);
+	if (!(*(volatile unsigned int *)w & b)) atomicOr(w, b);
+}
+
+// One pixel's bytes as they lie in memory (little-endian word), and its
+// samples (16-bit ones stored big-endian).
+template <int NC, int B16>
+using OptRaw = typename std::conditional<(NC * (B16 ? 2 : 1) > 4), unsigned long long, unsigned int>::type;
+
+template <int NC, int B16>
+__device__ __forceinline__ OptRaw<NC, B16> opt_raw(const g_u8 *q)
+{
+	constexpr int PB = NC * (B16 ? 2 : 1);
+	typedef const __attribute__((address_space(1))) unsigned int g_cu32;
+	typedef const __attribute__((address_space(1))) unsigned long long g_cu64;
+	if constexpr (PB == 8) return *(g_cu64 *)q;
+	else if constexpr (PB == 4) return *(g_cu32 *)q;
+	else if constexpr (PB == 2) return *(g_cu16 *)q;
+	else {
+		OptRaw<NC, B16> w = 0;
+#pragma unroll
+		for (int k = 0; k < PB; k++) w |= (OptRaw<NC, B16>)q[k] << (8 * k);
+		return w;
+	}
+}
+
+template <int NC, int B16>
+__device__ __forceinline__ void opt_decode(OptRaw<NC, B16> w, unsigned int s[4])
+{
+#pragma unroll
+	for (int k = 0; k < NC; k++) {
+		if constexpr (B16) s[k] = (unsigned int)(((w >> (16 * k)) & 0xffu) << 8) | (unsigned int)((w >> (16 * k + 8)) & 0xffu);
+		else s[k] = (unsigned int)((w >> (8 * k)) & 0xffu);
+	}
+}
+
+template <int NC, int B16>
+__device__ __forceinline__ void opt_clear_px(g_u8 *q)
+{
+	constexpr int PB = NC * (B16 ? 2 : 1);
+	if constexpr (PB == 4) *(__attribute__((address_space(1))) unsigned int *)q = 0u;
+	else if constexpr (PB == 8) *(__attribute__((address_space(1))) unsigned long long *)q = 0ull;
+	else if constexpr (PB == 2) *(__attribute__((address_space(1))) unsigned short *)q = 0;
+	else {
+#pragma unroll
+		for (int k = 0; k < PB; k++) q[k] = 0;
+	}
+}
+
+template <int NC, int B16>
+__device__ void opt_scan_chunk(const OptJob &J, unsigned long long first, OptShared &S)
+{
+	constexpr int PB = NC * (B16 ? 2 : 1);
+	constexpr bool ALPHA = (NC == 2 || NC == 4), RGB = (NC >= 3);
+	constexpr unsigned int AMAX = B16 ? 65535u : 255u;
+	const bool keys = ALPHA && (J.want & OPT_WANT_KEYS);
+	const bool pal = (J.want & OPT_WANT_PAL) != 0;
+	unsigned int fl = 0;
+	// Loads of OPT_BATCH pixels are issued together, then worked through.
+	constexpr int OPT_BATCH = 8;
+	for (int k0 = 0; k0 < OPT_PPT; k0 += OPT_BATCH) {
+	OptRaw<NC, B16> raw[OPT_BATCH];
+#pragma unroll
+	for (int k = 0; k < OPT_BATCH; k++) {
+		const unsigned long long p = first + (unsigned long long)(k0 + k) * OPT_THREADS + threadIdx.x;
+		raw[k] = p < J.npx ? opt_raw<NC, B16>((const g_u8 *)J.px + p * PB) : 0;
+	}
+#pragma unroll
+	for (int k = 0; k < OPT_BATCH; k++) {
+		const unsigned long long p = first + (unsigned long long)(k0 + k) * OPT_THREADS + threadIdx.x;
+		if (p >= J.npx) break;
+		g_u8 *q = (g_u8 *)J.px + p * PB;
+		unsigned int s[4];
+		opt_decode<NC, B16>(raw[k], s);
+		const unsigned int a = ALPHA ? s[NC - 1] : AMAX;
+		if (ALPHA && a == 0) {
+			bool any = false;
+#pragma unroll
+			for (int c = 0; c < NC - 1; c++) any |= s[c] != 0;
+			if (any) {
+				opt_clear_px<NC, B16>(q);
+#pragma unroll
+				for (int c = 0; c < NC - 1; c++) s[c] = 0;
+			}
+		}
+		if (ALPHA && a < AMAX) fl |= IWX_SCAN_TRNS;
+		if (ALPHA && a > 0 && a < AMAX) fl |= IWX_SCAN_PARTIAL;
+		if (B16) {
+#pragma unroll
+			for (int c = 0; c < NC; c++)
+				if ((s[c] >> 8) != (s[c] & 0xffu)) fl |= IWX_SCAN_16BIT;
+		}
+		if (RGB && (s[0] != s[1] || s[0] != s[2])) fl |= IWX_SCAN_COLOR;
+		const unsigned int r8 = B16 ? s[0] >> 8 : s[0];
+		const unsigned int g8 = RGB ? (B16 ? s[1] >> 8 : s[1]) : r8;
+		const unsigned int b8 = RGB ? (B16 ? s[2] >> 8 : s[2]) : r8;
+		if (keys && a != 0) {
+			opt_bit(S.used[IWX_KEY_GRAY8], r8);
+			if (RGB && g8 == 192 && b8 == 192) opt_bit(S.used[IWX_KEY_RGB8], r8);
+			if (B16 && r8 == 192) {
+				opt_bit(S.used[IWX_KEY_GRAY16], s[0] & 0xffu);
+				if (RGB && s[1] == 192u * 257u && s[2] == 192u * 257u) opt_bit(S.used[IWX_KEY_RGB16], s[0] & 0xffu);
+			}
+		}
+		if (pal && !*(volatile unsigned int *)&S.pal_over) {
+			const unsigned int a8 = ALPHA ? (B16 ? a >> 8 : a) : 255u;
+			const unsigned int c = a8 == 0 ? 0u : r8 | (g8 << 8) | (b8 << 16) | (a8 << 24);
+			const int r = opt_set_add(S.tab, OPT_LSLOTS, c);
+			if (r < 0 || (r > 0 && atomicAdd(&S.pal_n, 1u) >= 256u)) *(volatile unsigned int *)&S.pal_over = 1u;
+		}
+	}
+	}
+	if (fl) atomicOr(&S.fl, fl);
+}
+
+__global__ __launch_bounds__(OPT_THREADS) void opt_scan_kernel(const OptJob *__restrict__ jobs, int njobs)
+{
+	__shared__ OptShared S;
+	const unsigned int chunk = blockIdx.x;
+	if (threadIdx.x == 0) {
+		// The last image whose first chunk is at or before this one.
+		int lo = 0, hi = njobs - 1;
+		while (lo < hi) {
+			const int mid = (lo + hi + 1) >> 1;
+			if (jobs[mid].chunk0 <= chunk) lo = mid;
+			else hi = mid - 1;
+		}
+		S.job = lo;
+		S.fl = 0;
+		S.pal_n = 0;
+		S.pal_over = __hip_atomic_load(&jobs[lo].res->pal_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+	for (int i = threadIdx.x; i < OPT_LSLOTS; i += OPT_THREADS) S.tab[i] = 0;
+	if (threadIdx.x < 32) (&S.used[0][0])[threadIdx.x] = 0;
+	__syncthreads();
+	const OptJob J = jobs[S.job];
+	const unsigned long long first = (unsigned long long)(chunk - J.chunk0) * OPT_CHUNK;
+	switch (J.nc * 2 + J.b16) {
+	case 2: opt_scan_chunk<1, 0>(J, first, S); break;
+	case 3: opt_scan_chunk<1, 1>(J, first, S); break;
+	case 4: opt_scan_chunk<2, 0>(J, first, S); break;
+	case 5: opt_scan_chunk<2, 1>(J, first, S); break;
+	case 6: opt_scan_chunk<3, 0>(J, first, S); break;
+	case 7: opt_scan_chunk<3, 1>(J, first, S); break;
+	case 8: opt_scan_chunk<4, 0>(J, first, S); break;
+	case 9: opt_scan_chunk<4, 1>(J, first, S); break;
+	default: break;
+	}
+	__syncthreads();
+	iwx_opt_scan *R = J.res;
+	if (threadIdx.x < 32) {
+		const unsigned int u = (&S.used[0][0])[threadIdx.x];
+		if (u) atomicOr(&(&R->used[0][0])[threadIdx.x], u);
+	}
+	if (threadIdx.x == 0 && S.fl) atomicOr(&R->flags, S.fl);
+	if (!(J.want & OPT_WANT_PAL)) return;
+	if (S.pal_over) {
+		if (threadIdx.x == 0) atomicOr(&R->pal_over, 1u);
+		return;
+	}
+	for (int i = threadIdx.x; i < OPT_LSLOTS; i += OPT_THREADS) {
+		const unsigned long long v = S.tab[i];
+		if (!v) continue;
+		if (__hip_atomic_load(&R->pal_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+		const int r = opt_set_add(R->pal, IWX_SCAN_PAL_SLOTS, (unsigned int)v);
+		if (r < 0 || (r > 0 && atomicAdd(&R->pal_count, 1u) >= 256u)) {
+			atomicOr(&R->pal_over, 1u);
+			break;
+		}
+	}
+}
+
+__global__ void opt_clear_kernel(uint4 *p, size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Which of the optimiser's inputs the device forms for this prepared
+// context (IWX_SCAN_HAVE_*); 0: the host scans.  Negated output is scanned
+// by the host after the negation (src/imagew-main.c:1439-1440).
+unsigned int opt_scan_want(const iw_context *ctx)
+{
+	const iw_image &im = ctx->img2;
+	if (im.sampletype != IW_SAMPLETYPE_UINT || ctx->reduced_output_maxcolor_flag || ctx->req.negate_target)
+		return 0;
+	if (im.bit_depth != 8 && im.bit_depth != 16) return 0;
+	const int t = im.imgtype;
+	if (t != IW_IMGTYPE_GRAY && t != IW_IMGTYPE_GRAYA && t != IW_IMGTYPE_RGB && t != IW_IMGTYPE_RGBA) return 0;
+	if (im.bpr != (size_t)im.width * iw_imgtype_num_channels(t) * (im.bit_depth / 8)) return 0;
+	if ((size_t)im.width * im.height == 0) return 0;
+	unsigned int w = IWX_SCAN_HAVE_FLAGS;
+	if (IW_IMGTYPE_HAS_ALPHA(t) && (ctx->output_profile & IW_PROFILE_BINARYTRNS) && ctx->opt_binary_trns)
+		w |= IWX_SCAN_HAVE_KEYS;
+	if (ctx->output_profile & (IW_PROFILE_PAL1 | IW_PROFILE_PAL2 | IW_PROFILE_PAL4 | IW_PROFILE_PAL8 |
+			IW_PROFILE_GRAY1 | IW_PROFILE_GRAY2 | IW_PROFILE_GRAY4))
+		w |= IWX_SCAN_HAVE_PAL;
+	return w;
+}
+
+OptJob opt_job(const iw_context *ctx, void *px, iwx_opt_scan *res, unsigned int want)
+{
+	OptJob J;
+	J.px = (unsigned char *)px;
+	J.res = res;
+	J.npx = (unsigned long long)ctx->img2.width * ctx->img2.height;
+	J.chunk0 = 0;
+	J.nc = iw_imgtype_num_channels(ctx->img2.imgtype);
+	J.b16 = ctx->img2.bit_depth == 16;
+	J.want = want;
+	return J;
+}
+
+// Bytes of device memory for the scans of n images: descriptors, then one
+// result per image.
+size_t opt_scan_bytes(size_t n)
+{
+	return ((n * sizeof(OptJob) + 255) & ~(size_t)255) + n * sizeof(iwx_opt_scan);
+}
+
+// Clears n results at `region` + descriptors' size and launches the scans
+// of `jobs` (each one's res in that region) on st.  `jobs` must live until
+// the stream has passed the launch.
+bool opt_scan_enqueue(std::vector<OptJob> &jobs, void *region, size_t n, hipStream_t st)
+{
+	OptJob *dj = (OptJob *)region;
+	iwx_opt_scan *res = (iwx_opt_scan *)((char *)region + ((n * sizeof(OptJob) + 255) & ~(size_t)255));
+	unsigned long long chunks = 0;
+	for (auto &J : jobs) {
+		J.chunk0 = (unsigned int)chunks;
+		chunks += (J.npx + OPT_CHUNK - 1) / OPT_CHUNK;
+		if (J.res < res || J.res >= res + n) return false;
+	}
+	if (chunks >= 0x7fffffffull) return false;
+	const size_t n16 = n * sizeof(iwx_opt_scan) / 16;
+	if (n16) {
+		const unsigned int g = (unsigned int)std::min<size_t>(1024, (n16 + 255) / 256);
+		opt_clear_kernel<<<g, 256, 0, st>>>((uint4 *)res, n16);
+	}
+	if (!jobs.empty() && chunks) {
+		if (hipMemcpyAsync(dj, jobs.data(), jobs.size() * sizeof(OptJob), hipMemcpyHostToDevice, st) != hipSuccess)
+			return false;
+		opt_scan_kernel<<<(unsigned int)chunks, OPT_THREADS, 0, st>>>(dj, (int)jobs.size());
+	}
+	return hipGetLastError() == hipSuccess;
+}
+
 }  // namespace
 
 // ---- a device-resident job --------------------------------------------------
@@ -3215,18 +3531,34 @@ extern "C" double iwamd_job_flops(struct iwamd_job *J)
 extern "C" int iwamd_job_finish(struct iwamd_job *J)
 {
 	std::vector<uint8_t> host(J->out_bytes + 1, 0);
-	if (!J->skip_pixels) {
-		if (hipMemcpyAsync(host.data(), (char *)J->block + J->out_off, J->out_bytes, hipMemcpyDeviceToHost,
-				J->stream) != hipSuccess || !iwamd_job_sync(J)) {
-			iw_set_error(J->ctx, "Device-to-host copy failed");
-			return 0;
-		}
+	// The optimiser's scans run on the output before it is copied back.
+	const unsigned int want = J->skip_pixels ? 0 : opt_scan_want(J->ctx);
+	const size_t sbytes = opt_scan_bytes(1);
+	void *sblk = want ? dev_alloc(sbytes) : nullptr;
+	std::vector<OptJob> scans;
+	std::unique_ptr<iwx_opt_scan> scan(want ? new iwx_opt_scan() : nullptr);
+	bool ok = !want || sblk;
+	if (ok && want) {
+		iwx_opt_scan *res = (iwx_opt_scan *)((char *)sblk + opt_scan_bytes(1) - sizeof(iwx_opt_scan));
+		scans.push_back(opt_job(J->ctx, (char *)J->block + J->out_off, res, want));
+		ok = opt_scan_enqueue(scans, sblk, 1, J->stream) &&
+			hipMemcpyAsync(scan.get(), res, sizeof(iwx_opt_scan), hipMemcpyDeviceToHost, J->stream) == hipSuccess;
 	}
+	if (ok && !J->skip_pixels)
+		ok = hipMemcpyAsync(host.data(), (char *)J->block + J->out_off, J->out_bytes, hipMemcpyDeviceToHost,
+			J->stream) == hipSuccess;
+	ok = ok && iwamd_job_sync(J);
+	if (sblk) dev_release(sblk, sbytes);
+	if (!ok) {
+		iw_set_error(J->ctx, "Device-to-host copy failed");
+		return 0;
+	}
+	if (scan) scan->valid = want;
 	double ms[3];
 	int k = iwamd_job_kernel_ms(J, ms, 3);
 	J->ctx->num_last_timings = k;
 	for (int i = 0; i < k; i++) J->ctx->last_timings_ms[i] = ms[i];
-	return iwamd_finish(J->ctx, host.data());
+	return iwx_finish_image(J->ctx, host.data(), scan.get());
 }
 
 extern "C" void iwamd_job_destroy(struct iwamd_job *J)
@@ -3702,22 +4034,56 @@ extern "C" int iwamd_batch_finish(struct iwamd_batch *B)
 {
 	if (!B->ran && !iwamd_batch_run(B)) return 0;
 	std::vector<uint8_t> host(B->out_total + 1, 0);
-	if (hipMemcpyAsync(host.data(), (char *)B->block + B->out_base, B->out_total, hipMemcpyDeviceToHost,
-			B->stream) != hipSuccess || !iwamd_batch_sync(B)) {
+	// The optimiser's scans of every output, one launch, before the copy.
+	const size_t n = B->jobs.size();
+	const size_t sbytes = opt_scan_bytes(n);
+	std::vector<unsigned int> want(n, 0);
+	std::vector<OptJob> scans;
+	void *sblk = nullptr;
+	for (size_t k = 0; k < n; k++)
+		want[k] = B->jobs[k]->skip_pixels ? 0 : opt_scan_want(B->jobs[k]->ctx);
+	bool ok = true;
+	std::vector<iwx_opt_scan> res;
+	if (std::any_of(want.begin(), want.end(), [](unsigned int w) { return w != 0; })) {
+		sblk = dev_alloc(sbytes);
+		ok = sblk != nullptr;
+		if (ok) {
+			iwx_opt_scan *dres = (iwx_opt_scan *)((char *)sblk + sbytes - n * sizeof(iwx_opt_scan));
+			for (size_t k = 0; k < n; k++)
+				if (want[k])
+					scans.push_back(opt_job(B->jobs[k]->ctx, (char *)B->block + B->out_base + B->jobs[k]->out_off,
+						dres + k, want[k]));
+			res.resize(n);
+			ok = opt_scan_enqueue(scans, sblk, n, B->stream) &&
+				hipMemcpyAsync(res.data(), dres, n * sizeof(iwx_opt_scan), hipMemcpyDeviceToHost, B->stream) ==
+					hipSuccess;
+		}
+	}
+	ok = ok && hipMemcpyAsync(host.data(), (char *)B->block + B->out_base, B->out_total, hipMemcpyDeviceToHost,
+		B->stream) == hipSuccess;
+	ok = ok && iwamd_batch_sync(B);
+	if (sblk) dev_release(sblk, sbytes);
+	if (!ok) {
 		for (auto *J : B->jobs) iw_set_error(J->ctx, "Device-to-host copy failed");
 		return 0;
 	}
-	int ok = 1;
+	int all = 1;
 	std::vector<uint8_t> zeros;
-	for (auto *J : B->jobs) {
+	for (size_t k = 0; k < n; k++) {
+		auto *J = B->jobs[k];
 		const uint8_t *img2 = host.data() + J->out_off;
 		if (J->skip_pixels) {
 			zeros.assign(J->plan.out_bpr * (size_t)J->plan.out_height + 1, 0);
 			img2 = zeros.data();
 		}
-		ok &= iwamd_finish(J->ctx, img2);
+		const iwx_opt_scan *scan = nullptr;
+		if (want[k]) {
+			res[k].valid = want[k];
+			scan = &res[k];
+		}
+		all &= iwx_finish_image(J->ctx, img2, scan);
 	}
-	return ok;
+	return all;
 }
 
 extern "C" int iwamd_batch_kernel_ms(struct iwamd_batch *B, double *ms, int n)
@@ -3781,6 +4147,8 @@ struct SrvReq {
 	Staged S;
 	size_t off = 0, out_off = 0;
 	int widx = 0;
+	int k = 0;                  // its place in the flight
+	unsigned int scan_want = 0; // optimiser scans done on the device
 	bool skip = false;
 };
 
@@ -3788,9 +4156,11 @@ struct SrvFlight {
 	WorkSet W;
 	void *block = nullptr;
 	size_t bytes = 0, out_base = 0, out_total = 0;
-	uint8_t *host = nullptr;  // pinned, coherent: outputs, then one flag per request
+	uint8_t *host = nullptr;  // pinned, coherent: outputs, one flag per request, scan results
 	size_t host_bytes = 0;
 	int *flags = nullptr;
+	size_t scan_dev = 0, scan_host = 0;  // scan region in the block; results in `host`
+	std::vector<OptJob> scans;           // descriptors, kept until the flight ends
 	hipEvent_t ev = nullptr;
 };
 
@@ -3885,9 +4255,12 @@ bool ServerExec::launch(Loop::Flight &F, int slot)
 	}
 	D.out_total = arena;
 	ws_layout(D.W);
-	D.bytes = off + arena + D.W.bytes;
+	const size_t n = F.reqs.size();
+	D.scan_dev = (off + arena + D.W.bytes + 255) & ~(size_t)255;
+	D.bytes = D.scan_dev + opt_scan_bytes(n);
 	D.block = dev_alloc(D.bytes);
-	const size_t host_bytes = arena + sizeof(int) * F.reqs.size() + 16;
+	D.scan_host = (arena + sizeof(int) * n + 255) & ~(size_t)255;
+	const size_t host_bytes = D.scan_host + n * sizeof(iwx_opt_scan);
 	bool ok = D.block != nullptr;
 	if (ok) {
 		D.host = pool.get(host_bytes, D.host_bytes);
@@ -3903,6 +4276,8 @@ bool ServerExec::launch(Loop::Flight &F, int slot)
 		char *base = (char *)D.block;
 		for (size_t k = 0; k < F.reqs.size(); k++) {
 			SrvReq &R = F.reqs[k]->data;
+			R.k = (int)k;
+			R.scan_want = 0;
 			DevPlan &P = D.W.plans[R.widx];
 			fill_dev_plan(R.S, R.plan, base + R.off, P);
 			P.out = (uint8_t *)(base + D.out_base + R.out_off);
@@ -3921,6 +4296,23 @@ bool ServerExec::launch(Loop::Flight &F, int slot)
 	if (ok) {
 		ws_enqueue_resample(D.W, st, true);
 		ws_enqueue_errdiff(D.W, st);
+		// The optimiser's scans of the outputs that complete with the
+		// flight (one that completes on its own has been copied already).
+		char *sreg = (char *)D.block + D.scan_dev;
+		iwx_opt_scan *dres = (iwx_opt_scan *)(sreg + opt_scan_bytes(n) - n * sizeof(iwx_opt_scan));
+		D.scans.clear();
+		for (auto *R : F.reqs) {
+			if (R->flag >= 0 || R->data.skip) continue;
+			R->data.scan_want = opt_scan_want(R->data.ctx);
+			if (R->data.scan_want)
+				D.scans.push_back(opt_job(R->data.ctx, (char *)D.block + D.out_base + R->data.out_off,
+					dres + R->data.k, R->data.scan_want));
+		}
+		if (!D.scans.empty()) {
+			ok &= opt_scan_enqueue(D.scans, sreg, n, st);
+			ok &= hipMemcpyAsync(D.host + D.scan_host, dres, n * sizeof(iwx_opt_scan), hipMemcpyDeviceToHost, st) ==
+				hipSuccess;
+		}
 		// Requests without error diffusion complete with the flight.
 		for (auto *R : F.reqs)
 			if (R->flag < 0 && !R->data.skip)
@@ -3945,7 +4337,14 @@ bool ServerExec::finish(Loop::Flight &F, Loop::Req &R, bool failed)
 		zeros.assign(R.data.plan.out_bpr * (size_t)R.data.plan.out_height + 1, 0);
 		img2 = zeros.data();
 	}
-	return iwamd_finish(R.data.ctx, img2) != 0;
+	std::unique_ptr<iwx_opt_scan> scan;
+	if (R.data.scan_want) {
+		scan.reset(new iwx_opt_scan());
+		memcpy(scan.get(), F.data.host + F.data.scan_host + (size_t)R.data.k * sizeof(iwx_opt_scan),
+			sizeof(iwx_opt_scan));
+		scan->valid = R.data.scan_want;
+	}
+	return iwx_finish_image(R.data.ctx, img2, scan.get()) != 0;
 }
 
 void ServerExec::release(Loop::Flight &F)

@@ -697,7 +697,9 @@ extern "C" int iwamd_prepare(struct iw_context *ctx, struct iwamd_plan *plan)
 	return 1;
 }
 
-extern "C" int iwamd_finish(struct iw_context *ctx, const uint8_t *img2)
+// Completes a prepared context from its output image, with the device's
+// optimiser scan when there is one.
+int iwx_finish_image(struct iw_context *ctx, const uint8_t *img2, const iwx_opt_scan *scan)
 {
 	if (!ctx->prepared) {
 		iw_set_error(ctx, "Internal: finish without prepare");
@@ -707,7 +709,12 @@ extern "C" int iwamd_finish(struct iw_context *ctx, const uint8_t *img2)
 	if (!ctx->img2.pixels) return 0;
 	memcpy(ctx->img2.pixels, img2, ctx->img2.bpr * ctx->img2.height);
 	ctx->prepared = 0;
-	return iwx_finish(ctx);
+	return iwx_finish(ctx, scan);
+}
+
+extern "C" int iwamd_finish(struct iw_context *ctx, const uint8_t *img2)
+{
+	return iwx_finish_image(ctx, img2, nullptr);
 }
 
 // prepare -> HIP kernels -> finish (src/imagew-main.c:2250-2272).
