@@ -129,11 +129,17 @@ struct DevPlan {
 	int v_shared;  // every channel has the same vertical weights (dense form) in WEIGHTS mode
 	int h_shared;  // the same for the horizontal weights
 	double bk1a, bk2a;
+	// Chosen-precision form: the shared horizontal list by 4-column blocks.
+	// Output column i: (first block, block count, offset) in h4i[i]; block b
+	// of it (columns 4b..4b+3) is h4w[4*(offset + b - first) + 0..3];
+	// h4w[0..3] is a zero block.
+	const int4 *h4i;
+	const float *h4w;
 };
 
 // ---- input sampling (src/imagew-main.c:129-276) ---------------------------
 
-__device__ __forceinline__ void logical_to_physical(const DevPlan &P, int x, int y, int &rx, int &ry)
+__host__ __device__ inline void logical_to_physical(const DevPlan &P, int x, int y, int &rx, int &ry)
 {
 	const int X = P.sx + x, Y = P.sy + y;
 	switch (P.orient) {
@@ -470,7 +476,7 @@ struct EmitSimple {
 	const unsigned short *six[IWAMD_MAX_CH];
 };
 
-__device__ __forceinline__ bool emit_simple_ctx(const DevPlan &P, EmitSimple &E)
+__host__ __device__ inline bool emit_simple_ctx(const DevPlan &P, EmitSimple &E)
 {
 	if (P.out_float || P.checker || (P.apply_bkgd && P.late) || P.bk_has_trans) return false;
 	E.nch = P.inter_nch;
@@ -752,7 +758,7 @@ struct FastCin {
 	int lean;                // no channel in mode 2 and none with a transparent edge
 };
 
-__device__ __forceinline__ int fast_cin_depth(const DevPlan &P, FastCin &F)
+__host__ __device__ inline int fast_cin_depth(const DevPlan &P, FastCin &F)
 {
 	if (P.in_float || (P.bit_depth != 8 && P.bit_depth != 16)) return 0;
 	const int bps = P.bit_depth / 8, pb = P.nch_phys * bps;
@@ -1076,64 +1082,81 @@ __global__ void reset_counters_kernel(int *c, int n)
 	if (k < n) c[k] = 0;
 }
 
-// ---- chosen-precision form: FP16 MFMA, FP32 accumulation ------------------
-// A work item is a tile of FAST_ROWS output rows x up to FAST_MAX_TI output
-// columns.  Both passes are small dense matrix products on
-// v_mfma_f32_16x16x16f16:
-//   vertical:   inter[r][x] = sum_k Wv[r][k] * in[k][x]   (k: the band's input
-//               rows, x: the tile's intermediate columns)
-//   horizontal: out[r][i]   = sum_x inter[r][x] * Wh[x][i]
-// with the band's weights zero outside each row's list.  Inputs, weights and
-// the intermediate are FP16, sums FP32; everything after the resampled value
-// (unpremultiply, background, the linear-to-output transfer, quantisation
-// and dithering) is the plain form's FP64 code.  A tile's sums depend only on
-// its own plan (band and tile geometry come from the plan), never on the
-// batch around it.
+// ---- chosen-precision form: FP32 MFMA, intermediate in registers ---------
+//
+// A band is FAST_ROWS (16) output rows; a tile is a band's rows x up to
+// 16 * FAST_G output columns (one or two output groups of 16), and one wave
+// takes a tile from input bytes to output bytes.  Both passes are dense
+// products on v_mfma_f32_16x16x4f32 (FP32 operands and sums) with the band's
+// weights zero outside each row's list:
+//   vertical, transposed:  interT[x][r] = sum_k in[k][x] * Wv[r][k]
+//     x: the 16 intermediate columns of a 16-aligned group; k: the band's
+//     input rows, four per step;
+//   horizontal:            out[r][i] = sum_x inter[r][x] * Wh[x][i]
+//     i: the 16 output columns of an output group.
+// Operand layout of v_mfma_f32_16x16x4f32 for lane l: A[l%16][l/16],
+// B[l/16][l%16], D[4(l/16)+t][l%16] (t = 0..3).  The vertical product leaves
+// lane l holding inter[r = l%16][x = xg + 4(l/16) + t] in D[t]: the A operand
+// of the horizontal product's step t when that step sums the positions
+// x = xg + 4k + t (k = 0..3 over the lanes' quarters).  So each intermediate
+// group goes from the vertical accumulators into four horizontal steps
+// without leaving the registers, and the horizontal accumulators leave lane l
+// holding every channel of the pixels (row 4(l/16)+t, column i0 + l%16),
+// which the per-pixel work takes from there.
+//
+// The grouping of the sums depends on the plan alone: intermediate groups
+// are 16-aligned in absolute columns, an output group's horizontal steps run
+// over the groups its own columns' lists reach (the same for both tile
+// widths), and the vertical steps start at the band's first input row.  So a
+// job's bytes depend neither on the tile width nor on the batch around it.
+//
+// A sample is its input table's value and premultiply (FP64, or in FP32
+// through the band's FP32 copies of 8-bit tables) rounded to FP32; weights
+// are the FP64 lists rounded to FP32; the vertical sum is FP32 where the
+// reference rounds its intermediate to FP32.  Everything after the
+// horizontal sum -- unpremultiply, background, the linear-to-output transfer,
+// quantisation, dithering and error diffusion -- is the plain form's FP64
+// code.
 
 #define FAST_ROWS 16
-#define FAST_MAX_KV 256   // input rows of a band
-#define FAST_MAX_XT 160   // intermediate columns of a tile
-#define FAST_MAX_TI 32    // output columns of a tile
-// Tiles whose band and span fit these run in a kernel with a third of the
-// LDS, four workgroups per CU instead of two.
-#define FAST_S_KV 96
-#define FAST_S_XT 64
+#define FAST_MAX_KV 256   // input rows of a band (its weights live in LDS)
+#define FAST_G 2          // output groups of 16 columns per tile, at most
+#define FAST_MAX_TI (16 * FAST_G)
+#define FAST_KC 8         // vertical steps whose input loads are issued together
+#define FAST_RUN 8        // tiles per work item, at most
+#define FAST_WG_PER_CU 4  // workgroups per CU of a persistent launch
 
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+// Kernel variants, by what a plan's tiles need (fast_variant): 8-bit input
+// through the LDS tables with the lean colour-in and the common per-pixel
+// work; 16-bit input likewise; anything else.
+enum { FAST_LEAN8 = 0, FAST_LEAN16 = 1, FAST_GENERIC = 2, FAST_NVAR = 3 };
+
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 struct FastItem {
-	int job, j0, i0, ni;  // plan; first output row; first output column, count
-	int x_lo, nx;         // the tile's intermediate columns
+	int job, j0, i0, ni;  // plan; first output row; first output column, count (<= FAST_MAX_TI)
+	int x_lo, nx;         // intermediate columns the tile reads: 16-aligned start, count (multiple of 16)
 	int p_lo, kv;         // the band's input rows
 };
 
-// Operands are split: v = hi + lo/2048 with hi = fp16(v) and lo =
-// fp16((v - hi)*2048), so a product is hi*hi + (hi*lo + lo*hi)/2048 to about
-// 22 bits -- three MFMAs, the two cross terms in their own FP32 sum.  (A
-// single FP16 product carries 11 bits, which the division by a small
-// resampled alpha would magnify past one output level.)  Intermediate rows
-// carry 16 zero columns past the tile's last 16-column group, so the
-// horizontal pass reads four-column groups without bounds checks.  The band
-// weights stay for all of a band's tiles.
-template <int KV, int XT>
+// Per workgroup: the band's vertical weights by input row (so a step's four
+// rows x 16 output rows are 64 consecutive words) and FP32 copies of the
+// 8-bit input tables.
 struct __attribute__((aligned(16))) FastLds {
-	static constexpr int kKV = KV, kXT = XT;
-	_Float16 wv[2][FAST_ROWS][KV];                        // hi, lo
-	float hout[IWAMD_MAX_CH][FAST_ROWS][FAST_MAX_TI];
-	_Float16 inter[2][IWAMD_MAX_CH][FAST_ROWS][XT + 16];  // hi, lo
-	float lut[IWAMD_MAX_CH + 1][256];  // 8-bit input tables, FP32: channels, then alpha
+	float wv[FAST_MAX_KV][FAST_ROWS];
+	float lut[IWAMD_MAX_CH + 1][256];  // channels, then alpha
 };
 
-// The 32-bit word of a 4-byte pixel (fast_fetch_lds<32>'s load): logical
-// row p of the column at byte offset cb, clamped to the image.
+// The 32-bit word of a 4-byte pixel: logical row p of the column at byte
+// offset cb, clamped to the image.
 __device__ __forceinline__ unsigned int fast_word(const DevPlan &P, const FastCin &F, long long cb, int p)
 {
 	const int pin = p < 0 ? 0 : (p >= P.ih ? P.ih - 1 : p);
 	return *(const __attribute__((address_space(1))) unsigned int *)((const g_u8 *)P.in + cb + (long long)pin * F.dy);
 }
 
-// fast_fetch_lds<32>'s conversion of a loaded word.
+// A loaded RGBA8-style word through the LDS tables (LEAN plans: a channel is
+// its table value times alpha, or times 1).
 __device__ __forceinline__ void fast_decode_word(const FastCin &F, const float (*lut)[256], unsigned int word,
 	float v[IWAMD_MAX_CH])
 {
@@ -1142,311 +1165,275 @@ __device__ __forceinline__ void fast_decode_word(const FastCin &F, const float (
 	for (int c = 0; c < IWAMD_MAX_CH; c++) v[c] = lut[c][(word >> (8 * F.chb[c])) & 0xffu] * (F.mode[c] ? a : 1.0f);
 }
 
-// fast_fetch for LEAN plans of 8-bit input through the band's FP32 copies of
-// the input tables in LDS (lut[c], alpha last): the chosen form rounds the
-// sample to FP32 anyway, so the table values and the premultiply are taken in
-// FP32 (within 1.5 FP32 ulps of the FP64 route).
-template <int DEPTH>
-__device__ __forceinline__ void fast_fetch_lds(const DevPlan &P, const FastCin &F, const float (*lut)[256],
+// fast_fetch for LEAN plans of 8-bit input through the LDS tables.
+__device__ __forceinline__ void fast_fetch_lds8(const DevPlan &P, const FastCin &F, const float (*lut)[256],
 	long long cb, int p, float v[IWAMD_MAX_CH])
 {
 	const int pin = p < 0 ? 0 : (p >= P.ih ? P.ih - 1 : p);
 	const long long off = cb + (long long)pin * F.dy;
 	const g_u8 *in = (const g_u8 *)P.in;
-	unsigned int raw[IWAMD_MAX_CH], araw;
-	if (DEPTH == 32) {
-		const unsigned int word = *(const __attribute__((address_space(1))) unsigned int *)(in + off);
+	unsigned int raw[IWAMD_MAX_CH];
 #pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = (word >> (8 * F.chb[c])) & 0xffu;
-		araw = (word >> (8 * F.ab)) & 0xffu;
-	} else {
-#pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = in[off + F.chb[c]];
-		araw = in[off + F.ab];
-	}
-	const float a = lut[IWAMD_MAX_CH][araw];
+	for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = in[off + F.chb[c]];
+	const float a = lut[IWAMD_MAX_CH][in[off + F.ab]];
 #pragma unroll
 	for (int c = 0; c < IWAMD_MAX_CH; c++) v[c] = lut[c][raw[c]] * (F.mode[c] ? a : 1.0f);
 }
 
-__device__ __forceinline__ void split16(float v, _Float16 &hi, _Float16 &lo)
+__device__ __forceinline__ float4_t mfma4(float a, float b, float4_t c)
 {
-	hi = (_Float16)v;
-	lo = (_Float16)((v - (float)hi) * 2048.0f);
+	return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c)
-{
-	return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
+// Floor division by 16 (columns can be negative: virtual pixels).
+__device__ __forceinline__ int fast_grp(int x) { return x >> 4; }
 
-// Vertical pass of a tile: 16-column groups of the intermediate, one per wave
-// in turn; each lane fetches its own B operands (rows kk + 4*lq .. +3 of its
-// column), so no sample is read twice within a group.
-template <int DEPTH, bool LEAN, class Lds>
-__device__ __forceinline__ void fast_vertical(const DevPlan &P, const FastItem &w, Lds &S, const FastCin &F)
+// One tile on one wave.  rows: the band's output rows that exist.  SIMPLE:
+// the plan takes the common per-pixel work (EmitSimple) -- known, so the
+// general per-pixel code is left out; else simple says which applies.
+template <int DEPTH, bool LEAN, bool SIMPLE>
+__device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, const FastItem &w, const FastLds &S,
+	const EmitSimple &ES, bool simple, int rows)
 {
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
 	const int C = P.inter_nch;
-	const int lr = lane & 15, lq = lane >> 4;
-	const int kv_pad = (w.kv + 15) & ~15;
-	const float inv = 1.0f / 2048.0f;
-	for (int ct = wave; ct * 16 < w.nx; ct += 4) {
-		const int col = ct * 16 + lr;
-		const int x = w.x_lo + col;
-		const bool outside = x < 0 || x >= P.W;
-		const bool live = col < w.nx;
+	const int kv4 = (w.kv + 3) & ~3;
+	const int ng = (w.ni + 15) >> 4;
+	const DevWeights &dh = P.ic[0].h;  // every channel shares the lists
+	const g_cint *hfirst = (const g_cint *)dh.dfirst, *hcount = (const g_cint *)dh.dcount;
+	const int4 *h4i = P.h4i;
+	const float4_t *h4w = (const float4_t *)P.h4w;
+
+	// Each output group: the lane's column's weights by 4-column blocks
+	// (first block, block count, offset in h4w) and the intermediate groups
+	// that the group's lists reach.
+	int4 hb[FAST_G];
+	int ga[FAST_G], gb[FAST_G];
+	int g_first = 0x7fffffff, g_last = -0x7fffffff;
+#pragma unroll
+	for (int og = 0; og < FAST_G; og++) {
+		const int ii = 16 * og + lr;
+		int f = 0x7fffffff, e = -0x7fffffff;
+		hb[og] = make_int4(0, 0, 0, 0);
+		if (og < ng && ii < w.ni) {
+			const int i = w.i0 + ii;
+			hb[og] = h4i[i];
+			const int n = hcount[i];
+			if (n > 0) {
+				f = hfirst[i];
+				e = f + n - 1;
+			}
+		}
+		for (int m = 1; m < 16; m <<= 1) {
+			f = min(f, __shfl_xor(f, m));
+			e = max(e, __shfl_xor(e, m));
+		}
+		ga[og] = e >= f ? fast_grp(f) : 0x7fffffff;
+		gb[og] = e >= f ? fast_grp(e) : -0x7fffffff;
+		if (og < ng && e >= f) {
+			g_first = min(g_first, ga[og]);
+			g_last = max(g_last, gb[og]);
+		}
+	}
+
+	float4_t acc[FAST_G][IWAMD_MAX_CH];
+#pragma unroll
+	for (int og = 0; og < FAST_G; og++)
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) acc[og][c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+
+	for (int g = g_first; g <= g_last; g++) {
+		const int xg = 16 * g;
+		// This group's horizontal weights, loaded before the vertical steps.
+		float4_t hw[FAST_G];
+#pragma unroll
+		for (int og = 0; og < FAST_G; og++) {
+			const int rel = 4 * g + lq - hb[og].x;
+			hw[og] = h4w[(rel >= 0 && rel < hb[og].y) ? hb[og].z + rel : 0];
+		}
+		const int x = xg + lr;
 		const int xc = x < 0 ? 0 : (x >= P.W ? P.W - 1 : x);
 		const long long cb = DEPTH ? F.base + (long long)xc * F.dx : (long long)xc;
-		float4_t acc[IWAMD_MAX_CH], acl[IWAMD_MAX_CH];
+		float4_t dv[IWAMD_MAX_CH];
 #pragma unroll
-		for (int c = 0; c < IWAMD_MAX_CH; c++) {
-			acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-			acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-		}
-		// 4-byte pixels through the LDS tables: the next K step's words are
-		// loaded while this step's are converted and multiplied.
-		constexpr bool WORDS = LEAN && DEPTH == 32;
-		unsigned int wcur[4], wnxt[4];
-		if constexpr (WORDS) {
+		for (int c = 0; c < IWAMD_MAX_CH; c++) dv[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+		if constexpr (DEPTH == 32 && LEAN) {
+			// FAST_KC steps' words are loaded before the first is used.
+			for (int k0 = 0; k0 < kv4; k0 += 4 * FAST_KC) {
+				unsigned int wd[FAST_KC];
 #pragma unroll
-			for (int t = 0; t < 4; t++) wcur[t] = fast_word(P, F, cb, w.p_lo + 4 * lq + t);
-		}
-		for (int kk = 0; kk < kv_pad; kk += 16) {
-			const half4_t ah = *(const half4_t *)&S.wv[0][lr][kk + 4 * lq];
-			const half4_t al = *(const half4_t *)&S.wv[1][lr][kk + 4 * lq];
-			float v[4][IWAMD_MAX_CH];
-			if constexpr (WORDS) {
-				if (kk + 16 < kv_pad) {
+				for (int s = 0; s < FAST_KC; s++) wd[s] = fast_word(P, F, cb, w.p_lo + k0 + 4 * s + lq);
 #pragma unroll
-					for (int t = 0; t < 4; t++) wnxt[t] = fast_word(P, F, cb, w.p_lo + kk + 16 + 4 * lq + t);
-				}
+				for (int s = 0; s < FAST_KC; s++) {
+					if (k0 + 4 * s >= kv4) break;
+					float v[IWAMD_MAX_CH];
+					fast_decode_word(F, S.lut, wd[s], v);
+					const float b = S.wv[k0 + 4 * s + lq][lr];
 #pragma unroll
-				for (int t = 0; t < 4; t++) fast_decode_word(F, S.lut, wcur[t], v[t]);
-			} else {
-#pragma unroll
-				for (int t = 0; t < 4; t++) {
-					if constexpr (LEAN && DEPTH != 16 && DEPTH != 0)
-						fast_fetch_lds<DEPTH>(P, F, S.lut, cb, w.p_lo + kk + 4 * lq + t, v[t]);
-					else
-						fast_fetch<DEPTH, LEAN>(P, F, cb, w.p_lo + kk + 4 * lq + t, v[t]);
+					for (int c = 0; c < IWAMD_MAX_CH; c++)
+						if (c < C) dv[c] = mfma4(v[c], b, dv[c]);
 				}
 			}
-			half4_t bh[IWAMD_MAX_CH], bl[IWAMD_MAX_CH];
+		} else {
+			for (int k = 0; k < kv4; k += 4) {
+				float v[IWAMD_MAX_CH];
+				if constexpr (LEAN && DEPTH == 8)
+					fast_fetch_lds8(P, F, S.lut, cb, w.p_lo + k + lq, v);
+				else
+					fast_fetch<DEPTH, LEAN>(P, F, cb, w.p_lo + k + lq, v);
+				const float b = S.wv[k + lq][lr];
 #pragma unroll
-			for (int t = 0; t < 4; t++) {
-				const bool in = live && kk + 4 * lq + t < w.kv;
-#pragma unroll
-				for (int c = 0; c < IWAMD_MAX_CH; c++) {
-					_Float16 h, l;
-					split16(in ? v[t][c] : 0.0f, h, l);
-					bh[c][t] = h;
-					bl[c][t] = l;
-				}
-			}
-#pragma unroll
-			for (int c = 0; c < IWAMD_MAX_CH; c++) {
-				if (c >= C) continue;
-				acc[c] = mfma16(ah, bh[c], acc[c]);
-				acl[c] = mfma16(ah, bl[c], acl[c]);
-				acl[c] = mfma16(al, bh[c], acl[c]);
-			}
-			if constexpr (WORDS) {
-#pragma unroll
-				for (int t = 0; t < 4; t++) wcur[t] = wnxt[t];
+				for (int c = 0; c < IWAMD_MAX_CH; c++)
+					if (c < C) dv[c] = mfma4(v[c], b, dv[c]);
 			}
 		}
+		// The intermediate: a virtual column outside a TRANSPARENT edge is the
+		// edge value; intclamp clamps.
 #pragma unroll
 		for (int c = 0; c < IWAMD_MAX_CH; c++) {
 			if (c >= C) continue;
 #pragma unroll
 			for (int t = 0; t < 4; t++) {
-				float v = acc[c][t] + acl[c][t] * inv;
-				// A virtual column outside a TRANSPARENT edge is the edge value.
-				if (outside && P.ic[c].h.edge_transparent) v = (float)P.ic[c].h.edge;
+				float v = dv[c][t];
+				const int xt = xg + 4 * lq + t;
+				if ((xt < 0 || xt >= P.W) && P.ic[c].h.edge_transparent) v = (float)P.ic[c].h.edge;
 				if (P.intclamp) v = fminf(fmaxf(v, 0.0f), 1.0f);
-				if (!live) v = 0.0f;
-				split16(v, S.inter[0][c][4 * lq + t][col], S.inter[1][c][4 * lq + t][col]);
+				dv[c][t] = v;
+			}
+		}
+#pragma unroll
+		for (int og = 0; og < FAST_G; og++) {
+			if (og >= ng || g < ga[og] || g > gb[og]) continue;
+#pragma unroll
+			for (int t = 0; t < 4; t++)
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++)
+					if (c < C) acc[og][c] = mfma4(dv[c][t], hw[og][t], acc[og][c]);
+		}
+	}
+
+	// Per-pixel work in FP64, as in the plain form: one pixel per step, the
+	// step's sums picked out of the accumulators (og and t are uniform), so
+	// that the per-pixel code appears once.
+	auto pick = [&](int og, int t, int c) {
+		const float4_t &a = og ? acc[FAST_G - 1][c] : acc[0][c];
+		return t == 0 ? a[0] : (t == 1 ? a[1] : (t == 2 ? a[2] : a[3]));
+	};
+#pragma unroll 1
+	for (int e = 0; e < 4 * ng; e++) {
+		const int og = e >> 2, t = e & 3;
+		const int ii = 16 * og + lr, i = w.i0 + ii, r = 4 * lq + t;
+		const bool live = ii < w.ni && r < rows;
+		if (SIMPLE || simple) {
+			const int pi[1] = {i}, pj[1] = {w.j0 + r};
+			const bool lv[1] = {live};
+			// The flags are made opaque per step: otherwise every condition
+			// derived from them is computed once and kept live across the
+			// tile as a lane mask, which spills.
+			EmitSimple E = ES;
+#pragma unroll
+			for (int c = 0; c < IWAMD_MAX_CH; c++) {
+				asm volatile("" : "+s"(E.f[c]));
+				asm volatile("" : "+s"(E.n[c]));
+			}
+			emit_px_simple<1, true>(E, pi, pj, lv, [&](int, double (&gv)[IWAMD_MAX_CH]) {
+#pragma unroll
+				for (int c = 0; c < IWAMD_MAX_CH; c++) gv[c] = (double)pick(og, t, c);
+			});
+		} else if constexpr (!SIMPLE) {
+			if (live) {
+				EmitCtx E;
+				emit_ctx(P, E);
+				emit_px(E, i, w.j0 + r, [&](int c) {
+					return (double)(c == 0 ? pick(og, t, 0) : c == 1 ? pick(og, t, 1) : c == 2 ? pick(og, t, 2) : pick(og, t, 3));
+				});
 			}
 		}
 	}
 }
 
-// Operand layout of v_mfma_f32_16x16x16f16 for lane l: A[l%16][4*(l/16)+t],
-// B[4*(l/16)+t][l%16], D[4*(l/16)+t][l%16], t = 0..3.
-//
-// A work item is a band: the tiles tiles[first .. first+count) of one plan,
-// which share their 16 output rows and their band of input rows.  The band's
-// weights go to LDS once; then each tile runs the vertical pass, the
-// horizontal pass and the per-pixel work.
-template <class Lds>
+// A work item: tiles tiles[first .. first+count) of one band.  The band's
+// weights and the input tables go to LDS once; then the waves take the tiles
+// in turn.
+template <int VAR>
 __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__restrict__ tiles, int first, int count,
-	Lds &S)
+	FastLds &S)
 {
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const int C = P.inter_nch;
-	const int lr = lane & 15, lq = lane >> 4;
+	const int tid = threadIdx.x, wave = tid >> 6;
 	const FastItem w0 = tiles[first];
-	const int kv_pad = (w0.kv + 15) & ~15;
-	const DevWeights &dv = P.ic[0].v;  // every channel shares the lists
-	const DevWeights &dh = P.ic[0].h;
-	const float inv = 1.0f / 2048.0f;
-
-	// Vertical weights of the band, split.
+	const int kv4 = (w0.kv + 3) & ~3;
+	const DevWeights &dv = P.ic[0].v;
 	{
-		const FastItem &w = w0;
 		const g_cint *vfirst = (const g_cint *)dv.dfirst, *vcount = (const g_cint *)dv.dcount;
 		const g_cint *voff = (const g_cint *)dv.doff;
 		const g_cf64 *vw = (const g_cf64 *)dv.dw;
-		for (int t = tid; t < FAST_ROWS * kv_pad; t += blockDim.x) {
-			const int r = t / kv_pad, k = t % kv_pad;
+		for (int t = tid; t < kv4 * FAST_ROWS; t += blockDim.x) {
+			const int k = t >> 4, r = t & 15;
 			float v = 0.0f;
-			const int j = w.j0 + r;
-			if (j < P.Hp && k < w.kv) {
-				const int kk = w.p_lo + k - vfirst[j];
+			const int j = w0.j0 + r;
+			if (j < P.Hp && k < w0.kv) {
+				const int kk = w0.p_lo + k - vfirst[j];
 				if (kk >= 0 && kk < vcount[j]) v = (float)vw[voff[j] + kk];
 			}
-			split16(v, S.wv[0][r][k], S.wv[1][r][k]);
+			S.wv[k][r] = v;
 		}
 	}
-	{
-		// FP32 copies of the 8-bit input tables for the lean direct colour-in.
-		FastCin F0;
-		const int d0 = fast_cin_depth(P, F0);
-		if ((d0 == 8 || d0 == 32) && F0.lean)
-			for (int t = tid; t < 256; t += blockDim.x) {
+	FastCin F;
+	const int depth = fast_cin_depth(P, F);
+	if ((depth == 8 || depth == 32) && F.lean)
+		for (int t = tid; t < 256; t += blockDim.x) {
 #pragma unroll
-				for (int c = 0; c < IWAMD_MAX_CH; c++) S.lut[c][t] = (float)((const g_cf64 *)F0.lut[c])[t];
-				S.lut[IWAMD_MAX_CH][t] = (float)((const g_cf64 *)F0.alut)[t];
-			}
-	}
+			for (int c = 0; c < IWAMD_MAX_CH; c++) S.lut[c][t] = (float)((const g_cf64 *)F.lut[c])[t];
+			S.lut[IWAMD_MAX_CH][t] = (float)((const g_cf64 *)F.alut)[t];
+		}
+	__syncthreads();
+	EmitSimple ES;
+	const bool simple = emit_simple_ctx(P, ES);
 	const int rows = min(FAST_ROWS, P.Hp - w0.j0);
-	for (int tt = 0; tt < count; tt++) {
+	for (int tt = wave; tt < count; tt += (int)(blockDim.x >> 6)) {
 		const FastItem w = tiles[first + tt];
-		FastCin F;
-		const int depth = fast_cin_depth(P, F);
-		EmitSimple ES;
-		const bool simple = emit_simple_ctx(P, ES);
-		// The band's weights are in place, and the previous tile's horizontal
-		// pass and per-pixel work are done with the intermediate and the sums.
-		__syncthreads();
-		// Zero columns past the last 16-column group (read by the horizontal pass).
-		const int nx16 = (w.nx + 15) & ~15;
-		for (int t = tid; t < 2 * C * FAST_ROWS * 16; t += blockDim.x) {
-			const int col = t & 15, r = (t >> 4) & 15, hc = t >> 8;
-			S.inter[hc & 1][hc >> 1][r][nx16 + col] = (_Float16)0.0f;
-		}
-
-		switch (depth) {
-		case 32:
-			if (F.lean) fast_vertical<32, true, Lds>(P, w, S, F);
-			else fast_vertical<32, false, Lds>(P, w, S, F);
-			break;
-		case 8:
-			if (F.lean) fast_vertical<8, true, Lds>(P, w, S, F);
-			else fast_vertical<8, false, Lds>(P, w, S, F);
-			break;
-		case 16:
-			if (F.lean) fast_vertical<16, true, Lds>(P, w, S, F);
-			else fast_vertical<16, false, Lds>(P, w, S, F);
-			break;
-		default: fast_vertical<0, false, Lds>(P, w, S, F); break;
-		}
-		__syncthreads();
-
-		// Horizontal pass: 16-column output groups, one per wave.  K runs over
-		// intermediate positions from the group's first listed one rounded down to
-		// a multiple of 4 (x_lo is one, so the groups are four-aligned in LDS);
-		// positions a lane's list does not hold weigh 0.
-		const g_cint *hfirst = (const g_cint *)dh.dfirst, *hcount = (const g_cint *)dh.dcount;
-		const g_cint *hoff = (const g_cint *)dh.doff;
-		const g_cf64 *hw = (const g_cf64 *)dh.dw;
-		for (int g = wave; g * 16 < w.ni; g += 4) {
-			const int i = w.i0 + g * 16 + lr;
-			const bool live = (g * 16 + lr) < w.ni;
-			int f = 0x7fffffff, e = -0x7fffffff, lf = 0, ln = 0, lo = 0;
-			if (live) {
-				ln = hcount[i];
-				lf = hfirst[i];
-				lo = hoff[i];
-				if (ln > 0) {
-					f = lf;
-					e = lf + ln - 1;
-				}
-			}
-			for (int m = 1; m < 16; m <<= 1) {
-				f = min(f, __shfl_xor(f, m));
-				e = max(e, __shfl_xor(e, m));
-			}
-			float4_t acc[IWAMD_MAX_CH], acl[IWAMD_MAX_CH];
-#pragma unroll
-			for (int c = 0; c < IWAMD_MAX_CH; c++) {
-				acc[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-				acl[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-			}
-			if (e >= f) {
-				for (int x0 = (f & ~3) - w.x_lo; x0 <= e - w.x_lo; x0 += 16) {
-					half4_t bh, bl;
-#pragma unroll
-					for (int t = 0; t < 4; t++) {
-						const int kk = w.x_lo + x0 + 4 * lq + t - lf;
-						const float v = (kk >= 0 && kk < ln) ? (float)hw[lo + kk] : 0.0f;
-						_Float16 h, l;
-						split16(v, h, l);
-						bh[t] = h;
-						bl[t] = l;
-					}
-#pragma unroll
-					for (int c = 0; c < IWAMD_MAX_CH; c++) {
-						if (c >= C) continue;
-						const half4_t ah = *(const half4_t *)&S.inter[0][c][lr][x0 + 4 * lq];
-						const half4_t al = *(const half4_t *)&S.inter[1][c][lr][x0 + 4 * lq];
-						acc[c] = mfma16(ah, bh, acc[c]);
-						acl[c] = mfma16(ah, bl, acl[c]);
-						acl[c] = mfma16(al, bh, acl[c]);
-					}
-				}
-			}
-#pragma unroll
-			for (int c = 0; c < IWAMD_MAX_CH; c++)
-				if (c < C)
-#pragma unroll
-					for (int t = 0; t < 4; t++) S.hout[c][4 * lq + t][g * 16 + lr] = acc[c][t] + acl[c][t] * inv;
-		}
-		__syncthreads();
-
-		// Per-pixel work in FP64, as in the plain form.
-		const int npx = rows * w.ni;
-		if (simple) {
-			for (int t = tid; t < npx; t += blockDim.x) {
-				const int r[1] = {t / w.ni}, ii[1] = {t % w.ni};
-				const int pi[1] = {w.i0 + ii[0]}, pj[1] = {w.j0 + r[0]};
-				const bool live[1] = {true};
-				emit_px_simple<1, false>(ES, pi, pj, live, [&](int n, int c) { return (double)S.hout[c][r[n]][ii[n]]; });
-			}
+		if constexpr (VAR == FAST_LEAN8) {
+			if (depth == 32) fast_tile<32, true, true>(P, F, w, S, ES, true, rows);
+			else fast_tile<8, true, true>(P, F, w, S, ES, true, rows);
+		} else if constexpr (VAR == FAST_LEAN16) {
+			fast_tile<16, true, true>(P, F, w, S, ES, true, rows);
 		} else {
-			EmitCtx E;
-			emit_ctx(P, E);
-			for (int t = tid; t < npx; t += blockDim.x) {
-				const int r = t / w.ni, ii = t % w.ni;
-				emit_px(E, w.i0 + ii, w.j0 + r, [&](int c) { return (double)S.hout[c][r][ii]; });
+			switch (depth) {
+			case 32:
+				if (F.lean) fast_tile<32, true, false>(P, F, w, S, ES, simple, rows);
+				else fast_tile<32, false, false>(P, F, w, S, ES, simple, rows);
+				break;
+			case 8:
+				if (F.lean) fast_tile<8, true, false>(P, F, w, S, ES, simple, rows);
+				else fast_tile<8, false, false>(P, F, w, S, ES, simple, rows);
+				break;
+			case 16:
+				if (F.lean) fast_tile<16, true, false>(P, F, w, S, ES, simple, rows);
+				else fast_tile<16, false, false>(P, F, w, S, ES, simple, rows);
+				break;
+			default: fast_tile<0, false, false>(P, F, w, S, ES, simple, rows); break;
 			}
 		}
 	}
+	__syncthreads();  // the next item's weights replace these
 }
 
-template <int KV, int XT, int WAVES_PER_SIMD>
-__global__ void __launch_bounds__(256, WAVES_PER_SIMD) fast_kernel(const DevPlan *__restrict__ plans,
-	const FastItem *__restrict__ tiles, const int2 *__restrict__ items, int n_items, int *counter)
+// Work item (first tile, tile count).  Workgroup g first takes item g; with a
+// counter (reset to 0 before the launch) it then keeps taking items
+// gridDim.x + atomicAdd(counter, 1) until none is left.  VAR: the variant
+// every item's plan has (fast_variant).
+template <int VAR>
+__global__ void __launch_bounds__(256, VAR == FAST_GENERIC ? 1 : FAST_WG_PER_CU) fast_kernel(
+	const DevPlan *__restrict__ plans, const FastItem *__restrict__ tiles, const int2 *__restrict__ items,
+	int n_items, int *counter)
 {
-	__shared__ FastLds<KV, XT> S;
+	__shared__ FastLds S;
 	__shared__ int next_item;
 	int it = blockIdx.x;
 	while (it < n_items) {
 		const int2 band = items[it];  // first tile, tile count
-		fast_band(plans[tiles[band.x].job], tiles, band.x, band.y, S);
+		fast_band<VAR>(plans[tiles[band.x].job], tiles, band.x, band.y, S);
 		if (!counter) break;
-		__syncthreads();
 		if (threadIdx.x == 0) next_item = (int)gridDim.x + atomicAdd(counter, 1);
 		__syncthreads();
 		it = next_item;
@@ -2454,6 +2441,9 @@ struct Staged {
 	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
 	int band_rows = 0;         // rows per band of the fused kernel; 0: two-pass kernels
 	int fast_ti = 0;           // output columns per chosen-precision tile
+	std::vector<int4> h4i;     // chosen form: the horizontal list by 4-column blocks (DevPlan::h4i)
+	std::vector<float> h4w;
+	size_t o_h4i = 0, o_h4w = 0;
 	bool shape_forced = false; // band height or tile width set by an option (not tuned)
 	size_t in_bytes = 0, out_bytes = 0;
 	size_t o_in = 0, o_inter = 0, o_out = 0, o_eds = 0, o_ede = 0;
@@ -2520,73 +2510,70 @@ void dense_span(const iwx::DenseWeights &d, int a, int n, int &lo, int &cnt)
 	cnt = h - l + 1;
 }
 
-// Intermediate columns of a tile: the span of its lists, started at a
-// multiple of 4 (the horizontal pass sums over four-aligned groups of
-// positions, the same for every tile width).
+// Intermediate columns a tile reads: the whole 16-column groups (absolute,
+// so the same for both tile widths) that its lists reach.
 void fast_tile_span(const iwx::DenseWeights &dh, int i0, int ni, int &lo, int &cnt)
 {
 	dense_span(dh, i0, ni, lo, cnt);
 	if (cnt > 0) {
 		const int hi = lo + cnt - 1;
-		lo &= ~3;
-		cnt = hi - lo + 1;
+		lo &= ~15;
+		cnt = (hi & ~15) + 16 - lo;
 	}
-}
-
-// Whether tiles of ti output columns span at most FAST_MAX_XT intermediate
-// columns.
-bool fast_tile_fits(const iwamd_plan &p, const iwx::DenseWeights &dh, int ti)
-{
-	for (int i0 = 0; i0 < p.out_width; i0 += ti) {
-		int lo, n;
-		fast_tile_span(dh, i0, std::min(ti, p.out_width - i0), lo, n);
-		if (n > FAST_MAX_XT) return false;
-	}
-	return true;
 }
 
 // Tiles of the chosen-precision form, or none where it does not apply: it
 // covers integer outputs (8 or 16 bits) whose channels share one weight list
-// per dimension, with band and tile spans that fit its LDS.  want_ti: tile
-// width (32 or 16 output columns) if it fits, else (or 0) the widest that
-// fits, for the whole image.
+// per dimension, with bands of at most FAST_MAX_KV input rows.  want_ti: tile
+// width (32 or 16 output columns; 0: 32).
 void plan_fast(const iwamd_plan &p, Staged &S, int want_ti = 0)
 {
 	S.fast.clear();
 	if (p.out_sampletype == IW_SAMPLETYPE_FLOATINGPOINT || p.out_bit_depth > 16) return;
-	// Floating-point inputs can hold values past FP16's range.
+	// Floating-point inputs can hold values past the form's range.
 	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT) return;
 	for (int c = 1; c < p.inter_nch; c++) {
 		if (!same_dense(S.dv[c], S.dv[0]) || !same_dense(S.dh[c], S.dh[0])) return;
 		if (S.wv[c].edge_value != S.wv[0].edge_value) return;
 	}
 	const iwx::DenseWeights &dv = S.dv[0], &dh = S.dh[0];
-	auto span = dense_span;
 	for (int j0 = 0; j0 < p.inter_height; j0 += FAST_ROWS) {
 		int lo, n;
-		span(dv, j0, std::min(FAST_ROWS, p.inter_height - j0), lo, n);
+		dense_span(dv, j0, std::min(FAST_ROWS, p.inter_height - j0), lo, n);
 		if (n > FAST_MAX_KV) return;
 	}
-	int ti = 0;
-	for (int cand : {want_ti, 32, 16}) {
-		if (cand && fast_tile_fits(p, dh, cand)) {
-			ti = cand;
-			break;
-		}
-	}
-	if (!ti) return;
+	const int ti = want_ti == 16 ? 16 : 32;
 	S.fast_ti = ti;
 	for (int j0 = 0; j0 < p.inter_height; j0 += FAST_ROWS) {
 		FastItem it;
 		it.job = 0;
 		it.j0 = j0;
-		span(dv, j0, std::min(FAST_ROWS, p.inter_height - j0), it.p_lo, it.kv);
+		dense_span(dv, j0, std::min(FAST_ROWS, p.inter_height - j0), it.p_lo, it.kv);
 		for (int i0 = 0; i0 < p.out_width; i0 += ti) {
 			it.i0 = i0;
 			it.ni = std::min(ti, p.out_width - i0);
 			fast_tile_span(dh, i0, it.ni, it.x_lo, it.nx);
 			S.fast.push_back(it);
 		}
+	}
+}
+
+// The shared horizontal list by 4-column blocks (DevPlan::h4i, h4w).
+void build_h4(const iwx::DenseWeights &dh, Staged &S)
+{
+	const int n = (int)dh.first.size();
+	S.h4i.assign(n, make_int4(0, 0, 0, 0));
+	S.h4w.assign(4, 0.0f);  // block 0: zeros
+	for (int i = 0; i < n; i++) {
+		if (!dh.count[i]) continue;
+		const int f = dh.first[i], e = f + dh.count[i] - 1;
+		const int b0 = f >> 2, b1 = e >> 2;  // floor, also for virtual columns below 0
+		S.h4i[i] = make_int4(b0, b1 - b0 + 1, (int)(S.h4w.size() / 4), 0);
+		for (int b = b0; b <= b1; b++)
+			for (int t = 0; t < 4; t++) {
+				const int x = 4 * b + t;
+				S.h4w.push_back(x >= f && x <= e ? (float)dh.w[dh.off[i] + (x - f)] : 0.0f);
+			}
 	}
 }
 
@@ -2719,7 +2706,10 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	if (opt_int("amd:ed_seg") >= 1) S.ed_seg = opt_int("amd:ed_seg");
 	if (p.precision == IWAMD_PRECISION_FAST && !S.skip_pixels)
 		plan_fast(p, S, (want_ti == 16 || want_ti == 32) ? want_ti : 0);
-	if (!S.fast.empty()) S.shape_forced = S.fast_ti == want_ti;
+	if (!S.fast.empty()) {
+		S.shape_forced = S.fast_ti == want_ti;
+		build_h4(S.dh[0], S);
+	}
 	if (S.fast.empty()) {
 		S.band_rows = force_two ? 0 : choose_band_rows(p, S.dv);
 		if (!force_band && S.band_rows && p.inter_width > 2048) S.band_rows = 0;
@@ -2761,6 +2751,8 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 		S.oh[c].doff = L.add<int>(S.dh[c].off.size());
 		S.oh[c].dw = L.add<double>(S.dh[c].w.size());
 	}
+	S.o_h4i = L.add<int4>(S.h4i.size());
+	S.o_h4w = L.add<float>(S.h4w.size());
 	S.oq.resize(p.out_nch);
 	for (int o = 0; o < (int)S.qt.size(); o++) {
 		QOff &q = S.oq[o];
@@ -2812,6 +2804,8 @@ bool upload_job(Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
 		up(S.oh[c].doff, S.dh[c].off.data(), S.dh[c].off.size() * 4);
 		up(S.oh[c].dw, S.dh[c].w.data(), S.dh[c].w.size() * 8);
 	}
+	up(S.o_h4i, S.h4i.data(), S.h4i.size() * sizeof(int4));
+	up(S.o_h4w, S.h4w.data(), S.h4w.size() * sizeof(float));
 	for (int o = 0; o < (int)S.qt.size(); o++) {
 		const QOff &q = S.oq[o];
 		up(q.T, S.qt[o].T.data(), S.qt[o].T.size() * 8);
@@ -2914,6 +2908,8 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 		d.h.dmax = S.dh[c].max_count;
 	}
 	D.inter = (float *)(base + S.o_inter);
+	D.h4i = S.h4i.empty() ? nullptr : (const int4 *)(base + S.o_h4i);
+	D.h4w = S.h4w.empty() ? nullptr : (const float *)(base + S.o_h4w);
 	D.intclamp = p.intclamp;
 	D.out_w = p.out_width;
 	D.out_h = p.out_height;
@@ -2976,6 +2972,7 @@ constexpr int kBandR[3] = {1, 2, 4};
 // one persistent launch per group, so a batch's widest rows do not lower the
 // occupancy of its narrow ones.
 constexpr int kLdsClasses = 5;
+constexpr int kCounters = 3 * kLdsClasses + FAST_NVAR;  // band lists, then the chosen form's variants
 
 int lds_class(size_t bytes)
 {
@@ -3011,14 +3008,15 @@ struct WorkSet {
 	std::vector<int2> planes[8];      // (plan, output channel), by ed_variant
 	std::vector<int> twopass;         // plans resampled by vpass + hpass
 	// Chosen-precision tiles, and work items over them: runs of one band's
-	// tiles (first tile, count) by LDS class, small (FAST_S_*) and large.
+	// tiles (first tile, count), by kernel variant.
 	std::vector<FastItem> fast_tiles;
-	std::vector<int4> fast_runs;      // (first, count, class, -) per band, before splitting
-	std::vector<int2> fast[2];
-	size_t o_plans = 0, o_bands[3][kLdsClasses] = {}, o_planes[8] = {0}, o_counters = 0, o_fast[2] = {0, 0}, o_done = 0;
+	std::vector<int4> fast_runs;      // (first, count, variant, -) per band, before splitting
+	std::vector<int2> fast[FAST_NVAR];
+	size_t o_plans = 0, o_bands[3][kLdsClasses] = {}, o_planes[8] = {0}, o_counters = 0, o_fast[FAST_NVAR] = {},
+		o_done = 0;
 	size_t bytes = 0;
 	const FastItem *d_fast_tiles = nullptr;
-	const int2 *d_fast[2] = {nullptr, nullptr};
+	const int2 *d_fast[FAST_NVAR] = {};
 	size_t o_fast_tiles = 0;
 	int *d_done = nullptr;  // one error-diffusion completion counter per plan
 	const DevPlan *d_plans = nullptr;
@@ -3026,6 +3024,23 @@ struct WorkSet {
 	const int2 *d_planes[8] = {nullptr};
 	int *d_counters = nullptr;
 };
+
+// The chosen-form kernel variant of a plan's tiles (FAST_*), decided by the
+// kernels' own tests (fast_cin_depth, emit_simple_ctx) on the plan's kernel
+// arguments as placed at a stand-in base: which tables exist and the byte
+// layout of the input do not depend on where the block lands.  (Whether
+// 8-bit pixels load as 32-bit words, which does, FAST_LEAN8 decides per
+// item.)
+int fast_variant(const Staged &S, const iwamd_plan &p)
+{
+	DevPlan D;
+	fill_dev_plan(S, p, (char *)(uintptr_t)0x100000000ull, D);
+	FastCin F;
+	EmitSimple ES;
+	const int depth = fast_cin_depth(D, F);
+	if (!depth || !F.lean || !emit_simple_ctx(D, ES)) return FAST_GENERIC;
+	return depth == 16 ? FAST_LEAN16 : FAST_LEAN8;
+}
 
 // Adds a job's work; its kernel arguments are set later (W.plans[index]),
 // once its device data has a place.  Returns the job's index.
@@ -3035,17 +3050,17 @@ int ws_add(WorkSet &W, const Staged &S, const iwamd_plan &p)
 	W.plans.push_back(DevPlan());
 	if (S.skip_pixels) return jx;
 	if (!S.fast.empty()) {
+		const int var = fast_variant(S, p);
 		for (size_t k = 0; k < S.fast.size();) {
 			size_t e = k;
-			int maxnx = 0;
-			while (e < S.fast.size() && S.fast[e].j0 == S.fast[k].j0) maxnx = std::max(maxnx, S.fast[e++].nx);
+			while (e < S.fast.size() && S.fast[e].j0 == S.fast[k].j0) e++;
 			const int first = (int)W.fast_tiles.size();
 			for (size_t u = k; u < e; u++) {
 				FastItem it = S.fast[u];
 				it.job = jx;
 				W.fast_tiles.push_back(it);
 			}
-			W.fast_runs.push_back(make_int4(first, (int)(e - k), S.fast[k].kv <= FAST_S_KV && maxnx <= FAST_S_XT ? 0 : 1, 0));
+			W.fast_runs.push_back(make_int4(first, (int)(e - k), var, 0));
 			k = e;
 		}
 	} else if (S.band_rows) {
@@ -3069,10 +3084,11 @@ int device_cu_count();
 void ws_layout(WorkSet &W)
 {
 	// Work items of the chosen form: a band's tiles share its weights in LDS,
-	// in runs of up to eight tiles, fewer when the work set holds too few
-	// tiles to give every workgroup slot several items.
-	const int per = std::max(1, std::min(8, (int)(W.fast_tiles.size() / (8 * (size_t)device_cu_count()))));
-	for (int k = 0; k < 2; k++) W.fast[k].clear();
+	// in runs of up to FAST_RUN tiles (a wave takes every fourth), fewer when
+	// the work set holds too few tiles to give every workgroup slot several
+	// items.
+	const int per = std::max(1, std::min(FAST_RUN, (int)(W.fast_tiles.size() / (8 * (size_t)device_cu_count()))));
+	for (auto &l : W.fast) l.clear();
 	for (const int4 &r : W.fast_runs)
 		for (int a = 0; a < r.y; a += per) W.fast[r.z].push_back(make_int2(r.x + a, std::min(per, r.y - a)));
 	Layout L;
@@ -3080,9 +3096,9 @@ void ws_layout(WorkSet &W)
 	for (int r = 0; r < 3; r++)
 		for (int q = 0; q < kLdsClasses; q++) W.o_bands[r][q] = L.add<int2>(W.bands[r][q].size());
 	for (int v = 0; v < 8; v++) W.o_planes[v] = L.add<int2>(W.planes[v].size());
-	W.o_counters = L.add<int>(3 * kLdsClasses + 2);
+	W.o_counters = L.add<int>(kCounters);
 	W.o_fast_tiles = L.add<FastItem>(W.fast_tiles.size());
-	for (int k = 0; k < 2; k++) W.o_fast[k] = L.add<int2>(W.fast[k].size());
+	for (int v = 0; v < FAST_NVAR; v++) W.o_fast[v] = L.add<int2>(W.fast[v].size());
 	W.o_done = L.add<int>(W.plans.size());
 	W.bytes = (L.off + 255) & ~(size_t)255;
 }
@@ -3110,9 +3126,9 @@ bool ws_upload(WorkSet &W, char *base, hipStream_t st)
 	W.d_counters = (int *)(base + W.o_counters);
 	W.d_fast_tiles = (const FastItem *)(base + W.o_fast_tiles);
 	up(W.o_fast_tiles, W.fast_tiles.data(), W.fast_tiles.size() * sizeof(FastItem));
-	for (int k = 0; k < 2; k++) {
-		W.d_fast[k] = (const int2 *)(base + W.o_fast[k]);
-		up(W.o_fast[k], W.fast[k].data(), W.fast[k].size() * sizeof(int2));
+	for (int v = 0; v < FAST_NVAR; v++) {
+		W.d_fast[v] = (const int2 *)(base + W.o_fast[v]);
+		up(W.o_fast[v], W.fast[v].data(), W.fast[v].size() * sizeof(int2));
 	}
 	W.d_done = (int *)(base + W.o_done);
 	return ok;
@@ -3138,21 +3154,29 @@ int device_cu_count()
 void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 {
 	if (persistent) {
-		hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, 3 * kLdsClasses + 2);
+		hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, W.d_counters, kCounters);
 		const int np = (int)W.plans.size();
 		hipLaunchKernelGGL(reset_counters_kernel, dim3((np + 255) / 256), dim3(256), 0, st, W.d_done, np);
 	}
-	if (!W.fast[0].empty()) {
-		const int n = (int)W.fast[0].size();
-		const int grid = persistent ? std::min(n, device_cu_count() * 4) : n;
-		hipLaunchKernelGGL((fast_kernel<FAST_S_KV, FAST_S_XT, 4>), dim3(grid), dim3(256), 0, st, W.d_plans,
-			W.d_fast_tiles, W.d_fast[0], n, persistent ? W.d_counters + 3 * kLdsClasses : nullptr);
-	}
-	if (!W.fast[1].empty()) {
-		const int n = (int)W.fast[1].size();
-		const int grid = persistent ? std::min(n, device_cu_count() * 2) : n;
-		hipLaunchKernelGGL((fast_kernel<FAST_MAX_KV, FAST_MAX_XT, 2>), dim3(grid), dim3(256), 0, st, W.d_plans,
-			W.d_fast_tiles, W.d_fast[1], n, persistent ? W.d_counters + 3 * kLdsClasses + 1 : nullptr);
+	for (int v = 0; v < FAST_NVAR; v++) {
+		const int n = (int)W.fast[v].size();
+		if (!n) continue;
+		const int grid = persistent ? std::min(n, device_cu_count() * (v == FAST_GENERIC ? 1 : FAST_WG_PER_CU)) : n;
+		int *ctr = persistent ? W.d_counters + 3 * kLdsClasses + v : nullptr;
+		switch (v) {
+		case FAST_LEAN8:
+			hipLaunchKernelGGL(fast_kernel<FAST_LEAN8>, dim3(grid), dim3(256), 0, st, W.d_plans, W.d_fast_tiles,
+				W.d_fast[v], n, ctr);
+			break;
+		case FAST_LEAN16:
+			hipLaunchKernelGGL(fast_kernel<FAST_LEAN16>, dim3(grid), dim3(256), 0, st, W.d_plans, W.d_fast_tiles,
+				W.d_fast[v], n, ctr);
+			break;
+		default:
+			hipLaunchKernelGGL(fast_kernel<FAST_GENERIC>, dim3(grid), dim3(256), 0, st, W.d_plans, W.d_fast_tiles,
+				W.d_fast[v], n, ctr);
+			break;
+		}
 	}
 	for (int r = 0; r < 3; r++)
 		for (int q = 0; q < kLdsClasses; q++) {
@@ -3284,8 +3308,8 @@ int tune_cands(const iwamd_plan &p, const Staged &S, int form, int *cand)
 		for (int R : {4, 2, 1})
 			if (band_fits(p, S.dv, R)) cand[n++] = R;
 	} else {
-		for (int ti : {32, 16})
-			if (fast_tile_fits(p, S.dh[0], ti)) cand[n++] = ti;
+		cand[n++] = 32;
+		cand[n++] = 16;
 	}
 	return n;
 }
