@@ -1123,13 +1123,14 @@ __global__ void reset_counters_kernel(int *c, int n)
 #define FAST_G 2          // output groups of 16 columns per tile, at most
 #define FAST_MAX_TI (16 * FAST_G)
 #define FAST_KC 8         // vertical steps whose input loads are issued together
-#define FAST_RUN 8        // tiles per work item, at most
+#define FAST_RUN 16       // tiles per work item, at most
 #define FAST_WG_PER_CU 4  // workgroups per CU of a persistent launch
 
-// Kernel variants, by what a plan's tiles need (fast_variant): 8-bit input
-// through the LDS tables with the lean colour-in and the common per-pixel
-// work; 16-bit input likewise; anything else.
-enum { FAST_LEAN8 = 0, FAST_LEAN16 = 1, FAST_GENERIC = 2, FAST_NVAR = 3 };
+// Kernel variants, by what a plan's tiles need (fast_variant): RGBA8 to RGBA8
+// with the lean colour-in and straight-line per-pixel work (fast_rgba8_ok);
+// 8-bit input through the LDS tables with the lean colour-in and the common
+// per-pixel work; 16-bit input likewise; anything else.
+enum { FAST_RGBA8 = 0, FAST_LEAN8 = 1, FAST_LEAN16 = 2, FAST_GENERIC = 3, FAST_NVAR = 4 };
 
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
@@ -1142,10 +1143,60 @@ struct FastItem {
 // Per workgroup: the band's vertical weights by input row (so a step's four
 // rows x 16 output rows are 64 consecutive words) and FP32 copies of the
 // 8-bit input tables.
+#define FAST_QIX 2049  // bin index of an 8-bit quantiser table (2^11 bins)
 struct __attribute__((aligned(16))) FastLds {
 	float wv[FAST_MAX_KV][FAST_ROWS];
 	float lut[IWAMD_MAX_CH + 1][256];  // channels, then alpha
+	// FAST_RGBA8: the searched quantiser tables, colour then alpha.
+	double qa[2][256];
+	unsigned short qix[2][FAST_QIX + 7];
 };
+
+// Whether a plan takes FAST_RGBA8's per-pixel work: the common per-pixel work
+// (EmitSimple) with four channels processed alpha (channel 3) first, output
+// channel c from intermediate channel c, 8-bit output, every colour channel
+// unassociated, no clamp of the intermediate, no error diffusion, and each
+// channel's code the count of its table's entries <= s (levels as codes) over
+// at most 256 entries with a 2^11-bin index, one table for the three colour
+// channels.
+__host__ __device__ inline bool fast_rgba8_ok(const DevPlan &P, const EmitSimple &E)
+{
+	if (E.nch != 4 || P.inter_alpha != 3 || P.out_nch != 4 || P.out_depth != 8 || P.intclamp) return false;
+	for (int idx = 0; idx < 4; idx++) {
+		const int f = E.f[idx], c = f & 3, o = (f >> 2 & 7) - 1;
+		if (c != (idx + 3) % 4 || o != c || (f & (1 << 7)) || !(f & (1 << 8)) || (f >> 9 & 31) != 11) return false;
+		if (((f >> 5) & 1) != (idx ? 1 : 0) || ((f >> 6) & 1) != (idx ? 0 : 1)) return false;
+		if (E.n[idx] < 1 || E.n[idx] > 256 || !E.sa[idx] || !E.six[idx]) return false;
+		if (idx > 1 && (E.sa[idx] != E.sa[1] || E.six[idx] != E.six[1] || E.n[idx] != E.n[1])) return false;
+	}
+	return true;
+}
+
+// FAST_RGBA8's code of a clamped sample s through LDS table q: the count of
+// entries a[0..n) <= s, found in s's bin (at most two entries there for these
+// tables; more, and the bin is bisected).
+__device__ __forceinline__ int fast_q8(const FastLds &S, int q, int n, double s)
+{
+	const int b = min((int)(s * 2048.0), 2047);
+	int lo = S.qix[q][b];
+	const int m = S.qix[q][b + 1] - lo;
+	const double a0 = S.qa[q][min(lo, n - 1)], a1 = S.qa[q][min(lo + 1, n - 1)];
+	int k = lo + (m > 0 && a0 <= s) + (m > 1 && a1 <= s);
+	if (m > 2) {
+		int len = m;
+		while (len > 0) {
+			const int half = len >> 1;
+			if (S.qa[q][lo + half] <= s) {
+				lo += half + 1;
+				len -= half + 1;
+			} else {
+				len = half;
+			}
+		}
+		k = lo;
+	}
+	return k;
+}
 
 // The 32-bit word of a 4-byte pixel: logical row p of the column at byte
 // offset cb, clamped to the image.
@@ -1190,8 +1241,9 @@ __device__ __forceinline__ int fast_grp(int x) { return x >> 4; }
 
 // One tile on one wave.  rows: the band's output rows that exist.  SIMPLE:
 // the plan takes the common per-pixel work (EmitSimple) -- known, so the
-// general per-pixel code is left out; else simple says which applies.
-template <int DEPTH, bool LEAN, bool SIMPLE>
+// general per-pixel code is left out; else simple says which applies.  RGBA8:
+// the plan takes FAST_RGBA8's (fast_rgba8_ok; implies SIMPLE and LEAN).
+template <int DEPTH, bool LEAN, bool SIMPLE, bool RGBA8 = false>
 __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, const FastItem &w, const FastLds &S,
 	const EmitSimple &ES, bool simple, int rows)
 {
@@ -1202,7 +1254,7 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 	const DevWeights &dh = P.ic[0].h;  // every channel shares the lists
 	const g_cint *hfirst = (const g_cint *)dh.dfirst, *hcount = (const g_cint *)dh.dcount;
 	const int4 *h4i = P.h4i;
-	const float4_t *h4w = (const float4_t *)P.h4w;
+	const __attribute__((address_space(1))) float4_t *h4w = (const __attribute__((address_space(1))) float4_t *)P.h4w;
 
 	// Each output group: the lane's column's weights by 4-column blocks
 	// (first block, block count, offset in h4w) and the intermediate groups
@@ -1228,13 +1280,21 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 			f = min(f, __shfl_xor(f, m));
 			e = max(e, __shfl_xor(e, m));
 		}
-		ga[og] = e >= f ? fast_grp(f) : 0x7fffffff;
-		gb[og] = e >= f ? fast_grp(e) : -0x7fffffff;
-		if (og < ng && e >= f) {
+		// Every lane holds the same bounds now; say so to the compiler.
+		ga[og] = __builtin_amdgcn_readfirstlane(e >= f ? fast_grp(f) : 0x7fffffff);
+		gb[og] = __builtin_amdgcn_readfirstlane(e >= f ? fast_grp(e) : -0x7fffffff);
+		if (og < ng && ga[og] <= gb[og]) {
 			g_first = min(g_first, ga[og]);
 			g_last = max(g_last, gb[og]);
 		}
 	}
+	// Channels past the plan's are computed too in the lean forms (their
+	// samples repeat channel 0, so they stay finite) and never read: the
+	// products then run without a test between them.
+	const int CM = SIMPLE && LEAN ? IWAMD_MAX_CH : C;
+	bool fix = P.intclamp != 0;
+#pragma unroll
+	for (int c = 0; c < IWAMD_MAX_CH; c++) fix |= c < C && P.ic[c].h.edge_transparent;
 
 	float4_t acc[FAST_G][IWAMD_MAX_CH];
 #pragma unroll
@@ -1263,6 +1323,9 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 				unsigned int wd[FAST_KC];
 #pragma unroll
 				for (int s = 0; s < FAST_KC; s++) wd[s] = fast_word(P, F, cb, w.p_lo + k0 + 4 * s + lq);
+				// Keeps the loads here, all in flight together (otherwise each is
+				// sunk to its use behind the step's test and waited for alone).
+				asm volatile("" ::: "memory");
 #pragma unroll
 				for (int s = 0; s < FAST_KC; s++) {
 					if (k0 + 4 * s >= kv4) break;
@@ -1271,7 +1334,7 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 					const float b = S.wv[k0 + 4 * s + lq][lr];
 #pragma unroll
 					for (int c = 0; c < IWAMD_MAX_CH; c++)
-						if (c < C) dv[c] = mfma4(v[c], b, dv[c]);
+						if (c < CM) dv[c] = mfma4(v[c], b, dv[c]);
 				}
 			}
 		} else {
@@ -1284,14 +1347,14 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 				const float b = S.wv[k + lq][lr];
 #pragma unroll
 				for (int c = 0; c < IWAMD_MAX_CH; c++)
-					if (c < C) dv[c] = mfma4(v[c], b, dv[c]);
+					if (c < CM) dv[c] = mfma4(v[c], b, dv[c]);
 			}
 		}
 		// The intermediate: a virtual column outside a TRANSPARENT edge is the
 		// edge value; intclamp clamps.
 #pragma unroll
 		for (int c = 0; c < IWAMD_MAX_CH; c++) {
-			if (c >= C) continue;
+			if (!fix || c >= C) continue;
 #pragma unroll
 			for (int t = 0; t < 4; t++) {
 				float v = dv[c][t];
@@ -1308,10 +1371,44 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 			for (int t = 0; t < 4; t++)
 #pragma unroll
 				for (int c = 0; c < IWAMD_MAX_CH; c++)
-					if (c < C) acc[og][c] = mfma4(dv[c][t], hw[og][t], acc[og][c]);
+					if (c < CM) acc[og][c] = mfma4(dv[c][t], hw[og][t], acc[og][c]);
 		}
 	}
 
+	if constexpr (RGBA8) {
+		// Alpha first: its code, and alpha rounded to FP32 (as the reference's
+		// final_alpha32) whose reciprocal unassociates the colour channels
+		// (one FP64 division per pixel; the sums themselves carry FP32
+		// rounding already).  Then one 32-bit store per pixel.
+		const int n_a = ES.n[0], n_c = ES.n[1];
+		const bool pack = ES.pack4;
+#pragma unroll
+		for (int og = 0; og < FAST_G; og++) {
+			if (og >= ng) break;
+			const int ii = 16 * og + lr, i = w.i0 + ii;
+#pragma unroll
+			for (int t = 0; t < 4; t++) {
+				const int r = 4 * lq + t;
+				const float af = acc[og][3][t];
+				const double a = (double)af;
+				unsigned int word = (unsigned int)fast_q8(S, 1, n_a, clamp01(a)) << 24;
+				const double inv = af != 0.0f ? 1.0 / a : 1.0;
+#pragma unroll
+				for (int c = 0; c < 3; c++)
+					word |= (unsigned int)fast_q8(S, 0, n_c, clamp01((double)acc[og][c][t] * inv)) << (8 * c);
+				if (ii < w.ni && r < rows) {
+					g_u8 *px = (g_u8 *)P.out + (size_t)(w.j0 + r) * P.out_bpr + 4 * (size_t)i;
+					if (pack) {
+						*(__attribute__((address_space(1))) unsigned int *)px = word;
+					} else {
+#pragma unroll
+						for (int c = 0; c < 4; c++) px[c] = (uint8_t)(word >> (8 * c));
+					}
+				}
+			}
+		}
+		return;
+	}
 	// Per-pixel work in FP64, as in the plain form: one pixel per step, the
 	// step's sums picked out of the accumulators (og and t are uniform), so
 	// that the per-pixel code appears once.
@@ -1359,7 +1456,9 @@ template <int VAR>
 __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__restrict__ tiles, int first, int count,
 	FastLds &S)
 {
-	const int tid = threadIdx.x, wave = tid >> 6;
+	// The wave index is uniform: so is every tile field read with it, which
+	// keeps the tiles' loops free of lane masks.
+	const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 	const FastItem w0 = tiles[first];
 	const int kv4 = (w0.kv + 3) & ~3;
 	const DevWeights &dv = P.ic[0].v;
@@ -1386,13 +1485,28 @@ __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__re
 			for (int c = 0; c < IWAMD_MAX_CH; c++) S.lut[c][t] = (float)((const g_cf64 *)F.lut[c])[t];
 			S.lut[IWAMD_MAX_CH][t] = (float)((const g_cf64 *)F.alut)[t];
 		}
-	__syncthreads();
 	EmitSimple ES;
 	const bool simple = emit_simple_ctx(P, ES);
+	if constexpr (VAR == FAST_RGBA8) {
+		// The colour table (slot 0, from the first colour channel) and the
+		// alpha table (slot 1).
+		for (int t = tid; t < 2 * 256; t += blockDim.x) {
+			const int q = t >> 8, e = t & 255, idx = q ? 0 : 1;
+			S.qa[q][e] = e < ES.n[idx] ? ((const g_cf64 *)ES.sa[idx])[e] : 2.0;
+		}
+		for (int t = tid; t < 2 * FAST_QIX; t += blockDim.x) {
+			const int q = t >= FAST_QIX, e = t - q * FAST_QIX, idx = q ? 0 : 1;
+			S.qix[q][e] = ((const g_cu16 *)ES.six[idx])[e];
+		}
+	}
+	__syncthreads();
 	const int rows = min(FAST_ROWS, P.Hp - w0.j0);
 	for (int tt = wave; tt < count; tt += (int)(blockDim.x >> 6)) {
 		const FastItem w = tiles[first + tt];
-		if constexpr (VAR == FAST_LEAN8) {
+		if constexpr (VAR == FAST_RGBA8) {
+			if (depth == 32) fast_tile<32, true, true, true>(P, F, w, S, ES, true, rows);
+			else fast_tile<8, true, true, true>(P, F, w, S, ES, true, rows);
+		} else if constexpr (VAR == FAST_LEAN8) {
 			if (depth == 32) fast_tile<32, true, true>(P, F, w, S, ES, true, rows);
 			else fast_tile<8, true, true>(P, F, w, S, ES, true, rows);
 		} else if constexpr (VAR == FAST_LEAN16) {
@@ -2754,8 +2868,23 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	S.o_h4i = L.add<int4>(S.h4i.size());
 	S.o_h4w = L.add<float>(S.h4w.size());
 	S.oq.resize(p.out_nch);
+	// Channels whose tables are the same (the host builds each once per
+	// process) share one device copy.
+	auto same_tables = [&](int a, int b) {
+		const iwx::QuantTable &x = S.qt[a], &y = S.qt[b];
+		return x.P == y.P && x.ix_log2 == y.ix_log2 && x.code_is_level == y.code_is_level && x.T == y.T &&
+			x.U == y.U && x.lin == y.lin && x.code == y.code && x.nc == y.nc && x.ix == y.ix && x.D == y.D &&
+			S.pt[a].row_q == S.pt[b].row_q && S.pt[a].col_mont == S.pt[b].col_mont && S.et[a].M == S.et[b].M;
+	};
 	for (int o = 0; o < (int)S.qt.size(); o++) {
 		QOff &q = S.oq[o];
+		int same = -1;
+		for (int k = 0; k < o && same < 0; k++)
+			if (same_tables(k, o)) same = k;
+		if (same >= 0) {
+			q = S.oq[same];
+			continue;
+		}
 		q.T = L.add<double>(S.qt[o].T.size());
 		q.U = L.add<double>(S.qt[o].U.size());
 		q.lin = L.add<double>(S.qt[o].lin.size());
@@ -3039,7 +3168,8 @@ int fast_variant(const Staged &S, const iwamd_plan &p)
 	EmitSimple ES;
 	const int depth = fast_cin_depth(D, F);
 	if (!depth || !F.lean || !emit_simple_ctx(D, ES)) return FAST_GENERIC;
-	return depth == 16 ? FAST_LEAN16 : FAST_LEAN8;
+	if (depth == 16) return FAST_LEAN16;
+	return fast_rgba8_ok(D, ES) ? FAST_RGBA8 : FAST_LEAN8;
 }
 
 // Adds a job's work; its kernel arguments are set later (W.plans[index]),
@@ -3164,6 +3294,10 @@ void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 		const int grid = persistent ? std::min(n, device_cu_count() * (v == FAST_GENERIC ? 1 : FAST_WG_PER_CU)) : n;
 		int *ctr = persistent ? W.d_counters + 3 * kLdsClasses + v : nullptr;
 		switch (v) {
+		case FAST_RGBA8:
+			hipLaunchKernelGGL(fast_kernel<FAST_RGBA8>, dim3(grid), dim3(256), 0, st, W.d_plans, W.d_fast_tiles,
+				W.d_fast[v], n, ctr);
+			break;
 		case FAST_LEAN8:
 			hipLaunchKernelGGL(fast_kernel<FAST_LEAN8>, dim3(grid), dim3(256), 0, st, W.d_plans, W.d_fast_tiles,
 				W.d_fast[v], n, ctr);
