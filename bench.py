@@ -7,8 +7,9 @@ block, a descriptor table, persistent kernels, one hipGraph replay per step.
 Inputs are resident in HBM before the timed region.  Both forms run:
 
   plain  FP64 resampling, byte-identical to the reference core
-  fast   the chosen precision form: split hi/lo FP16 products on MFMA with
-         FP32 sums for the resampling, FP64 from the resampled value on
+  fast   the chosen precision form: FP32 operands and sums on MFMA
+         (v_mfma_f32_16x16x4f32) for the resampling, FP64 from the resampled
+         value on
 
 `value` is the chosen form's output Mpix/s; it is reported as the headline only
 when its error against the plain form is <= 1 LSB on every one of the 4096
@@ -40,13 +41,13 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "output Mpix/s on 1 GPU, chosen vs FP64 plain at <=1 LSB err; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
-MFMA_F16_PEAK_TFLOPS = 2500.0
+MFMA_F32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libiwref.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "libiwo.so")
 # Arithmetic of each form's resampling (what the line's dtype names).
-DTYPE = {"plain": "f64", "fast": "f16"}
+DTYPE = {"plain": "f64", "fast": "f32"}
 # Per-kernel slots of iwamd_job_kernel_ms: the vertical pass (or the whole
-# fused resample: band kernel or the FP16-MFMA kernel), the horizontal pass
+# fused resample: band kernel or the MFMA kernel), the horizontal pass
 # (0 when fused), error diffusion.
 KERNELS = ["resample", "hpass", "errdiff"]
 
@@ -244,15 +245,14 @@ def measure_job(lib, job, steps, warmup):
 
 def mfma_use(flops, ms):
     """Useful resampling FLOP rate of the chosen form's band contraction
-    against MI355X's dense FP16 matrix peak.  Useful = a multiply and an add
-    per listed tap of both passes; the split hi/lo operands execute three
-    FP16 products per useful one, so the issued rate is three times this."""
+    against MI355X's dense FP32 matrix peak.  Useful = a multiply and an add
+    per listed tap of both passes (the dense band blocks also multiply the
+    zeros around each list, which this leaves out)."""
     if not flops or not ms:
         return None
     t = flops / (ms * 1e-3) / 1e12
-    return {"useful_tflops": round(t, 3), "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(t / MFMA_F16_PEAK_TFLOPS, 6), "split_products": 3,
-            "issued_frac": round(3 * t / MFMA_F16_PEAK_TFLOPS, 6), "useful_flops_per_launch": flops}
+    return {"useful_tflops": round(t, 3), "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(t / MFMA_F32_PEAK_TFLOPS, 6), "useful_flops_per_launch": flops}
 
 
 def measured_traffic(kernel, summary):
@@ -322,7 +322,7 @@ def bench_c3(a):
         if "fast" in forms:
             base["fast_vs_reference"] = error_over(list(zip(ref_res, forms["fast"]["res"][:k])))
     ach = m["alg_bytes"] / (m["kernel_ms"] * 1e-3) / 1e9
-    kname = ("fast_kernel, both LDS classes" if chosen == "fast" else "band_kernel, all band heights and LDS classes")
+    kname = ("fast_kernel, every plan variant" if chosen == "fast" else "band_kernel, all band heights and LDS classes")
     line = {"metric": METRIC, "value": round(m["out_px"] / 1e6 / (m["ms"] / 1e3), 3), "unit": "Mpix/s",
             "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(m["ms"], 4),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -331,8 +331,8 @@ def bench_c3(a):
                                    "axis, triangle/catrom/lanczos2/lanczos3, 8-bit sRGB, no dither; one batch, "
                                    "one hipGraph replay per step" % m["n"],
                        "global_batch": m["n"], "precision": chosen,
-                       "dtype_note": "fast: FP16 hi/lo split operands on MFMA, FP32 sums, FP64 from the "
-                                     "resampled value on; plain: FP64 throughout"},
+                       "dtype_note": "fast: FP32 operands and sums on MFMA, FP64 from the resampled value "
+                                     "on; plain: FP64 throughout"},
             "roofline": {"kernel": "%s (persistent launches of one hipGraph step)" % kname,
                          "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(ach / HBM_PEAK_GBS, 5),

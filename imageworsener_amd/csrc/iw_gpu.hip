@@ -88,6 +88,7 @@ struct DevOChan {
 	int ix_log2;
 	const double *D;                   // QuantTable::D (null: not folded)
 	int code_is_level;
+	int ix_le2;                        // every bin of ix holds at most two entries
 	const double *M;                   // error diffusion: round-up points (EdTables)
 	int ed_warm;                       // error diffusion: warm-up pixels of a segment's walk
 	double bk1, bk2;
@@ -1148,20 +1149,31 @@ struct __attribute__((aligned(16))) FastLds {
 	float wv[FAST_MAX_KV][FAST_ROWS];
 	float lut[IWAMD_MAX_CH + 1][256];  // channels, then alpha
 	// FAST_RGBA8: the searched quantiser tables, colour then alpha.
-	double qa[2][256];
+	double qa[2][258];  // entries, then 2.0 (above any sample)
 	unsigned short qix[2][FAST_QIX + 7];
 };
 
-// Whether a plan takes FAST_RGBA8's per-pixel work: the common per-pixel work
-// (EmitSimple) with four channels processed alpha (channel 3) first, output
-// channel c from intermediate channel c, 8-bit output, every colour channel
-// unassociated, no clamp of the intermediate, no error diffusion, and each
-// channel's code the count of its table's entries <= s (levels as codes) over
-// at most 256 entries with a 2^11-bin index, one table for the three colour
-// channels.
-__host__ __device__ inline bool fast_rgba8_ok(const DevPlan &P, const EmitSimple &E)
+// Whether a plan takes FAST_RGBA8's colour-in and per-pixel work: RGBA8
+// input whose colour samples are premultiplied by alpha (byte 3, through its
+// own table); the common per-pixel work (EmitSimple) with four channels
+// processed alpha (channel 3) first, output channel c from intermediate
+// channel c, 8-bit output, every colour channel unassociated, no clamp of
+// the intermediate, no error diffusion; each channel's code the count of its
+// table's entries <= s (levels as codes) over at most 256 entries with a
+// 2^11-bin index whose bins hold at most two entries, one table for the three
+// colour channels.
+__host__ __device__ inline bool fast_rgba8_ok(const DevPlan &P, const EmitSimple &E, const FastCin &F)
 {
 	if (E.nch != 4 || P.inter_alpha != 3 || P.out_nch != 4 || P.out_depth != 8 || P.intclamp) return false;
+	// Input: samples in bytes 0..3, alpha in byte 3 through channel 3's own
+	// table; colour premultiplied, alpha not.
+	for (int c = 0; c < 4; c++)
+		if (F.chb[c] != c || F.mode[c] != (c < 3 ? 1 : 0)) return false;
+	if (F.ab != 3 || F.alut != F.lut[3]) return false;
+	// Byte offsets within the input fit 31 bits (fast_tile's row addressing).
+	if ((double)P.in_bpr * (double)P.in_h >= 2147483648.0) return false;
+	for (int o = 0; o < 4; o++)
+		if (!P.oc[o].ix_le2) return false;
 	for (int idx = 0; idx < 4; idx++) {
 		const int f = E.f[idx], c = f & 3, o = (f >> 2 & 7) - 1;
 		if (c != (idx + 3) % 4 || o != c || (f & (1 << 7)) || !(f & (1 << 8)) || (f >> 9 & 31) != 11) return false;
@@ -1172,30 +1184,23 @@ __host__ __device__ inline bool fast_rgba8_ok(const DevPlan &P, const EmitSimple
 	return true;
 }
 
-// FAST_RGBA8's code of a clamped sample s through LDS table q: the count of
-// entries a[0..n) <= s, found in s's bin (at most two entries there for these
-// tables; more, and the bin is bisected).
-__device__ __forceinline__ int fast_q8(const FastLds &S, int q, int n, double s)
+// FAST_RGBA8's code of a sample s in [0, 1] through LDS table q: the count of
+// entries <= s.  The lo entries below s's bin are below s; the bin holds at
+// most two more; every later entry (the table's own, or the 2.0 padding) is
+// at or above the bin's upper edge, which is above s.
+__device__ __forceinline__ int fast_q8(const FastLds &S, int q, double s)
 {
 	const int b = min((int)(s * 2048.0), 2047);
-	int lo = S.qix[q][b];
-	const int m = S.qix[q][b + 1] - lo;
-	const double a0 = S.qa[q][min(lo, n - 1)], a1 = S.qa[q][min(lo + 1, n - 1)];
-	int k = lo + (m > 0 && a0 <= s) + (m > 1 && a1 <= s);
-	if (m > 2) {
-		int len = m;
-		while (len > 0) {
-			const int half = len >> 1;
-			if (S.qa[q][lo + half] <= s) {
-				lo += half + 1;
-				len -= half + 1;
-			} else {
-				len = half;
-			}
-		}
-		k = lo;
-	}
-	return k;
+	const int lo = S.qix[q][b];
+	return lo + (S.qa[q][lo] <= s) + (S.qa[q][lo + 1] <= s);
+}
+
+// 1/a for a finite nonzero double: the hardware reciprocal and one Newton
+// step (within an ulp or so; the chosen form's sums carry FP32 rounding).
+__device__ __forceinline__ double fast_rcp(double a)
+{
+	const double r = __builtin_amdgcn_rcp(a);
+	return __builtin_fma(r, __builtin_fma(-a, r, 1.0), r);
 }
 
 // The 32-bit word of a 4-byte pixel: logical row p of the column at byte
@@ -1229,6 +1234,17 @@ __device__ __forceinline__ void fast_fetch_lds8(const DevPlan &P, const FastCin 
 	const float a = lut[IWAMD_MAX_CH][in[off + F.ab]];
 #pragma unroll
 	for (int c = 0; c < IWAMD_MAX_CH; c++) v[c] = lut[c][raw[c]] * (F.mode[c] ? a : 1.0f);
+}
+
+// fast_decode_word for FAST_RGBA8 plans: colour bytes 0..2 premultiplied by
+// the alpha of byte 3, which is channel 3's value itself.
+__device__ __forceinline__ void fast_decode_rgba8(const float (*lut)[256], unsigned int word, float v[IWAMD_MAX_CH])
+{
+	const float a = lut[3][word >> 24];
+	v[0] = lut[0][word & 0xffu] * a;
+	v[1] = lut[1][(word >> 8) & 0xffu] * a;
+	v[2] = lut[2][(word >> 16) & 0xffu] * a;
+	v[3] = a;
 }
 
 __device__ __forceinline__ float4_t mfma4(float a, float b, float4_t c)
@@ -1317,7 +1333,34 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 		float4_t dv[IWAMD_MAX_CH];
 #pragma unroll
 		for (int c = 0; c < IWAMD_MAX_CH; c++) dv[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-		if constexpr (DEPTH == 32 && LEAN) {
+		// FAST_RGBA8, rows inside the image, ascending addresses: each load is a
+		// uniform row address plus the lane's 32-bit offset, and only the
+		// steps the band has are loaded.
+		bool direct = false;
+		if constexpr (RGBA8 && DEPTH == 32)
+			direct = w.p_lo >= 0 && w.p_lo + kv4 <= P.ih && F.dx > 0 && F.dy > 0;
+		if (direct) {
+			const size_t voff = (unsigned int)(xc * (int)F.dx + lq * (int)F.dy);
+			const g_u8 *row0 = (const g_u8 *)P.in + F.base + (long long)w.p_lo * F.dy;
+			for (int k0 = 0; k0 < kv4; k0 += 4 * FAST_KC) {
+				unsigned int wd[FAST_KC];
+#pragma unroll
+				for (int s = 0; s < FAST_KC; s++)
+					if (k0 + 4 * s < kv4)
+						wd[s] = *(const __attribute__((address_space(1))) unsigned int *)(row0 +
+							(long long)(k0 + 4 * s) * F.dy + voff);
+				asm volatile("" ::: "memory");
+#pragma unroll
+				for (int s = 0; s < FAST_KC; s++) {
+					if (k0 + 4 * s >= kv4) break;
+					float v[IWAMD_MAX_CH];
+					fast_decode_rgba8(S.lut, wd[s], v);
+					const float b = S.wv[k0 + 4 * s + lq][lr];
+#pragma unroll
+					for (int c = 0; c < IWAMD_MAX_CH; c++) dv[c] = mfma4(v[c], b, dv[c]);
+				}
+			}
+		} else if constexpr (DEPTH == 32 && LEAN) {
 			// FAST_KC steps' words are loaded before the first is used.
 			for (int k0 = 0; k0 < kv4; k0 += 4 * FAST_KC) {
 				unsigned int wd[FAST_KC];
@@ -1380,8 +1423,8 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 		// final_alpha32) whose reciprocal unassociates the colour channels
 		// (one FP64 division per pixel; the sums themselves carry FP32
 		// rounding already).  Then one 32-bit store per pixel.
-		const int n_a = ES.n[0], n_c = ES.n[1];
 		const bool pack = ES.pack4;
+		auto clamp = [](double v) { return fmin(fmax(v, 0.0), 1.0); };
 #pragma unroll
 		for (int og = 0; og < FAST_G; og++) {
 			if (og >= ng) break;
@@ -1391,11 +1434,11 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 				const int r = 4 * lq + t;
 				const float af = acc[og][3][t];
 				const double a = (double)af;
-				unsigned int word = (unsigned int)fast_q8(S, 1, n_a, clamp01(a)) << 24;
-				const double inv = af != 0.0f ? 1.0 / a : 1.0;
+				unsigned int word = (unsigned int)fast_q8(S, 1, clamp(a)) << 24;
+				const double inv = af != 0.0f ? fast_rcp(a) : 1.0;
 #pragma unroll
 				for (int c = 0; c < 3; c++)
-					word |= (unsigned int)fast_q8(S, 0, n_c, clamp01((double)acc[og][c][t] * inv)) << (8 * c);
+					word |= (unsigned int)fast_q8(S, 0, clamp((double)acc[og][c][t] * inv)) << (8 * c);
 				if (ii < w.ni && r < rows) {
 					g_u8 *px = (g_u8 *)P.out + (size_t)(w.j0 + r) * P.out_bpr + 4 * (size_t)i;
 					if (pack) {
@@ -1490,8 +1533,8 @@ __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__re
 	if constexpr (VAR == FAST_RGBA8) {
 		// The colour table (slot 0, from the first colour channel) and the
 		// alpha table (slot 1).
-		for (int t = tid; t < 2 * 256; t += blockDim.x) {
-			const int q = t >> 8, e = t & 255, idx = q ? 0 : 1;
+		for (int t = tid; t < 2 * 258; t += blockDim.x) {
+			const int q = t >= 258, e = t - q * 258, idx = q ? 0 : 1;
 			S.qa[q][e] = e < ES.n[idx] ? ((const g_cf64 *)ES.sa[idx])[e] : 2.0;
 		}
 		for (int t = tid; t < 2 * FAST_QIX; t += blockDim.x) {
@@ -3068,6 +3111,9 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 			d.ix = S.qt[o].ix.empty() ? nullptr : (const unsigned short *)(base + q.ix);
 			d.D = S.qt[o].D.empty() ? nullptr : (const double *)(base + q.D);
 			d.code_is_level = S.qt[o].code_is_level ? 1 : 0;
+			d.ix_le2 = !S.qt[o].ix.empty();
+			for (size_t b = 0; b + 1 < S.qt[o].ix.size(); b++)
+				if (S.qt[o].ix[b + 1] - S.qt[o].ix[b] > 2) d.ix_le2 = 0;
 			d.ix_log2 = S.qt[o].ix_log2;
 			d.prng_row = (const uint64_t *)(base + q.prow);
 			d.prng_col = (const uint64_t *)(base + q.pcol);
@@ -3169,7 +3215,7 @@ int fast_variant(const Staged &S, const iwamd_plan &p)
 	const int depth = fast_cin_depth(D, F);
 	if (!depth || !F.lean || !emit_simple_ctx(D, ES)) return FAST_GENERIC;
 	if (depth == 16) return FAST_LEAN16;
-	return fast_rgba8_ok(D, ES) ? FAST_RGBA8 : FAST_LEAN8;
+	return fast_rgba8_ok(D, ES, F) ? FAST_RGBA8 : FAST_LEAN8;
 }
 
 // Adds a job's work; its kernel arguments are set later (W.plans[index]),
