@@ -99,6 +99,7 @@ struct DevPlan {
 	const uint8_t *in;
 	size_t in_bpr;
 	int in_w, in_h, bit_depth, in_float, nch_phys;
+	int in_f64;  // 32-bit float input held as linear doubles (converted on the host)
 	unsigned int orient;
 	int sx, sy, iw, ih;
 	int alpha_in_index;
@@ -178,6 +179,11 @@ __device__ __forceinline__ unsigned int raw_int(const DevPlan &P, int rx, int ry
 __device__ __forceinline__ double raw_float(const DevPlan &P, int rx, int ry, int ch)
 {
 	if (P.bit_depth != 32) return 0.0;
+	if (P.in_f64) {
+		double d;
+		memcpy(&d, P.in + (size_t)ry * P.in_bpr + (size_t)(P.nch_phys * rx + ch) * 8, 8);
+		return d;
+	}
 	float f;
 	memcpy(&f, P.in + (size_t)ry * P.in_bpr + (size_t)(P.nch_phys * rx + ch) * 4, 4);
 	return (double)f;
@@ -2595,6 +2601,8 @@ struct Staged {
 	int ed_list[4] = {-1, -1, -1, -1};
 	bool in_int = true;
 	bool in_elsewhere = false; // the input image is already in device memory (no region, no copy)
+	std::vector<double> in_f64;  // 32-bit float input in linear light (non-linear colour spaces), else empty
+	size_t in_f64_bpr = 0;
 	bool skip_pixels = false;  // unknown filter: the reference reports and outputs zeros
 	int band_rows = 0;         // rows per band of the fused kernel; 0: two-pass kernels
 	int fast_ti = 0;           // output columns per chosen-precision tile
@@ -2734,6 +2742,46 @@ void build_h4(const iwx::DenseWeights &dh, Staged &S)
 	}
 }
 
+// 32-bit float input in linear light as doubles (Staged::in_f64): stored
+// channel k of every pixel through the transfer function of the intermediate
+// channel that reads it (src/imagew-main.c:355-364; channels no intermediate
+// channel converts -- alpha, or unread -- as stored).
+bool build_f64_input(iw_context *ctx, const iwamd_plan &p, Staged &S, bool in_elsewhere)
+{
+	if (in_elsewhere) {
+		iw_set_error(ctx, "Device-resident floating-point input needs a linear input colorspace");
+		return false;
+	}
+	const int nch = p.in_nch_physical;
+	std::vector<const struct iw_csdescr *> cs(nch, nullptr);
+	for (int c = 0; c < p.inter_nch; c++) {
+		const int nk = p.ich[c].cvt_to_grayscale ? 3 : 1;
+		for (int k = 0; k < nk; k++) {
+			const int ch = p.ich[c].input_channel + k;
+			if (ch >= nch || p.ich[c].in_cs.cstype == IW_CSTYPE_LINEAR) continue;
+			if (cs[ch] && (cs[ch]->cstype != p.ich[c].in_cs.cstype || cs[ch]->gamma != p.ich[c].in_cs.gamma)) {
+				iw_set_error(ctx, "Floating-point input channel read in two colorspaces");
+				return false;
+			}
+			cs[ch] = &p.ich[c].in_cs;
+		}
+	}
+	const bool tr = (p.orient_transform & 4) != 0;
+	const size_t rows = tr ? p.in_width : p.in_height, cols = tr ? p.in_height : p.in_width;
+	S.in_f64_bpr = cols * nch * 8;
+	S.in_f64.resize(rows * cols * nch);
+	const uint8_t *src = (const uint8_t *)p.in_pixels;
+	for (size_t y = 0; y < rows; y++)
+		for (size_t x = 0; x < cols; x++)
+			for (int ch = 0; ch < nch; ch++) {
+				float f;
+				memcpy(&f, src + y * p.in_bpr + (x * nch + ch) * 4, 4);
+				const double v = (double)f;
+				S.in_f64[(y * cols + x) * nch + ch] = cs[ch] ? iwx_x_to_linear(v, cs[ch]) : v;
+			}
+	return true;
+}
+
 // Build the host tables of a plan, choose its path and lay out its device
 // data (offsets from the job's base).  path: "band", "twopass" or null.
 bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path, bool out_elsewhere = false,
@@ -2743,16 +2791,15 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	// Floating-point input: 32-bit samples are read as stored; the reference
 	// reads any other stored depth as 0.0 (src/imagew-main.c:267-272), which
 	// every transfer function maps to 0.0.  A 32-bit sample in a non-linear
-	// colour space would need libm's pow per sample (src/imagew-main.c:
-	// 53-66, :359-365), which the device does not evaluate: such jobs are
-	// refused rather than run with other bits.
+	// colour space goes through libm's pow (src/imagew-main.c:53-66,
+	// :355-364), which the device does not evaluate: the host converts the
+	// image to linear doubles first, each stored channel with the colour
+	// space of the intermediate channel that reads it, and the kernels read
+	// those.
 	if (p.in_sampletype == IW_SAMPLETYPE_FLOATINGPOINT && p.in_bit_depth == 32) {
-		for (int c = 0; c < p.inter_nch; c++) {
-			if (p.ich[c].in_cs.cstype != IW_CSTYPE_LINEAR) {
-				iw_set_error(ctx, "Floating-point input needs a linear input colorspace on the GPU path");
-				return false;
-			}
-		}
+		bool nonlinear = false;
+		for (int c = 0; c < p.inter_nch; c++) nonlinear |= p.ich[c].in_cs.cstype != IW_CSTYPE_LINEAR;
+		if (nonlinear && !build_f64_input(ctx, p, S, in_elsewhere)) return false;
 	}
 	S.wv.resize(p.inter_nch);
 	S.wh.resize(p.inter_nch);
@@ -2877,7 +2924,7 @@ bool stage_job(iw_context *ctx, const iwamd_plan &p, Staged &S, const char *path
 	}
 
 	Layout L;
-	S.in_bytes = p.in_bpr * (size_t)phys_rows;
+	S.in_bytes = S.in_f64.empty() ? p.in_bpr * (size_t)phys_rows : S.in_f64.size() * sizeof(double);
 	S.in_elsewhere = in_elsewhere;
 	S.o_in = L.add<uint8_t>(in_elsewhere ? 0 : S.in_bytes);
 	S.o_inter = L.add<float>(S.band_rows ? 0 : (size_t)p.inter_nch * p.inter_height * p.inter_width);
@@ -2951,7 +2998,8 @@ bool upload_job(Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
 {
 	bool ok = true;
 	if (S.in_bytes && !S.in_elsewhere)
-		ok &= hipMemcpyAsync(base + S.o_in, p.in_pixels, S.in_bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+		ok &= hipMemcpyAsync(base + S.o_in, S.in_f64.empty() ? p.in_pixels : (const void *)S.in_f64.data(), S.in_bytes,
+			hipMemcpyHostToDevice, st) == hipSuccess;
 	const size_t t0 = S.o_tables;
 	S.pack.assign(S.bytes - t0, 0);
 	auto up = [&](size_t off, const void *src, size_t bytes) {
@@ -3015,7 +3063,8 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 {
 	memset(&D, 0, sizeof(D));
 	D.in = (const uint8_t *)(base + S.o_in);
-	D.in_bpr = p.in_bpr;
+	D.in_bpr = S.in_f64.empty() ? p.in_bpr : S.in_f64_bpr;
+	D.in_f64 = !S.in_f64.empty();
 	D.in_w = p.in_width;
 	D.in_h = p.in_height;
 	D.bit_depth = p.in_bit_depth;

@@ -113,8 +113,8 @@ MIFF_INPUTS = [
      lambda: miff(17, 15, True, False, 32, 24, profile=37,
                   extra=b"units=PixelsPerCentimeter\nresolution=37.80x42.50\nrendering-intent=Relative\n"
                         b"background-color=#ffff80000000\n"), 17, 15, True),
-    # No gamma attribute: the reference assumes sRGB (the GPU path refuses
-    # floating-point input in a non-linear colour space).
+    # No gamma attribute: the reference assumes sRGB and converts each float
+    # sample with libm (the GPU path converts the image on the host first).
     ("rgb32-srgb", lambda: miff(18, 14, False, False, 32, 25, gamma=False, lo=0.0, hi=1.0), 18, 14, False),
 ]
 
@@ -228,7 +228,7 @@ def test_pnm_through_gpu(prod, ref_codecs, name, make, w, h, kind, out):
     _assert_same(name, want, got)
 
 
-GPU_MIFF = [(n, f, w, h, s, o) for (n, f, w, h, lin) in MIFF_INPUTS if lin for s in ("same", "lanczos3")
+GPU_MIFF = [(n, f, w, h, s, o) for (n, f, w, h, _) in MIFF_INPUTS for s in ("same", "lanczos3")
             for o in ("miff", "pnm")]
 
 
@@ -237,13 +237,3 @@ GPU_MIFF = [(n, f, w, h, s, o) for (n, f, w, h, lin) in MIFF_INPUTS if lin for s
 def test_miff_through_gpu(prod, ref_codecs, name, make, w, h, kind, out):
     want, got = _pair(ref_codecs, prod, make(), "miff", out, _settings(kind, w, h), None)
     _assert_same(name, want, got)
-
-
-@pytest.mark.gpu
-def test_miff_srgb_float_refused_on_gpu(prod, ref_codecs):
-    """Float input in a non-linear colour space: the reference converts it
-    with libm per sample; the GPU path refuses it with an error."""
-    data = miff(18, 14, False, False, 32, 25, gamma=False, lo=0.0, hi=1.0)
-    want, got = _pair(ref_codecs, prod, data, "miff", "miff", _settings("same", 18, 14), None)
-    assert want[0] and not got[0]
-    assert "linear input colorspace" in got[2]

@@ -13,6 +13,9 @@ reference core (oracle/_ref/libiwref.so) on the same inputs:
 - floating-point input in a linear colour space (get_raw_sample_flt32,
   src/imagew-main.c:129-134, :267-272), and the input depths other than 32 that
   the reference reads as 0.0;
+- floating-point input in sRGB, Rec.709 and gamma colour spaces, which the
+  reference converts per sample with libm (src/imagew-main.c:53-66,
+  :355-364) and the host converts to linear doubles before the upload;
 - reduced input and output maximum colour codes (iw_set_input_max_color_code
   and iw_set_output_max_color_code, src/imagew-api.c:384, :442, reaching
   src/imagew-main.c:1982-2007 and :2145-2183);
@@ -29,6 +32,7 @@ from imageworsener_amd import api, synth
 pytestmark = pytest.mark.gpu
 
 RGB, RGBA, GRAY, GRAYA = cases.RGB, cases.RGBA, cases.GRAY, cases.GRAYA
+IW_VAL_CVT_TO_GRAYSCALE = 11  # include/imagew.h
 PROFILE_HDRI = 0x4000
 SAMPLETYPE_FLOAT = 1
 
@@ -179,17 +183,33 @@ def test_float_input(prod, ref, name, job, path):
     cases.assert_same(ref.run(job), prod.run(dict(job, options=[("amd:resample", path)])), name)
 
 
-def test_float_input_nonlinear_refused(prod, ref):
+def _float_nonlinear_jobs():
+    rng = np.random.default_rng(3)
+    rgb = (rng.random((20, 30, 3)) * 1.1 - 0.05).astype(np.float32)   # out-of-range samples too
+    rgba = rng.random((26, 19, 4)).astype(np.float32)
+    rgba[..., 3][rgba[..., 3] < 0.15] = 0.0
+    return [
+        ("rgb-f32-srgb", float_job(rgb, RGB, (15, 10), resize=api.resize_both("catrom"))),   # default input cs: sRGB
+        ("rgb-f32-rec709-fs", float_job(rgb, RGB, (41, 23), input_cs=(api.CS["rec709"], 0.0),
+                                        resize=api.resize_both("lanczos3"), dither=[(api.CH_ALL, 2, 0)])),
+        ("rgba-f32-gamma", float_job(rgba, RGBA, (33, 40), input_cs=(api.CS["gamma"], 1.8),
+                                     resize=api.resize_both("mitchell"))),
+        ("rgba-f32-srgb-gray-crop-orient", dict(float_job(rgba, RGBA, (12, 14), resize=api.resize_both("triangle"),
+                                                          values=[(IW_VAL_CVT_TO_GRAYSCALE, 1)]),
+                                                crop=(2, 3, 15, 20), reorient=6)),
+    ]
+
+
+@pytest.mark.parametrize("path", ["band", "twopass"])
+@pytest.mark.parametrize("name,job", _float_nonlinear_jobs(), ids=[n for n, _ in _float_nonlinear_jobs()])
+def test_float_input_nonlinear(prod, ref, name, job, path):
     """32-bit float input in a non-linear colour space: the reference converts
-    each sample with libm's pow (src/imagew-main.c:53-66, :359-365); this
-    library does not evaluate libm on the device and refuses the job with an
-    error instead of producing different bytes."""
-    rgb = np.random.default_rng(3).random((20, 30, 3)).astype(np.float32)
-    job = float_job(rgb, RGB, (15, 10), resize=api.resize_both("catrom"))   # default input cs: sRGB
-    assert ref.run(job)["ok"]
-    got = prod.run(job)
-    assert not got["ok"]
-    assert "linear" in got["error"], got["error"]
+    each sample with libm's pow (src/imagew-main.c:53-66, :355-364); the host
+    converts the image to linear doubles with the same expressions before the
+    upload, so the bytes are the reference's."""
+    want = ref.run(job)
+    assert want["ok"], want["error"]
+    cases.assert_same(want, prod.run(dict(job, options=[("amd:resample", path)])), name)
 
 
 # ---- reduced maximum colour codes -----------------------------------------
