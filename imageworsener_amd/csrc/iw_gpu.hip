@@ -76,6 +76,7 @@ struct DevIChan {
 	int input_channel, cvt_gray, output_channel, need_unassoc, early_bkgd;
 	double bkgd;
 	const double *lut[3];
+	double lut_lin_max;  // lut[0] is v / lut_lin_max (a linear channel), else 0
 	DevWeights v, h;
 };
 
@@ -103,7 +104,8 @@ struct DevPlan {
 	unsigned int orient;
 	int sx, sy, iw, ih;
 	int alpha_in_index;
-	const double *alpha_lut;
+	const double *alpha_lut;  // v / alpha_max (linear)
+	double alpha_max;
 	int gsf;
 	double gsw[3];
 	int inter_nch, inter_alpha;
@@ -129,6 +131,7 @@ struct DevPlan {
 	size_t out_bytes;
 	int apply_bkgd, late, checker, check_size, ox, oy, bk_has_trans;
 	int v_shared;  // every channel has the same vertical weights (dense form) in WEIGHTS mode
+	int vband_r;   // two-pass form: output rows per thread of vband_kernel (0: vpass_kernel)
 	int h_shared;  // the same for the horizontal weights
 	double bk1a, bk2a;
 	// Chosen-precision form: the shared horizontal list by 4-column blocks.
@@ -762,6 +765,10 @@ struct FastCin {
 	int et[IWAMD_MAX_CH];    // rows outside the image read edge[c]
 	int ab;                  // byte of the alpha sample
 	const double *alut;      // its table (channel 0's when alpha is not needed)
+	// 16-bit direct colour-in: a linear table's divisor (channel c, alpha),
+	// 0 for the others.  A linear channel's value is (double)v / max, which
+	// the device divides exactly (the table holds the same quotients).
+	double lmax[IWAMD_MAX_CH], amax;
 	int lean;                // no channel in mode 2 and none with a transparent edge
 };
 
@@ -779,6 +786,7 @@ __host__ __device__ inline int fast_cin_depth(const DevPlan &P, FastCin &F)
 		F.bkgd[c] = ic.bkgd;
 		F.edge[c] = ic.v.edge;
 		F.et[c] = ic.v.edge_transparent;
+		F.lmax[c] = ic.lut_lin_max;
 		need_a |= F.mode[c] != 0;
 	}
 	F.lean = 1;
@@ -786,10 +794,12 @@ __host__ __device__ inline int fast_cin_depth(const DevPlan &P, FastCin &F)
 		if (F.mode[c] == 2 || F.et[c]) F.lean = 0;
 	F.ab = F.chb[0];
 	F.alut = F.lut[0];
+	F.amax = F.lmax[0];
 	if (need_a) {
 		if (P.alpha_in_index >= P.nch_phys || !P.alpha_lut) return 0;
 		F.ab = P.alpha_in_index * bps;
 		F.alut = P.alpha_lut;
+		F.amax = P.alpha_max;
 	}
 	int rx, ry;
 	long long o[3];
@@ -802,9 +812,20 @@ __host__ __device__ inline int fast_cin_depth(const DevPlan &P, FastCin &F)
 	F.dx = o[1] - o[0];
 	F.dy = o[2] - o[0];
 	// Four 8-bit samples per pixel, every pixel 4-byte aligned: one 32-bit
-	// load per pixel ("depth" 32).
+	// load per pixel ("depth" 32).  Four 16-bit samples, 8-byte aligned, lean:
+	// one 64-bit load ("depth" 64).
 	if (P.bit_depth == 8 && pb == 4 && ((F.base | F.dx | F.dy) & 3) == 0 && ((uintptr_t)P.in & 3) == 0) return 32;
+	if (P.bit_depth == 16 && pb == 8 && F.lean && ((F.base | F.dx | F.dy) & 7) == 0 && ((uintptr_t)P.in & 7) == 0)
+		return 64;
 	return P.bit_depth;
+}
+
+// The plain form's vertical passes keep the tables for linear channels too:
+// with its FP64 sums per band row, a division costs more there than a gather.
+__host__ __device__ inline void fast_cin_tables_only(FastCin &F)
+{
+	for (int c = 0; c < IWAMD_MAX_CH; c++) F.lmax[c] = 0.0;
+	F.amax = 0.0;
 }
 
 template <int DEPTH>
@@ -840,15 +861,31 @@ __device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, l
 #pragma unroll
 		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = (word >> (8 * F.chb[c])) & 0xffu;
 		araw = (word >> (8 * F.ab)) & 0xffu;
+	} else if (DEPTH == 64) {
+		// Big-endian 16-bit samples at even byte offsets of the pixel's word.
+		const unsigned long long word = *(const __attribute__((address_space(1))) unsigned long long *)(in + off);
+		auto be16 = [&](int b) {
+			const unsigned int h = (unsigned int)(word >> (8 * b)) & 0xffffu;
+			return ((h & 0xffu) << 8) | (h >> 8);
+		};
+#pragma unroll
+		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = be16(F.chb[c]);
+		araw = be16(F.ab);
 	} else {
 #pragma unroll
 		for (int c = 0; c < IWAMD_MAX_CH; c++) raw[c] = fast_raw<DEPTH>(in, off + F.chb[c]);
 		araw = fast_raw<DEPTH>(in, off + F.ab);
 	}
-	const double a = ((const g_cf64 *)F.alut)[araw];
+	// 16-bit tables are 512 KiB, so a gather costs a cache line per lane; a
+	// linear channel divides instead.
+	auto value = [&](unsigned int v, const double *lut, double lmax) {
+		if ((DEPTH == 16 || DEPTH == 64) && lmax > 0.0) return (double)v / lmax;
+		return ((const g_cf64 *)lut)[v];
+	};
+	const double a = value(araw, F.alut, F.amax);
 #pragma unroll
 	for (int c = 0; c < IWAMD_MAX_CH; c++) {
-		const double x = ((const g_cf64 *)F.lut[c])[raw[c]];
+		const double x = value(raw[c], F.lut[c], F.lmax[c]);
 		if (LEAN) {
 			v[c] = (T)(x * (F.mode[c] ? a : 1.0));
 			continue;
@@ -867,11 +904,12 @@ __device__ __forceinline__ void fast_fetch(const DevPlan &P, const FastCin &F, l
 // load issued before the first is used.  Each accumulator still gets its terms
 // in list order, with the same FP64 operations as colour_in and the generic
 // pass.
+// Columns x0, x0 + xstep, ... below W; row r of channel c of column x goes to
+// out[c * cstride + r * W + x].
 template <int R, int DEPTH, bool LEAN>
 __device__ __forceinline__ void band_vertical_direct(const DevPlan &P, const FastCin &F, int j0, int rows,
-	const double *wd, float *inter)
+	const double *wd, float *out, size_t cstride, int x0, int xstep)
 {
-	const int tid = threadIdx.x;
 	const int W = P.W, C = P.inter_nch;
 	const DevWeights &dv = P.ic[0].v;
 	int p_lo = 0x7fffffff, p_hi = -0x7fffffff;
@@ -882,7 +920,7 @@ __device__ __forceinline__ void band_vertical_direct(const DevPlan &P, const Fas
 		p_hi = max(p_hi, dv.dfirst[j0 + r] + n - 1);
 	}
 	const int K = (p_hi >= p_lo) ? p_hi - p_lo + 1 : 0;
-	for (int x = tid; x < W; x += blockDim.x) {
+	for (int x = x0; x < W; x += xstep) {
 		const long long cb = F.base + (long long)x * F.dx;
 		double acc[IWAMD_MAX_CH][R];
 #pragma unroll
@@ -909,9 +947,10 @@ __device__ __forceinline__ void band_vertical_direct(const DevPlan &P, const Fas
 			if (c >= C) break;
 #pragma unroll
 			for (int r = 0; r < R; r++) {
+				if (r >= rows) break;
 				double a = acc[c][r];
 				if (P.intclamp) a = clamp01(a);
-				inter[((size_t)c * R + r) * W + x] = (float)a;
+				out[c * cstride + (size_t)r * W + x] = (float)a;
 			}
 		}
 	}
@@ -929,6 +968,7 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 
 	FastCin F;
 	const int depth = P.v_shared ? fast_cin_depth(P, F) : 0;
+	fast_cin_tables_only(F);
 	for (int c = 0; c < (depth ? 1 : C); c++) {
 		const DevIChan &ic = P.ic[c];
 		const DevWeights &dv = ic.v;
@@ -954,15 +994,19 @@ __device__ __forceinline__ void band_body(const DevPlan &P, int j0, unsigned cha
 		__syncthreads();
 		if (depth) {
 			// Channel 0's weights are every channel's.
+			const size_t cs = (size_t)R * W;
+			const int x0 = threadIdx.x, xs = blockDim.x;
 			if (depth == 32) {
-				if (F.lean) band_vertical_direct<R, 32, true>(P, F, j0, rows, wd, inter);
-				else band_vertical_direct<R, 32, false>(P, F, j0, rows, wd, inter);
+				if (F.lean) band_vertical_direct<R, 32, true>(P, F, j0, rows, wd, inter, cs, x0, xs);
+				else band_vertical_direct<R, 32, false>(P, F, j0, rows, wd, inter, cs, x0, xs);
 			} else if (depth == 8) {
-				if (F.lean) band_vertical_direct<R, 8, true>(P, F, j0, rows, wd, inter);
-				else band_vertical_direct<R, 8, false>(P, F, j0, rows, wd, inter);
+				if (F.lean) band_vertical_direct<R, 8, true>(P, F, j0, rows, wd, inter, cs, x0, xs);
+				else band_vertical_direct<R, 8, false>(P, F, j0, rows, wd, inter, cs, x0, xs);
+			} else if (depth == 64) {
+				band_vertical_direct<R, 64, true>(P, F, j0, rows, wd, inter, cs, x0, xs);
 			} else {
-				if (F.lean) band_vertical_direct<R, 16, true>(P, F, j0, rows, wd, inter);
-				else band_vertical_direct<R, 16, false>(P, F, j0, rows, wd, inter);
+				if (F.lean) band_vertical_direct<R, 16, true>(P, F, j0, rows, wd, inter, cs, x0, xs);
+				else band_vertical_direct<R, 16, false>(P, F, j0, rows, wd, inter, cs, x0, xs);
 			}
 			continue;
 		}
@@ -1080,6 +1124,73 @@ __global__ void __launch_bounds__(256) band_kernel(const DevPlan *__restrict__ p
 		if (threadIdx.x == 0) next_item = (int)gridDim.x + atomicAdd(counter, 1);
 		__syncthreads();
 		it = next_item;
+	}
+}
+
+// The two-pass form's vertical pass by bands of R intermediate rows, for
+// plans whose channels share one vertical list (DevPlan::v_shared) and take
+// the direct colour-in: each thread keeps one column's R rows per channel and
+// visits the union of their input positions once, so every input sample is
+// read and converted once per band instead of once per tap.  Each row's
+// accumulator gets its list's terms in list order plus +0.0 for the band's
+// other positions, which leaves it unchanged -- the same sums as vpass_kernel
+// (src/imagew-main.c:831-927).  Grid: (ceil(W / 256), ceil(Hp / R)).
+template <int R>
+__global__ void __launch_bounds__(256) vband_kernel(DevPlan P)
+{
+	__shared__ double wd[R * BAND_MAX_K];
+	const int j0 = blockIdx.y * R;
+	const int rows = min(R, P.Hp - j0);
+	const DevWeights &dv = P.ic[0].v;
+	int p_lo = 0x7fffffff, p_hi = -0x7fffffff;
+	for (int r = 0; r < rows; r++) {
+		const int n = dv.dcount[j0 + r];
+		if (n == 0) continue;
+		p_lo = min(p_lo, dv.dfirst[j0 + r]);
+		p_hi = max(p_hi, dv.dfirst[j0 + r] + n - 1);
+	}
+	const int K = (p_hi >= p_lo) ? p_hi - p_lo + 1 : 0;
+	for (int t = threadIdx.x; t < R * BAND_MAX_K; t += blockDim.x) {
+		const int r = t / BAND_MAX_K, k = t % BAND_MAX_K;
+		double w = 0.0;
+		if (r < rows && k < K) {
+			const int kk = p_lo + k - dv.dfirst[j0 + r];
+			if (kk >= 0 && kk < dv.dcount[j0 + r]) w = dv.dw[dv.doff[j0 + r] + kk];
+		}
+		wd[t] = w;
+	}
+	__syncthreads();
+	FastCin F;
+	const int depth = fast_cin_depth(P, F);
+	fast_cin_tables_only(F);
+	const int x0 = blockIdx.x * blockDim.x + threadIdx.x, xs = gridDim.x * blockDim.x;
+	float *out = P.inter + (size_t)j0 * P.W;
+	const size_t cs = (size_t)P.Hp * P.W;
+	if (depth == 32) {
+		if (F.lean) band_vertical_direct<R, 32, true>(P, F, j0, rows, wd, out, cs, x0, xs);
+		else band_vertical_direct<R, 32, false>(P, F, j0, rows, wd, out, cs, x0, xs);
+	} else if (depth == 8) {
+		if (F.lean) band_vertical_direct<R, 8, true>(P, F, j0, rows, wd, out, cs, x0, xs);
+		else band_vertical_direct<R, 8, false>(P, F, j0, rows, wd, out, cs, x0, xs);
+	} else if (depth == 64) {
+		band_vertical_direct<R, 64, true>(P, F, j0, rows, wd, out, cs, x0, xs);
+	} else {
+		if (F.lean) band_vertical_direct<R, 16, true>(P, F, j0, rows, wd, out, cs, x0, xs);
+		else band_vertical_direct<R, 16, false>(P, F, j0, rows, wd, out, cs, x0, xs);
+	}
+}
+
+// Queue the two-pass form's vertical pass of a plan: vband_kernel<vband_r>
+// where the plan takes it, else vpass_kernel.
+void enqueue_vpass(const DevPlan &D, hipStream_t st)
+{
+	const dim3 blk(256);
+	switch (D.vband_r) {
+	case 8: hipLaunchKernelGGL(vband_kernel<8>, dim3((D.W + 255) / 256, (D.Hp + 7) / 8), blk, 0, st, D); break;
+	case 4: hipLaunchKernelGGL(vband_kernel<4>, dim3((D.W + 255) / 256, (D.Hp + 3) / 4), blk, 0, st, D); break;
+	case 2: hipLaunchKernelGGL(vband_kernel<2>, dim3((D.W + 255) / 256, (D.Hp + 1) / 2), blk, 0, st, D); break;
+	case 1: hipLaunchKernelGGL(vband_kernel<1>, dim3((D.W + 255) / 256, D.Hp), blk, 0, st, D); break;
+	default: hipLaunchKernelGGL(vpass_kernel, dim3((D.W + 255) / 256, D.Hp), blk, 0, st, D); break;
 	}
 }
 
@@ -1559,7 +1670,8 @@ __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__re
 			if (depth == 32) fast_tile<32, true, true>(P, F, w, S, ES, true, rows);
 			else fast_tile<8, true, true>(P, F, w, S, ES, true, rows);
 		} else if constexpr (VAR == FAST_LEAN16) {
-			fast_tile<16, true, true>(P, F, w, S, ES, true, rows);
+			if (depth == 64) fast_tile<64, true, true>(P, F, w, S, ES, true, rows);
+			else fast_tile<16, true, true>(P, F, w, S, ES, true, rows);
 		} else {
 			switch (depth) {
 			case 32:
@@ -1570,6 +1682,7 @@ __device__ __forceinline__ void fast_band(const DevPlan &P, const FastItem *__re
 				if (F.lean) fast_tile<8, true, false>(P, F, w, S, ES, simple, rows);
 				else fast_tile<8, false, false>(P, F, w, S, ES, simple, rows);
 				break;
+			case 64: fast_tile<64, true, false>(P, F, w, S, ES, simple, rows); break;
 			case 16:
 				if (F.lean) fast_tile<16, true, false>(P, F, w, S, ES, simple, rows);
 				else fast_tile<16, false, false>(P, F, w, S, ES, simple, rows);
@@ -3044,6 +3157,22 @@ bool upload_job(Staged &S, const iwamd_plan &p, char *base, hipStream_t st)
 	return ok;
 }
 
+// Output rows per thread of vband_kernel: the most (8, 4, 2, 1) whose bands
+// reach at most BAND_MAX_K input positions.
+int vband_rows(const iwamd_plan &p, const iwx::DenseWeights &d)
+{
+	for (int R : {8, 4, 2, 1}) {
+		bool ok = true;
+		for (int j0 = 0; j0 < p.inter_height && ok; j0 += R) {
+			int lo, n;
+			dense_span(d, j0, std::min(R, p.inter_height - j0), lo, n);
+			ok = n <= BAND_MAX_K;
+		}
+		if (ok) return R;
+	}
+	return 0;
+}
+
 // Kernel arguments of a staged job placed at base.
 // Pixels each segment's walk takes before its own first pixel.  Walks with
 // equal levels close their gap by the in-row weights at every pixel, so the
@@ -3077,6 +3206,7 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 	D.ih = p.input_h;
 	D.alpha_in_index = p.in_has_alpha ? p.in_alpha_index : 1000;
 	D.alpha_lut = S.alpha_lut >= 0 ? (const double *)(base + S.o_lut[S.alpha_lut]) : nullptr;
+	D.alpha_max = (p.in_has_alpha && p.in_alpha_index < 4) ? p.in_ch_maxcolorcode[p.in_alpha_index] : 0.0;
 	D.gsf = p.grayscale_formula;
 	for (int k = 0; k < 3; k++) D.gsw[k] = p.grayscale_weight[k];
 	D.inter_nch = p.inter_nch;
@@ -3106,6 +3236,11 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 		d.bkgd = s.bkgd_color_lin;
 		for (int k = 0; k < 3; k++)
 			d.lut[k] = (S.in_int && S.ic_lut[c][k] >= 0) ? (const double *)(base + S.o_lut[S.ic_lut[c][k]]) : nullptr;
+		// The table of a linear channel holds v / max (build_input_lut), max as
+		// stage_job chose it (the channel's own on the slow path).
+		d.lut_lin_max = 0.0;
+		if (d.lut[0] && s.in_cs.cstype == IW_CSTYPE_LINEAR && s.input_channel < p.in_nch_physical)
+			d.lut_lin_max = p.in_ch_slow[s.input_channel] ? p.in_ch_maxcolorcode[s.input_channel] : p.in_maxcolorcode;
 		auto fill = [&](const iwx::WeightList &wl, const WOff &o, DevWeights &dw, int dim) {
 			dw.mode = wl.mode;
 			dw.start = (const int *)(base + o.start);
@@ -3160,9 +3295,7 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 			d.ix = S.qt[o].ix.empty() ? nullptr : (const unsigned short *)(base + q.ix);
 			d.D = S.qt[o].D.empty() ? nullptr : (const double *)(base + q.D);
 			d.code_is_level = S.qt[o].code_is_level ? 1 : 0;
-			d.ix_le2 = !S.qt[o].ix.empty();
-			for (size_t b = 0; b + 1 < S.qt[o].ix.size(); b++)
-				if (S.qt[o].ix[b + 1] - S.qt[o].ix[b] > 2) d.ix_le2 = 0;
+			d.ix_le2 = !S.qt[o].ix.empty() && S.qt[o].ix_max <= 2;
 			d.ix_log2 = S.qt[o].ix_log2;
 			d.prng_row = (const uint64_t *)(base + q.prow);
 			d.prng_col = (const uint64_t *)(base + q.pcol);
@@ -3183,6 +3316,9 @@ void fill_dev_plan(const Staged &S, const iwamd_plan &p, char *base, DevPlan &D)
 	D.bk_has_trans = p.bkgd_has_transparency;
 	D.bk1a = p.bkgd1alpha;
 	D.bk2a = p.bkgd2alpha;
+	D.vband_r = 0;
+	FastCin F;
+	if (!S.band_rows && S.fast.empty() && D.v_shared && fast_cin_depth(D, F)) D.vband_r = vband_rows(p, S.dv[0]);
 }
 
 // ---- work sets ---------------------------------------------------------------
@@ -3263,7 +3399,7 @@ int fast_variant(const Staged &S, const iwamd_plan &p)
 	EmitSimple ES;
 	const int depth = fast_cin_depth(D, F);
 	if (!depth || !F.lean || !emit_simple_ctx(D, ES)) return FAST_GENERIC;
-	if (depth == 16) return FAST_LEAN16;
+	if (depth == 16 || depth == 64) return FAST_LEAN16;
 	return fast_rgba8_ok(D, ES, F) ? FAST_RGBA8 : FAST_LEAN8;
 }
 
@@ -3427,7 +3563,7 @@ void ws_enqueue_resample(const WorkSet &W, hipStream_t st, bool persistent)
 		}
 	for (int jx : W.twopass) {
 		const DevPlan &D = W.plans[jx];
-		hipLaunchKernelGGL(vpass_kernel, dim3((D.W + 255) / 256, D.Hp), dim3(256), 0, st, D);
+		enqueue_vpass(D, st);
 		hipLaunchKernelGGL(hpass_kernel, dim3((D.out_w + 255) / 256, D.out_h), dim3(256), 0, st, D);
 	}
 }
@@ -3725,7 +3861,7 @@ void enqueue_job(const WorkSet &W, const DevPlan &D, hipStream_t st, hipEvent_t 
 		ws_enqueue_resample(W, st, false);
 		if (ev) hipEventRecord(ev[1], st);
 	} else {
-		hipLaunchKernelGGL(vpass_kernel, dim3((D.W + 255) / 256, D.Hp), dim3(256), 0, st, D);
+		enqueue_vpass(D, st);
 		if (ev) hipEventRecord(ev[1], st);
 		hipLaunchKernelGGL(hpass_kernel, dim3((D.out_w + 255) / 256, D.out_h), dim3(256), 0, st, D);
 	}

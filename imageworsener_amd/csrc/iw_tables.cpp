@@ -387,6 +387,7 @@ QuantTable build_quant_table_uncached(const iwamd_ochan &oc, bool want_nc, bool 
 		q.ix[b] = (uint16_t)m;
 	}
 	q.ix[NB] = (uint16_t)n;
+	for (int b = 0; b < NB; b++) q.ix_max = std::max(q.ix_max, (int)(q.ix[b + 1] - q.ix[b]));
 	return q;
 }
 }  // namespace

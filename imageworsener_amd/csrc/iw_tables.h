@@ -60,6 +60,7 @@ WeightList build_weight_list(const iwamd_resize &rs, int channeltype, double edg
 // floor(0.5000000001 + k*(M/P))), lin[k] its value in linear light.
 struct QuantTable {
 	int P = 0;                   // number of levels - 1
+	int ix_max = 0;              // most entries in one bin of ix
 	std::vector<double> T, U, lin, code;
 	std::vector<double> nc;      // nearest-colour midpoints (255), when used
 	// The no-dither quantiser in one search (src/imagew-main.c:669-736):
