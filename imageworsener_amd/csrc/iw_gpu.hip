@@ -3897,10 +3897,13 @@ extern "C" int iwamd_job_sync(struct iwamd_job *J)
 extern "C" int iwamd_job_kernel_ms(struct iwamd_job *J, double *ms, int n)
 {
 	if (!J->ran) return 0;
+	// A fused resample launches nothing between the first two events: its
+	// horizontal-pass slot is 0 (the events alone read a few microseconds).
+	const bool fused = ((const WorkSet *)J->ws)->twopass.empty();
 	int k = 0;
 	for (; k < 3 && k < n; k++) {
 		float t = 0.0f;
-		hipEventElapsedTime(&t, J->ev[k], J->ev[k + 1]);
+		if (!(k == 1 && fused)) hipEventElapsedTime(&t, J->ev[k], J->ev[k + 1]);
 		ms[k] = t;
 	}
 	return k;

@@ -15,9 +15,9 @@
  *
  * 2. Precision mode.  IWAMD_PRECISION_PLAIN (the default) is the FP64 form
  *    whose output bytes equal the reference's.  IWAMD_PRECISION_FAST resamples
- *    on FP16 MFMA with split hi/lo operands (FP32 accumulate) and quantises in
- *    FP64, for 8- and 16-bit integer outputs; its error against the plain form
- *    is bounded and reported, not zero.
+ *    on FP32 MFMA (FP32 operands and sums) and quantises in FP64, for 8- and
+ *    16-bit integer outputs; its error against the plain form is bounded and
+ *    reported, not zero.
  *
  * 3. Batches and a request loop (iwamd_batch_*, iwamd_server_*).
  */
@@ -221,15 +221,16 @@ IW_EXPORT(int) iwamd_job_sync(struct iwamd_job *job);
 IW_EXPORT(int) iwamd_job_finish(struct iwamd_job *job);
 IW_EXPORT(void) iwamd_job_destroy(struct iwamd_job *job);
 /* Kernel times of the last run, ms, from HIP events on the job's stream:
- * [0] vertical pass, [1] horizontal pass + quantise, [2] error diffusion.
+ * [0] vertical pass (or the whole fused resample), [1] horizontal pass +
+ * quantise (0 when the resample is one fused kernel), [2] error diffusion.
  * Returns the number of entries filled (at most n). */
 IW_EXPORT(int) iwamd_job_kernel_ms(struct iwamd_job *job, double *ms, int n);
 /* Output samples (pixels) and algorithmic HBM bytes (input + output image)
  * of one run. */
 IW_EXPORT(void) iwamd_job_sizes(struct iwamd_job *job, double *out_pixels, double *alg_bytes);
 /* Useful floating-point operations of a run's resampling: a multiply and an
- * add per listed tap of both passes, per channel (the chosen form's split
- * operands execute three products for each). */
+ * add per listed tap of both passes, per channel (the chosen form's dense
+ * band blocks also multiply the zeros around each list). */
 IW_EXPORT(double) iwamd_job_flops(struct iwamd_job *job);
 
 /* Batches.  iwamd_batch_add() runs the host decisions of a context

@@ -1,4 +1,4 @@
-"""The chosen-precision form (IWAMD_PRECISION_FAST: FP16 MFMA resampling,
+"""The chosen-precision form (IWAMD_PRECISION_FAST: FP32 MFMA resampling,
 FP32 sums, FP64 quantisation) against the plain FP64 form on the GPU, whose
 bytes equal the reference's.  Bar: at most 1 LSB on 8-bit outputs without
 error diffusion (ordered and random dither included: a decision can move by
@@ -79,8 +79,8 @@ def _jobs16():
 
 @pytest.mark.parametrize("name,job", _jobs16(), ids=lambda v: v if isinstance(v, str) else "")
 def test_fast_16bit(prod, name, job):
-    """16-bit outputs: the split operands keep the resampled value within a
-    small fraction of a 16-bit step, so codes differ by at most 1 LSB."""
+    """16-bit outputs: FP32 operands and sums keep the resampled value within
+    a small fraction of a 16-bit step, so codes differ by at most 1 LSB."""
     a, b = prod.run(plain(job)), prod.run(fast(job))
     assert a["bit_depth"] == 16
     mx, mean = lsb_stats(a, b)

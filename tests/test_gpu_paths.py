@@ -276,7 +276,7 @@ def _chosen_jobs():
 
 @pytest.mark.parametrize("name,job", _chosen_jobs(), ids=[n for n, _ in _chosen_jobs()])
 def test_chosen_form_vs_reference(prod, ref, name, job):
-    """The chosen form (FP16 MFMA resampling, FP32 sums) against the
+    """The chosen form (FP32 MFMA resampling) against the
     reference core's bytes: at most 1 LSB on 8-bit outputs without dither."""
     want = ref.run(dict(job, allow_opt=NO_OPT))
     got = prod.run(dict(job, allow_opt=NO_OPT, precision=api.PRECISION_FAST))
