@@ -64,6 +64,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="CPU baseline: seconds of work per leg (one thread; all cores)")
     p.add_argument("--images", type=int, default=4096, help="c3: images in the batch")
+    p.add_argument("--c4-max-side", type=int, default=8192, help="c4: largest input side (the configuration's 8192)")
     p.add_argument("--size", type=int, default=4096, help="c1: input side")
     p.add_argument("--out", type=int, default=1000, help="c1: output side")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -523,17 +524,37 @@ def bench_c2(a):
     print(json.dumps(line), flush=True)
 
 
+def _c4_image(args):
+    from imageworsener_amd import synth
+    w, h, depth, seed = args
+    img = synth.gratings_rgb(w, h, depth, seed)
+    return img if depth == 8 else synth.be16(img)
+
+
+def c4_requests(n, max_side):
+    """The serving requests (cases.c4_requests), images generated in forked
+    worker processes before the GPU is used."""
+    from imageworsener_amd import api, synth
+    lst = synth.c4_list(n, 5, 256, max_side)
+    with mp.get_context("fork").Pool(host_procs()) as pool:
+        imgs = pool.map(_c4_image, [(w, h, d, s) for (w, h, d, _, _, _, s, _) in lst], chunksize=1)
+    return [(api.image_job(pix, w, h, api.IMGTYPE["RGB"], d, canvas=(ow, oh), resize=api.resize_both(f),
+                           dither=[(api.CH_ALL, 2, 0)], depth=8), gap)
+            for pix, (w, h, d, ow, oh, f, _, gap) in zip(imgs, lst)]
+
+
 def bench_c4(a):
-    """Serving: requests (RGB 8/16-bit, sides log-uniform 256..2048, one scale
+    """Serving: requests (RGB 8/16-bit, sides log-uniform 256..8192, one scale
     0.1..1.5, Lanczos3 or Mitchell, 8-bit output with Floyd-Steinberg) through
     the request loop.  Saturation throughput first (all submitted at once),
     then Poisson arrivals at 70% of it for p50/p99 latency."""
     from imageworsener_amd import api
-    import cases
-    lib = api.Library()
     n = max(a.steps, 8) * 12
-    log("generating %d serving requests" % n)
-    reqs = cases.c4_requests(count=n, lo_side=256, hi_side=2048)
+    log("generating %d serving requests (sides 256..%d, %d processes)" % (n, a.c4_max_side, host_procs()))
+    t0 = time.perf_counter()
+    reqs = c4_requests(n, a.c4_max_side)
+    log("generated in %.1f s" % (time.perf_counter() - t0))
+    lib = api.Library()
     prec = api.PRECISION_FAST if a.precision == "fast" else api.PRECISION_PLAIN
     form = "fast" if prec == api.PRECISION_FAST else "plain"
     reqs = [(dict(j, precision=prec), g) for j, g in reqs]
@@ -570,8 +591,9 @@ def bench_c4(a):
     line = {"metric": METRIC, "value": round(out_px / 1e6 / (span / 1e3), 3), "unit": "Mpix/s", "n_gpus": 1,
             "steps": n, "warmup": 0, "ms_per_step": round(span / n, 4), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[form], "data": "synthetic",
-            "config": {"workload": "C4 (sides 256..2048): %d requests, RGB 8/16-bit, x0.1..1.5, "
-                                   "lanczos3/mitchell, 8-bit FS; request loop with continuous batching" % n,
+            "config": {"workload": "C4 (sides 256..%d): %d requests, RGB 8/16-bit, x0.1..1.5, "
+                                   "lanczos3/mitchell, 8-bit FS; request loop with continuous batching"
+                                   % (a.c4_max_side, n),
                        "global_batch": n, "precision": form},
             "latency_ms": {"rate_per_s": round(rate, 2), "p50": round(lat[len(lat) // 2], 3),
                            "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3), "max": round(lat[-1], 3)},
