@@ -169,10 +169,11 @@ IW_EXPORT(int) iwamd_tune_entries(struct iwamd_tune_entry *out, int max);
 
 /* The error-diffusion walk's decision for samples s[0..n) in [0, 1]: err[i]
  * and the output level level[i] as the walk computes them, through the level
- * records and bin index it keeps in LDS (tab = 1) or the bisection it runs
- * over the thresholds for channels with more than 257 levels (tab = 0).
- * Returns 1, 0 when tab = 1 and the channel has more levels than the records
- * hold, -1 for a nearest-colour plan.  No counterpart in the reference: it
+ * records it keeps in registers for channels of at most 4 levels (tab = 2),
+ * the level records and bin index it keeps in LDS (tab = 1) or the bisection
+ * it runs over the thresholds for channels with more than 257 levels
+ * (tab = 0).  Returns 1, 0 when the channel has more levels than the chosen
+ * records hold, -1 for a nearest-colour plan.  No counterpart in the reference: it
  * checks the device's restatement of the decision at
  * src/imagew-main.c:712-748. */
 IW_EXPORT(int) iwamd_ed_decide_check(const struct iwamd_ochan *och, int tab, int n, const double *s, double *err,

@@ -104,9 +104,9 @@ def test_ed_decide(prod, name, cstype, gamma, maxcc, cc):
     """The walk's decision (iwamd_ed_decide_check: the device's ed_decide run
     on the host) gives the reference's error and output level for samples
     at, below and above every threshold, exactness bound and round-up point,
-    at the level index's bin edges, and at random; through the LDS level
-    records (up to 257 levels) and through the bisection over the
-    thresholds."""
+    at the level index's bin edges, and at random; through the records in
+    registers (up to 4 levels), the LDS level records (up to 257 levels) and
+    the bisection over the thresholds."""
     P, T, U, lin, M = _tables(prod, cstype, gamma, maxcc, cc)
     lay = api._PlanLayout.get()
     och = lay.OChan()
@@ -139,11 +139,11 @@ def test_ed_decide(prod, name, cstype, gamma, maxcc, cc):
         k = _level(s, T)
         err, ceil_ = _ref_error(s, k, U, lin) if k < P else (0.0, False)
         want.append((err, k + 1 if ceil_ else k))
-    for tab in (1, 0):
+    for tab in (2, 1, 0):
         e_arr = (ctypes.c_double * n)()
         l_arr = (ctypes.c_int * n)()
         r = fn(ctypes.byref(och), tab, n, s_arr, e_arr, l_arr)
-        if tab and P > 256:
+        if (tab == 1 and P > 256) or (tab == 2 and P > 3):
             assert r == 0
             continue
         assert r == 1
