@@ -3,9 +3,10 @@ reference core (oracle/_ref/libiwref.so) on the same inputs:
 
 - error diffusion on output rows wider than 4096 pixels, where the error rows
   live in HBM instead of LDS (errdiff_kernel<*, *, false>): Floyd-Steinberg
-  and the x+2 matrices, 8-bit (decision tables in LDS) and 16-bit outputs
-  (tables in HBM) -- the reference's scatter is src/imagew-main.c:497-567 and
-  its row rotation :1124-1136;
+  and the x+2 matrices, 8-bit and 5-level outputs (decision records in LDS),
+  4- and 2-level outputs (records in registers) and 16-bit outputs (tables in
+  HBM) -- the reference's scatter is src/imagew-main.c:497-567 and its row
+  rotation :1124-1136;
 - the full 2048^2 -> 3072^2 Gray16 image with ordered, random (seed 3) and
   Jarvis-Judice-Ninke dithering to 4 levels;
 - floating-point output (IW_PROFILE_HDRI, put_raw_sample_flt32,
@@ -54,10 +55,12 @@ def _wide_jobs():
         jobs.append(("g16-5000x9-" + d, api.image_job(g16, 160, 40, GRAY, 16, canvas=(5000, 9),
                                                       resize=api.resize_both("catrom"),
                                                       dither=[(api.CH_ALL, fam, sub)])))
-        jobs.append(("g16-5000x9-cc5-" + d, api.image_job(g16, 160, 40, GRAY, 16, canvas=(5000, 9),
-                                                          resize=api.resize_both("catrom"),
-                                                          dither=[(api.CH_ALL, fam, sub)],
-                                                          color_count=[(api.CH_ALL, 5)])))
+        # 5 levels: records in LDS; 4 and 2 levels: records in registers.
+        for cc in (5, 4, 2):
+            jobs.append(("g16-5000x9-cc%d-%s" % (cc, d), api.image_job(g16, 160, 40, GRAY, 16, canvas=(5000, 9),
+                                                                      resize=api.resize_both("catrom"),
+                                                                      dither=[(api.CH_ALL, fam, sub)],
+                                                                      color_count=[(api.CH_ALL, cc)])))
     for d in ("f", "jjn"):
         fam, sub = api.DITHER[d]
         jobs.append(("rgb16-4600x6-" + d, api.image_job(rgb16, 200, 30, RGB, 16, canvas=(4600, 6),
