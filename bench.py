@@ -65,6 +65,8 @@ def parse():
                    help="CPU baseline: seconds of work per leg (one thread; all cores)")
     p.add_argument("--images", type=int, default=4096, help="c3: images in the batch")
     p.add_argument("--c4-max-side", type=int, default=8192, help="c4: largest input side (the configuration's 8192)")
+    p.add_argument("--c4-requests", type=int, default=1000, help="c4: requests per run (the configuration's 1000)")
+    p.add_argument("--c4-streams", type=int, default=8, help="c4: the request loop's streams (and HIP hardware queues)")
     p.add_argument("--size", type=int, default=4096, help="c1: input side")
     p.add_argument("--out", type=int, default=1000, help="c1: output side")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -549,7 +551,14 @@ def bench_c4(a):
     the request loop.  Saturation throughput first (all submitted at once),
     then Poisson arrivals at 70% of it for p50/p99 latency."""
     from imageworsener_amd import api
-    n = max(a.steps, 8) * 12
+    n = max(1, a.c4_requests)
+    # One HIP hardware queue per stream of the loop: with fewer (HIP's default
+    # is 4), streams share a queue and a flight's long error-diffusion kernels
+    # hold back the flights queued behind them.  Set before the HIP runtime
+    # starts (the library is loaded below).
+    queues = min(32, max(1, a.c4_streams))
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < queues:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(queues)
     log("generating %d serving requests (sides 256..%d, %d processes)" % (n, a.c4_max_side, host_procs()))
     t0 = time.perf_counter()
     reqs = c4_requests(n, a.c4_max_side)
@@ -561,7 +570,7 @@ def bench_c4(a):
     out_px = sum(j["canvas"][0] * j["canvas"][1] for j, _ in reqs)
 
     def serve(rate):
-        srv = api.Server(lib, hbm_budget=32 << 30, streams=8, timeout_s=120.0)
+        srv = api.Server(lib, hbm_budget=32 << 30, streams=a.c4_streams, timeout_s=120.0)
         try:
             t0 = time.perf_counter()
             for job, gap in reqs:
@@ -594,7 +603,8 @@ def bench_c4(a):
             "config": {"workload": "C4 (sides 256..%d): %d requests, RGB 8/16-bit, x0.1..1.5, "
                                    "lanczos3/mitchell, 8-bit FS; request loop with continuous batching"
                                    % (a.c4_max_side, n),
-                       "global_batch": n, "precision": form},
+                       "global_batch": n, "precision": form, "streams": a.c4_streams,
+                       "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)},
             "latency_ms": {"rate_per_s": round(rate, 2), "p50": round(lat[len(lat) // 2], 3),
                            "p99": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3), "max": round(lat[-1], 3)},
             "saturation": {"requests_per_s": round(sat_rate, 2), "p50": round(lat_sat[len(lat_sat) // 2], 3)},

@@ -272,7 +272,11 @@ IW_EXPORT(double) iwamd_batch_flops(struct iwamd_batch *batch);
  * the context's error set); iwamd_server_drain() waits for every request.
  * iwamd_server_request_ms() gives a finished request's submit, start and
  * done times in ms since the server started.  Contexts must outlive the
- * server's use of them (until their request is done). */
+ * server's use of them (until their request is done).  HIP maps streams onto
+ * GPU_MAX_HW_QUEUES hardware queues per process (4 by default): give the
+ * process at least `streams` queues (set the variable before the HIP runtime
+ * starts), or the streams share queues and a batch's long error-diffusion
+ * kernels hold back the batches queued behind them on the same queue. */
 struct iwamd_server;
 IW_EXPORT(struct iwamd_server*) iwamd_server_create(size_t hbm_budget_bytes, int streams);
 IW_EXPORT(int) iwamd_server_submit(struct iwamd_server *server, struct iw_context *ctx);
