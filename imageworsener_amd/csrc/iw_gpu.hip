@@ -1241,7 +1241,7 @@ __global__ void reset_counters_kernel(int *c, int n)
 #define FAST_MAX_KV 256   // input rows of a band (its weights live in LDS)
 #define FAST_G 2          // output groups of 16 columns per tile, at most
 #define FAST_MAX_TI (16 * FAST_G)
-#define FAST_KC 8         // vertical steps whose input loads are issued together
+#define FAST_KC 5         // vertical steps whose input loads are issued together
 #define FAST_RUN 64       // tiles per work item, at most
 #define FAST_WG_PER_CU 4  // workgroups per CU of a persistent launch
 
