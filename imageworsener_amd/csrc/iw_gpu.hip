@@ -1437,28 +1437,39 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 		for (int c = 0; c < IWAMD_MAX_CH; c++) acc[og][c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
 
 	// FAST_RGBA8, rows inside the image, ascending addresses: each load is a
-	// uniform row address plus the lane's 32-bit offset, only the steps the
-	// band has are loaded, and a group's first FAST_KC steps are issued
-	// before the previous group's horizontal products (in flight meanwhile).
+	// uniform row address plus the lane's 32-bit byte offset (the image is
+	// under 2^31 bytes, fast_rgba8_ok), the lane's part of it computed once
+	// per group; only the steps the band has are loaded, and a group's first
+	// FAST_KC steps are issued before the previous group's horizontal
+	// products (in flight meanwhile).  (Bands that reach past an edge take
+	// the clamped loads below: a clamped form of these loads spilled.)
 	bool direct = false;
 	const g_u8 *row0 = nullptr;
 	if constexpr (RGBA8 && DEPTH == 32) {
 		direct = w.p_lo >= 0 && w.p_lo + kv4 <= P.ih && F.dx > 0 && F.dy > 0;
 		row0 = (const g_u8 *)P.in + F.base + (long long)w.p_lo * F.dy;
 	}
-	unsigned int pre[FAST_KC];
-	auto issue = [&](int g, int k0) {
+	const unsigned int dy = (unsigned int)F.dy;
+	auto group_off = [&](int g) {
 		const int xq = 16 * g + lr;
 		const int xcq = xq < 0 ? 0 : (xq >= P.W ? P.W - 1 : xq);
-		const size_t vo = (unsigned int)(xcq * (int)F.dx + lq * (int)F.dy);
+		return (unsigned int)xcq * (unsigned int)F.dx + (unsigned int)lq * dy;
+	};
+	unsigned int pre[FAST_KC];
+	auto issue = [&](unsigned int vo, int k0) {
 #pragma unroll
 		for (int s = 0; s < FAST_KC; s++)
 			if (k0 + 4 * s < kv4)
-				pre[s] = *(const __attribute__((address_space(1))) unsigned int *)(row0 + (long long)(k0 + 4 * s) * F.dy + vo);
+				pre[s] = *(const __attribute__((address_space(1))) unsigned int *)(row0 +
+					(size_t)(vo + (unsigned int)(k0 + 4 * s) * dy));
 		// Keeps the loads here, all in flight together.
 		asm volatile("" ::: "memory");
 	};
-	if (direct) issue(g_first, 0);
+	unsigned int vo_next = 0;  // the lane's offset in the group whose loads are issued
+	if (direct) {
+		vo_next = group_off(g_first);
+		issue(vo_next, 0);
+	}
 
 	for (int g = g_first; g <= g_last; g++) {
 		const int xg = 16 * g;
@@ -1476,8 +1487,9 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 #pragma unroll
 		for (int c = 0; c < IWAMD_MAX_CH; c++) dv[c] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
 		if (direct) {
+			const unsigned int vo = vo_next;
 			for (int k0 = 0; k0 < kv4; k0 += 4 * FAST_KC) {
-				if (k0 > 0) issue(g, k0);
+				if (k0 > 0) issue(vo, k0);
 #pragma unroll
 				for (int s = 0; s < FAST_KC; s++) {
 					if (k0 + 4 * s >= kv4) break;
@@ -1488,7 +1500,10 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 					for (int c = 0; c < IWAMD_MAX_CH; c++) dv[c] = mfma4(v[c], b, dv[c]);
 				}
 			}
-			if (g < g_last) issue(g + 1, 0);
+			if (g < g_last) {
+				vo_next = group_off(g + 1);
+				issue(vo_next, 0);
+			}
 		} else if constexpr (DEPTH == 32 && LEAN) {
 			// FAST_KC steps' words are loaded before the first is used.
 			for (int k0 = 0; k0 < kv4; k0 += 4 * FAST_KC) {
