@@ -616,7 +616,7 @@ def bench_c4(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the path shards by image, "
+        print("bench.py: this library runs on one GPU; --gpus %d is not supported (the path shards by image, "
               "plane or output strip across processes; nothing here runs on more than one GPU)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
