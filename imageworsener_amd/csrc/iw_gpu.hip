@@ -1569,6 +1569,10 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 		// rounding already).  Then one 32-bit store per pixel.
 		const bool pack = ES.pack4;
 		auto clamp = [](double v) { return fmin(fmax(v, 0.0), 1.0); };
+		// Stores: the band's first output row (uniform) plus the lane's 32-bit
+		// byte offset within the band's 16 rows.
+		g_u8 *const orow = (g_u8 *)P.out + (size_t)w.j0 * P.out_bpr;
+		const unsigned int obpr = (unsigned int)P.out_bpr, lrow = (unsigned int)(4 * lq) * obpr;
 #pragma unroll
 		for (int og = 0; og < FAST_G; og++) {
 			if (og >= ng) break;
@@ -1584,7 +1588,7 @@ __device__ __forceinline__ void fast_tile(const DevPlan &P, const FastCin &F, co
 				for (int c = 0; c < 3; c++)
 					word |= (unsigned int)fast_q8(S, 0, clamp((double)acc[og][c][t] * inv)) << (8 * c);
 				if (ii < w.ni && r < rows) {
-					g_u8 *px = (g_u8 *)P.out + (size_t)(w.j0 + r) * P.out_bpr + 4 * (size_t)i;
+					g_u8 *px = orow + (size_t)(lrow + (unsigned int)t * obpr + 4u * (unsigned int)i);
 					if (pack) {
 						*(__attribute__((address_space(1))) unsigned int *)px = word;
 					} else {
