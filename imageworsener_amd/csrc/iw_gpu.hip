@@ -3584,18 +3584,21 @@ void ws_layout(WorkSet &W)
 	for (auto &l : W.fast) l.clear();
 	for (const int4 &r : W.fast_runs)
 		for (int a = 0; a < r.y; a += per) W.fast[r.z].push_back(make_int2(r.x + a, std::min(per, r.y - a)));
-	// Each chosen-form list cut into XCD_COUNT chunks of about equal tile
-	// counts, in list order (a job's bands stay neighbours).
+	// Each chosen-form list cut into XCD_COUNT chunks of about equal cost, in
+	// list order (a job's bands stay neighbours): a tile costs about its
+	// band's input rows plus a fixed part (its horizontal products and
+	// per-pixel work, counted as 32 rows).
 	for (int v = 0; v < FAST_NVAR; v++) {
 		const std::vector<int2> &l = W.fast[v];
+		auto wt = [&](const int2 &it) { return (size_t)it.y * (size_t)(W.fast_tiles[it.x].kv + 32); };
 		size_t total = 0;
-		for (const int2 &it : l) total += (size_t)it.y;
+		for (const int2 &it : l) total += wt(it);
 		int *ch = W.fast_chunks[v];
 		size_t acc = 0;
 		int x = 1;
 		ch[0] = 0;
 		for (size_t k = 0; k < l.size() && x < XCD_COUNT; k++) {
-			acc += (size_t)l[k].y;
+			acc += wt(l[k]);
 			while (x < XCD_COUNT && acc * XCD_COUNT >= total * (size_t)x) ch[x++] = (int)k + 1;
 		}
 		while (x <= XCD_COUNT) ch[x++] = (int)l.size();
