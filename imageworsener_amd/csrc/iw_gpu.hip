@@ -2142,7 +2142,7 @@ __device__ __forceinline__ void ed_walk_seg(const EdCtx &c, Lds &S, unsigned sho
 	const int wrap = 1 - (seg - 1) * nsp, guard = seg * nsp;
 	int r = r0, z = zs - warm;
 	double2 v = c.be[z < 0 ? guard : a];
-#pragma unroll 2
+#pragma unroll 8
 	for (int i = 0; i < warm; i++, z++) {
 		// The next pixel's address and sample, read a step ahead.
 		if (++r == seg) {
